@@ -1,0 +1,110 @@
+/*
+ * semigpu.h -- C ABI of the MI355X-native MNDO SCF engine (libsemigpu.so).
+ *
+ * Plain C types only: ints, doubles, opaque handles.  Every entry point
+ * returns SG_OK (0) or a negative SG_ERR_* code; sg_last_error() describes the
+ * last failure of the calling thread.  Units follow the reference at its
+ * object boundary: atom coordinates in Angstrom, matrices and energies in eV,
+ * packed symmetric matrices as the lower triangle in row order (index
+ * r(r+1)/2 + c, mymemory.cpp:363-371), eigenvectors column-major
+ * (scfcalculation.cpp:175-179).
+ *
+ * A batch holds any number of molecules; a single molecule is a batch of one.
+ * Each entry point below names the reference member it replaces
+ * (LuisAlfredoNu/semiGPU, src/common/...).  One HIP stream per context;
+ * a context is not thread-safe, separate contexts are.
+ */
+#ifndef SEMIGPU_H
+#define SEMIGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SG_OK 0
+#define SG_ERR_INVALID (-1)           /* bad argument / handle */
+#define SG_ERR_UNSUPPORTED (-2)       /* element outside H, C, N, O */
+#define SG_ERR_HIP (-3)               /* HIP runtime failure */
+#define SG_ERR_ORDER (-4)             /* stage called before its inputs exist */
+#define SG_ERR_NO_DEVICE (-5)         /* no usable GPU */
+
+enum sg_matrix { SG_HCORE = 0, SG_FOCK = 1, SG_DENSITY = 2 };
+
+typedef struct sg_context sg_context;
+typedef struct sg_batch sg_batch;
+
+typedef struct {
+  double e_el;    /* SCFCalculation::finalEnergy (scfcalculation.h:22), eV */
+  double e_cc;    /* CoreCoreRepulsion::ComputeRepulsion, eV */
+  double e_tot;   /* e_el + e_cc (semiGPU.cc:95) */
+  int steps;      /* SCF iterations performed */
+  int converged;  /* SCFCalculation::energyConverge (scfcalculation.h:24) */
+  int status;     /* 0, or >0 when the eigensolver failed (dsyev info > 0) */
+} sg_scf_result;
+
+const char* sg_last_error(void);
+int sg_version(void);
+
+/* Context = one GPU + one stream. */
+int sg_ctx_create(int device, sg_context** out);
+int sg_ctx_destroy(sg_context* ctx);
+
+/* Molecules: atoms of molecule m are atom_offsets[m] .. atom_offsets[m+1]-1 of
+ * Z[] / xyz_ang[3*i..3*i+2].  Replaces ReadXYZFile + ListAtomicOrbitals::SetOrbitals
+ * (readxyzfile.cpp:27-55, atomicOrbitals.cpp:76-119) and the allocations of
+ * TwoCenterIntegral::Alloc4AllTwoCenterIntegral (twocenterintegral.cpp:55-107),
+ * BaseMatrix::Alloc4Matrix (basematrix.cpp:23-27) and
+ * SCFCalculation::AllocSCFData (scfcalculation.cpp:28-46). */
+int sg_batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, const int* Z,
+                    const double* xyz_ang, sg_batch** out);
+int sg_batch_destroy(sg_batch* b);
+int sg_batch_molecule_info(const sg_batch* b, int mol, int* n_atoms, int* n_ao, int* n_occ);
+
+/* TwoCenterIntegral::ComputeAllTwoCenterIntegral + To_device
+ * (twocenterintegral.cpp:23-53, 229-279). */
+int sg_two_center(sg_batch* b);
+/* Hcore::ComputeMatrix (hcore.cpp:104-124 / basematrix.cpp:39-55). */
+int sg_hcore(sg_batch* b);
+/* CoreCoreRepulsion::ComputeRepulsion (corecorerepulsion.cpp:12-41); e_cc[n_mol]. */
+int sg_core_core(sg_batch* b, double* e_cc);
+/* DensityMatrix guess (GuessDensityMatrixSwitch(true) + ComputeMatrix,
+ * densitymatrix.cpp:26-36). */
+int sg_guess_density(sg_batch* b);
+/* FockMatrix::ComputeMatrix (fockmatrix.cpp:32-53) for every molecule. */
+int sg_fock(sg_batch* b);
+/* SCFCalculation::GetEigenValVecOfFmatrix, i.e. dsyev_('V','L') on F
+ * (scfcalculation.cpp:169-203). */
+int sg_eigh(sg_batch* b);
+/* DensityMatrix::ComputeMatrix from the eigenvectors (densitymatrix.cpp:38-45). */
+int sg_density(sg_batch* b);
+/* ElectronicEnergy::ComputeEnergy (electronicenergy.cpp:12-36); e_el[n_mol]. */
+int sg_energy(sg_batch* b, double* e_el);
+/* SCFCalculation::ComputeSCF (scfcalculation.cpp:66-146) plus the core-core
+ * energy, for every molecule of the batch; runs sg_two_center / sg_hcore first
+ * when they have not run.  threshold_ev = 5e-3 and max_steps = 100 reproduce
+ * the reference.  results[n_mol]. */
+int sg_scf_run(sg_batch* b, double threshold_ev, int max_steps, sg_scf_result* results);
+
+/* Host copies for the reference's host-side views. */
+long sg_pair_block_count(const sg_batch* b, int mol);
+/* double**** all2CenterIntegral contents in the reference jagged order
+ * [i][j<=i][k][l] (twocenterintegral.cpp:55-107); read by GetValueFromArray. */
+int sg_get_pair_blocks(const sg_batch* b, int mol, double* out);
+/* BaseMatrix::matrixHold_ of Hcore / FockMatrix / DensityMatrix (packed). */
+int sg_get_packed(const sg_batch* b, int mol, int which, double* out);
+int sg_set_packed(sg_batch* b, int mol, int which, const double* in);
+/* SCFCalculation::eigenVal / eigenVec (eigenvectors column-major, n_ao*n_ao). */
+int sg_get_eigen(const sg_batch* b, int mol, double* eigval, double* eigvec);
+
+/* Per-stage GPU time (HIP events on the context stream), milliseconds summed
+ * since the last reset; stage order: two_center, hcore, guess, fock, eigh,
+ * density, energy, control.  counts[] gets the number of launches timed. */
+#define SG_N_STAGES 8
+int sg_timing_enable(sg_context* ctx, int on);
+int sg_timing_read(sg_context* ctx, double* ms, int* counts);
+int sg_timing_reset(sg_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEMIGPU_H */

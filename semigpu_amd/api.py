@@ -1,0 +1,235 @@
+"""Python binding of the C ABI (include/semigpu.h) over the in-tree libsemigpu.so.
+
+There is no CPU fallback: importing works anywhere, but creating an
+:class:`Engine` needs the built library and a HIP device, and raises
+otherwise.
+
+    eng = Engine()                         # GPU 0, one stream
+    b = eng.batch([geometry.pyridine()])   # list of molecules (Z, x, y, z) in Angstrom
+    res = b.scf(threshold=1e-6)            # SCFCalculation::ComputeSCF for every molecule
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsemigpu.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include", "semigpu.h")
+
+SG_HCORE, SG_FOCK, SG_DENSITY = 0, 1, 2
+STAGES = ("two_center", "hcore", "guess", "fock", "eigh", "density", "energy", "control")
+
+# Every function declared in include/semigpu.h (checked by tests/test_abi.py).
+EXPORTS = (
+    "sg_last_error", "sg_version", "sg_ctx_create", "sg_ctx_destroy", "sg_batch_create",
+    "sg_batch_destroy", "sg_batch_molecule_info", "sg_two_center", "sg_hcore", "sg_core_core",
+    "sg_guess_density", "sg_fock", "sg_eigh", "sg_density", "sg_energy", "sg_scf_run",
+    "sg_pair_block_count", "sg_get_pair_blocks", "sg_get_packed", "sg_set_packed",
+    "sg_get_eigen", "sg_timing_enable", "sg_timing_read", "sg_timing_reset",
+)
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+
+class ScfResult(ctypes.Structure):
+    _fields_ = [("e_el", ctypes.c_double), ("e_cc", ctypes.c_double), ("e_tot", ctypes.c_double),
+                ("steps", ctypes.c_int), ("converged", ctypes.c_int), ("status", ctypes.c_int)]
+
+
+class SemiGpuError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libsemigpu.so and declares its signatures; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SemiGpuError("%s is not built: run `python -m semigpu_amd.build` "
+                           "(or __graft_entry__.build())" % path)
+    L = ctypes.CDLL(path)
+    L.sg_last_error.restype = ctypes.c_char_p
+    L.sg_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+    L.sg_ctx_destroy.argtypes = [_vp]
+    L.sg_batch_create.argtypes = [_vp, ctypes.c_int, _ip, _ip, _dp, ctypes.POINTER(_vp)]
+    L.sg_batch_destroy.argtypes = [_vp]
+    L.sg_batch_molecule_info.argtypes = [_vp, ctypes.c_int, _ip, _ip, _ip]
+    for f in ("sg_two_center", "sg_hcore", "sg_guess_density", "sg_fock", "sg_eigh", "sg_density"):
+        getattr(L, f).argtypes = [_vp]
+    L.sg_core_core.argtypes = [_vp, _dp]
+    L.sg_energy.argtypes = [_vp, _dp]
+    L.sg_scf_run.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ScfResult)]
+    L.sg_pair_block_count.argtypes = [_vp, ctypes.c_int]
+    L.sg_pair_block_count.restype = ctypes.c_long
+    L.sg_get_pair_blocks.argtypes = [_vp, ctypes.c_int, _dp]
+    L.sg_get_packed.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp]
+    L.sg_set_packed.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp]
+    L.sg_get_eigen.argtypes = [_vp, ctypes.c_int, _dp, _dp]
+    L.sg_timing_enable.argtypes = [_vp, ctypes.c_int]
+    L.sg_timing_read.argtypes = [_vp, _dp, _ip]
+    L.sg_timing_reset.argtypes = [_vp]
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise SemiGpuError("semigpu error %d: %s" % (rc, _lib.sg_last_error().decode()))
+
+
+def _d(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a: np.ndarray):
+    return a.ctypes.data_as(_ip)
+
+
+class Engine:
+    """One HIP device + one stream (sg_context)."""
+
+    def __init__(self, device: int = 0):
+        L = load_library()
+        h = _vp()
+        _check(L.sg_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+        self.L = L
+
+    def close(self):
+        if self.h:
+            self.L.sg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def batch(self, molecules: Sequence[Sequence[Sequence[float]]]) -> "Batch":
+        return Batch(self, molecules)
+
+    def timing(self, on: bool = True):
+        _check(self.L.sg_timing_enable(self.h, 1 if on else 0))
+
+    def timing_read(self):
+        ms = np.zeros(8)
+        cnt = np.zeros(8, dtype=np.int32)
+        _check(self.L.sg_timing_read(self.h, _d(ms), _i(cnt)))
+        return {s: (float(ms[k]), int(cnt[k])) for k, s in enumerate(STAGES)}
+
+    def timing_reset(self):
+        _check(self.L.sg_timing_reset(self.h))
+
+
+class Batch:
+    """A batch of molecules resident on the device (sg_batch)."""
+
+    def __init__(self, engine: Engine, molecules):
+        self.engine = engine
+        self.L = engine.L
+        offs = [0]
+        Z: List[int] = []
+        xyz: List[float] = []
+        for mol in molecules:
+            for a in mol:
+                Z.append(int(a[0]))
+                xyz.extend(float(c) for c in a[1:4])
+            offs.append(len(Z))
+        self.n_mol = len(molecules)
+        self.offsets = np.asarray(offs, dtype=np.int32)
+        self.Z = np.asarray(Z, dtype=np.int32)
+        self.xyz = np.asarray(xyz, dtype=np.float64)
+        h = _vp()
+        _check(self.L.sg_batch_create(engine.h, self.n_mol, _i(self.offsets), _i(self.Z),
+                                      _d(self.xyz), ctypes.byref(h)))
+        self.h = h
+        self.info = []
+        for m in range(self.n_mol):
+            na, nao, nocc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            _check(self.L.sg_batch_molecule_info(h, m, ctypes.byref(na), ctypes.byref(nao),
+                                                 ctypes.byref(nocc)))
+            self.info.append((na.value, nao.value, nocc.value))
+
+    def close(self):
+        if self.h:
+            self.L.sg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def n_ao(self, m: int) -> int:
+        return self.info[m][1]
+
+    # stages -------------------------------------------------------------
+    def two_center(self):
+        _check(self.L.sg_two_center(self.h))
+
+    def hcore(self):
+        _check(self.L.sg_hcore(self.h))
+
+    def core_core(self) -> np.ndarray:
+        out = np.zeros(self.n_mol)
+        _check(self.L.sg_core_core(self.h, _d(out)))
+        return out
+
+    def guess_density(self):
+        _check(self.L.sg_guess_density(self.h))
+
+    def fock(self):
+        _check(self.L.sg_fock(self.h))
+
+    def eigh(self):
+        _check(self.L.sg_eigh(self.h))
+
+    def density(self):
+        _check(self.L.sg_density(self.h))
+
+    def energy(self) -> np.ndarray:
+        out = np.zeros(self.n_mol)
+        _check(self.L.sg_energy(self.h, _d(out)))
+        return out
+
+    def scf(self, threshold: float = 5e-3, max_steps: int = 100):
+        res = (ScfResult * self.n_mol)()
+        _check(self.L.sg_scf_run(self.h, threshold, max_steps, res))
+        return [{"e_el": r.e_el, "e_cc": r.e_cc, "e_tot": r.e_tot, "steps": r.steps,
+                 "converged": bool(r.converged), "status": r.status} for r in res]
+
+    # host views ---------------------------------------------------------
+    def pair_blocks(self, m: int) -> np.ndarray:
+        n = self.L.sg_pair_block_count(self.h, m)
+        out = np.zeros(n)
+        _check(self.L.sg_get_pair_blocks(self.h, m, _d(out)))
+        return out
+
+    def packed(self, m: int, which: int) -> np.ndarray:
+        n = self.n_ao(m)
+        out = np.zeros(n * (n + 1) // 2)
+        _check(self.L.sg_get_packed(self.h, m, which, _d(out)))
+        return out
+
+    def set_packed(self, m: int, which: int, values: np.ndarray):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        _check(self.L.sg_set_packed(self.h, m, which, _d(v)))
+
+    def eigen(self, m: int):
+        n = self.n_ao(m)
+        w = np.zeros(n)
+        c = np.zeros(n * n)
+        _check(self.L.sg_get_eigen(self.h, m, _d(w), _d(c)))
+        return w, c.reshape(n, n).T.copy()  # columns are eigenvectors

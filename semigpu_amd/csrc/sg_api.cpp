@@ -1,0 +1,588 @@
+// sg_api.cpp -- host implementation of the C ABI in include/semigpu.h.
+//
+// A sg_batch owns one device allocation that holds every molecule's atoms,
+// pair blocks, matrices and SCF state (layout in sg_kernels.h).  The SCF loop
+// runs on the device; the host only reads back the count of molecules that are
+// still iterating after each step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/semigpu.h"
+#include "sg_kernels.h"
+
+using namespace sg;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t _e = (expr);                                                             \
+    if (_e != hipSuccess)                                                               \
+      return fail(SG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));        \
+  } while (0)
+
+struct sg_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool timing = false;
+  double ms[SG_N_STAGES] = {0};
+  int counts[SG_N_STAGES] = {0};
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+};
+
+struct sg_batch {
+  sg_context* ctx = nullptr;
+  BatchDev d;
+  void* dev_mem = nullptr;
+  std::vector<MolDesc> mols;
+  std::vector<AtomDesc> atoms;
+  std::vector<int64_t> blk_count;  // doubles of pair blocks per molecule
+  bool have_ints = false, have_hcore = false, have_ecc = false;
+  std::vector<double> e_cc;
+};
+
+enum { ST_TWO_CENTER = 0, ST_HCORE, ST_GUESS, ST_FOCK, ST_EIGH, ST_DENSITY, ST_ENERGY, ST_CONTROL };
+
+namespace {
+
+struct StageTimer {
+  sg_context* c;
+  int stage;
+  hipEvent_t a = nullptr, b = nullptr;
+  StageTimer(sg_context* ctx, int st) : c(ctx), stage(st) {
+    if (c->timing) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~StageTimer() {
+    if (c->timing) {
+      (void)hipEventRecord(b, c->stream);
+      c->pending.push_back({stage, {a, b}});
+    }
+  }
+};
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return SG_OK;
+}
+
+bool supported(int z) { return z == 1 || z == 6 || z == 7 || z == 8; }
+
+}  // namespace
+
+extern "C" {
+
+const char* sg_last_error(void) { return g_err.c_str(); }
+int sg_version(void) { return 100; }
+
+int sg_ctx_create(int device, sg_context** out) {
+  if (!out) return fail(SG_ERR_INVALID, "null output handle");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0)
+    return fail(SG_ERR_NO_DEVICE, "no HIP device " + std::to_string(device));
+  HIP_TRY(hipSetDevice(device));
+  auto* c = new sg_context();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(SG_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return SG_OK;
+}
+
+int sg_ctx_destroy(sg_context* c) {
+  if (!c) return SG_OK;
+  (void)hipSetDevice(c->device);
+  for (auto& p : c->pending) {
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return SG_OK;
+}
+
+int sg_timing_enable(sg_context* c, int on) {
+  if (!c) return fail(SG_ERR_INVALID, "null context");
+  c->timing = on != 0;
+  return SG_OK;
+}
+
+int sg_timing_read(sg_context* c, double* ms, int* counts) {
+  if (!c) return fail(SG_ERR_INVALID, "null context");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (auto& p : c->pending) {
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, p.second.first, p.second.second);
+    c->ms[p.first] += t;
+    c->counts[p.first] += 1;
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  c->pending.clear();
+  for (int i = 0; i < SG_N_STAGES; ++i) {
+    if (ms) ms[i] = c->ms[i];
+    if (counts) counts[i] = c->counts[i];
+  }
+  return SG_OK;
+}
+
+int sg_timing_reset(sg_context* c) {
+  if (!c) return fail(SG_ERR_INVALID, "null context");
+  int r = sg_timing_read(c, nullptr, nullptr);
+  for (int i = 0; i < SG_N_STAGES; ++i) {
+    c->ms[i] = 0;
+    c->counts[i] = 0;
+  }
+  return r;
+}
+
+int sg_batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, const int* Z,
+                    const double* xyz_ang, sg_batch** out) {
+  if (!ctx || !out || n_mol <= 0 || !atom_offsets || !Z || !xyz_ang)
+    return fail(SG_ERR_INVALID, "sg_batch_create: bad arguments");
+  if (atom_offsets[0] != 0) return fail(SG_ERR_INVALID, "atom_offsets[0] must be 0");
+  for (int m = 0; m < n_mol; ++m)
+    if (atom_offsets[m + 1] <= atom_offsets[m])
+      return fail(SG_ERR_INVALID, "molecule " + std::to_string(m) + " has no atoms");
+  const int n_atoms = atom_offsets[n_mol];
+  for (int i = 0; i < n_atoms; ++i) {
+    if (!supported(Z[i]))
+      return fail(SG_ERR_UNSUPPORTED, "atom " + std::to_string(i) + ": element Z=" +
+                                          std::to_string(Z[i]) + " has no MNDO parameters here");
+    for (int k = 0; k < 3; ++k)
+      if (!std::isfinite(xyz_ang[3 * i + k]))
+        return fail(SG_ERR_INVALID, "atom " + std::to_string(i) + " has a non-finite coordinate");
+  }
+  auto* b = new sg_batch();
+  b->ctx = ctx;
+  b->mols.resize(n_mol);
+  b->atoms.resize(n_atoms);
+  b->blk_count.resize(n_mol);
+  std::vector<int2> pxx, phx, phh, pall, diag;
+  int64_t mat0 = 0, vec0 = 0, blk0 = 0;
+  int max_n = 0;
+  for (int m = 0; m < n_mol; ++m) {
+    MolDesc& M = b->mols[m];
+    M.atom0 = atom_offsets[m];
+    M.n_atoms = atom_offsets[m + 1] - atom_offsets[m];
+    int ao = 0, ne = 0, hv = 0;
+    int64_t row = 0;
+    for (int a = 0; a < M.n_atoms; ++a) {
+      const int gi = M.atom0 + a;
+      AtomDesc& A = b->atoms[gi];
+      A.zn = Z[gi];
+      // AtomicOrbital::SetCoordinates: Angstrom -> bohr (atomicOrbitals.cpp:40-44)
+      A.x = xyz_ang[3 * gi + 0] * 1.8897259886;
+      A.y = xyz_ang[3 * gi + 1] * 1.8897259886;
+      A.z = xyz_ang[3 * gi + 2] * 1.8897259886;
+      A.mol = m;
+      A.local = a;
+      A.ao0 = ao;
+      A.hv_before = hv;
+      A.pad = 0;
+      A.row0 = row;
+      for (int j = 0; j <= a; ++j) row += block_size(A.zn, Z[M.atom0 + j]);
+      ao += n_ao_of(A.zn);
+      ne += core_charge(A.zn);
+      if (A.zn > 1) ++hv;
+      for (int j = 0; j < a; ++j) {
+        const int gj = M.atom0 + j;
+        int2 pr = make_int2(gi, gj);
+        pall.push_back(pr);
+        const int bs = block_size(A.zn, Z[gj]);
+        (bs == 100 ? pxx : bs == 10 ? phx : phh).push_back(pr);
+      }
+      for (int s = 0; s < (A.zn > 1 ? 10 : 1); ++s) diag.push_back(make_int2(gi, s));
+    }
+    M.n_ao = ao;
+    M.n_occ = ne / 2;
+    M.mat0 = mat0;
+    M.vec0 = vec0;
+    M.blk0 = blk0;
+    b->blk_count[m] = row;
+    mat0 += (int64_t)ao * ao;
+    vec0 += ao;
+    blk0 += row;
+    max_n = std::max(max_n, ao);
+  }
+  if (max_n > 1024) {
+    delete b;
+    return fail(SG_ERR_INVALID, "molecules above 1024 basis functions are not supported yet");
+  }
+  BatchDev& d = b->d;
+  d.n_mol = n_mol;
+  d.n_atoms = n_atoms;
+  d.n_mat = mat0;
+  d.n_vec = vec0;
+  d.n_blocks = blk0;
+  d.max_n = max_n;
+  d.n_xx = (int)pxx.size();
+  d.n_hx = (int)phx.size();
+  d.n_hh = (int)phh.size();
+  d.n_pairs_all = (int)pall.size();
+  d.n_diag_tasks = (int)diag.size();
+
+  // one allocation, 256-byte aligned sub-buffers
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) & ~size_t(255);
+    return o;
+  };
+  const size_t o_mols = take(sizeof(MolDesc) * n_mol), o_atoms = take(sizeof(AtomDesc) * n_atoms);
+  const size_t o_xx = take(sizeof(int2) * pxx.size()), o_hx = take(sizeof(int2) * phx.size()),
+               o_hh = take(sizeof(int2) * phh.size()), o_all = take(sizeof(int2) * pall.size()),
+               o_diag = take(sizeof(int2) * diag.size());
+  const size_t o_blk = take(sizeof(double) * blk0);
+  const size_t o_H = take(sizeof(double) * mat0), o_F = take(sizeof(double) * mat0),
+               o_P = take(sizeof(double) * mat0), o_C = take(sizeof(double) * mat0),
+               o_W = take(sizeof(double) * 2 * mat0);
+  const size_t o_eig = take(sizeof(double) * vec0), o_tau = take(sizeof(double) * vec0);
+  const size_t o_ecc = take(sizeof(double) * n_mol), o_eel = take(sizeof(double) * n_mol),
+               o_enew = take(sizeof(double) * n_mol);
+  const size_t o_steps = take(sizeof(int) * n_mol), o_act = take(sizeof(int) * n_mol),
+               o_conv = take(sizeof(int) * n_mol), o_stat = take(sizeof(int) * n_mol),
+               o_nact = take(sizeof(int) * 64);
+  (void)hipSetDevice(ctx->device);
+  hipError_t e = hipMalloc(&b->dev_mem, off);
+  if (e != hipSuccess) {
+    delete b;
+    return fail(SG_ERR_HIP, std::string("hipMalloc(") + std::to_string(off) + "): " +
+                                hipGetErrorString(e));
+  }
+  char* base = (char*)b->dev_mem;
+  d.mols = (MolDesc*)(base + o_mols);
+  d.atoms = (AtomDesc*)(base + o_atoms);
+  d.pairs_xx = (int2*)(base + o_xx);
+  d.pairs_hx = (int2*)(base + o_hx);
+  d.pairs_hh = (int2*)(base + o_hh);
+  d.pairs_all = (int2*)(base + o_all);
+  d.diag_tasks = (int2*)(base + o_diag);
+  d.blocks = (double*)(base + o_blk);
+  d.H = (double*)(base + o_H);
+  d.F = (double*)(base + o_F);
+  d.P = (double*)(base + o_P);
+  d.C = (double*)(base + o_C);
+  d.W = (double*)(base + o_W);
+  d.eig = (double*)(base + o_eig);
+  d.tau = (double*)(base + o_tau);
+  d.e_cc = (double*)(base + o_ecc);
+  d.e_el = (double*)(base + o_eel);
+  d.e_new = (double*)(base + o_enew);
+  d.steps = (int*)(base + o_steps);
+  d.active = (int*)(base + o_act);
+  d.converged = (int*)(base + o_conv);
+  d.status = (int*)(base + o_stat);
+  d.n_active = (int*)(base + o_nact);
+  hipStream_t s = ctx->stream;
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
+  };
+  e = up(d.mols, b->mols.data(), sizeof(MolDesc) * n_mol);
+  if (e == hipSuccess) e = up(d.atoms, b->atoms.data(), sizeof(AtomDesc) * n_atoms);
+  if (e == hipSuccess) e = up(d.pairs_xx, pxx.data(), sizeof(int2) * pxx.size());
+  if (e == hipSuccess) e = up(d.pairs_hx, phx.data(), sizeof(int2) * phx.size());
+  if (e == hipSuccess) e = up(d.pairs_hh, phh.data(), sizeof(int2) * phh.size());
+  if (e == hipSuccess) e = up(d.pairs_all, pall.data(), sizeof(int2) * pall.size());
+  if (e == hipSuccess) e = up(d.diag_tasks, diag.data(), sizeof(int2) * diag.size());
+  for (double* z : {d.H, d.F, d.P, d.C})
+    if (e == hipSuccess) e = hipMemsetAsync(z, 0, sizeof(double) * mat0, s);
+  if (e == hipSuccess) e = hipMemsetAsync(d.status, 0, sizeof(int) * n_mol, s);
+  if (e == hipSuccess) e = hipMemsetAsync(d.active, 0, sizeof(int) * n_mol, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(b->dev_mem);
+    delete b;
+    return fail(SG_ERR_HIP, std::string("batch upload: ") + hipGetErrorString(e));
+  }
+  b->e_cc.assign(n_mol, 0.0);
+  *out = b;
+  return SG_OK;
+}
+
+int sg_batch_destroy(sg_batch* b) {
+  if (!b) return SG_OK;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  if (b->dev_mem) (void)hipFree(b->dev_mem);
+  delete b;
+  return SG_OK;
+}
+
+int sg_batch_molecule_info(const sg_batch* b, int mol, int* n_atoms, int* n_ao, int* n_occ) {
+  if (!b || mol < 0 || mol >= b->d.n_mol) return fail(SG_ERR_INVALID, "bad molecule index");
+  if (n_atoms) *n_atoms = b->mols[mol].n_atoms;
+  if (n_ao) *n_ao = b->mols[mol].n_ao;
+  if (n_occ) *n_occ = b->mols[mol].n_occ;
+  return SG_OK;
+}
+
+int sg_two_center(sg_batch* b) {
+  if (!b) return fail(SG_ERR_INVALID, "null batch");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  {
+    StageTimer t(b->ctx, ST_TWO_CENTER);
+    launch_pair_integrals(b->d, b->ctx->stream);
+  }
+  int r = check_launch("two-center integrals");
+  if (r) return r;
+  b->have_ints = true;
+  return SG_OK;
+}
+
+int sg_hcore(sg_batch* b) {
+  if (!b) return fail(SG_ERR_INVALID, "null batch");
+  if (!b->have_ints) return fail(SG_ERR_ORDER, "sg_hcore needs sg_two_center first");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  {
+    StageTimer t(b->ctx, ST_HCORE);
+    launch_hcore(b->d, b->ctx->stream);
+  }
+  int r = check_launch("hcore");
+  if (r) return r;
+  b->have_hcore = true;
+  return SG_OK;
+}
+
+int sg_core_core(sg_batch* b, double* e_cc) {
+  if (!b) return fail(SG_ERR_INVALID, "null batch");
+  if (!b->have_ints) return fail(SG_ERR_ORDER, "sg_core_core needs sg_two_center first");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  {
+    StageTimer t(b->ctx, ST_HCORE);
+    launch_core_core(b->d, b->ctx->stream);
+  }
+  int r = check_launch("core-core");
+  if (r) return r;
+  HIP_TRY(hipMemcpyAsync(b->e_cc.data(), b->d.e_cc, sizeof(double) * b->d.n_mol,
+                         hipMemcpyDeviceToHost, b->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  b->have_ecc = true;
+  if (e_cc) memcpy(e_cc, b->e_cc.data(), sizeof(double) * b->d.n_mol);
+  return SG_OK;
+}
+
+int sg_guess_density(sg_batch* b) {
+  if (!b) return fail(SG_ERR_INVALID, "null batch");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  {
+    StageTimer t(b->ctx, ST_GUESS);
+    launch_guess_density(b->d, b->ctx->stream);
+  }
+  return check_launch("guess density");
+}
+
+int sg_fock(sg_batch* b) {
+  if (!b) return fail(SG_ERR_INVALID, "null batch");
+  if (!b->have_hcore) return fail(SG_ERR_ORDER, "sg_fock needs sg_hcore first");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  {
+    StageTimer t(b->ctx, ST_FOCK);
+    launch_fock(b->d, b->ctx->stream, true);
+  }
+  return check_launch("fock");
+}
+
+int sg_eigh(sg_batch* b) {
+  if (!b) return fail(SG_ERR_INVALID, "null batch");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  HIP_TRY(hipMemsetAsync(b->d.status, 0, sizeof(int) * b->d.n_mol, b->ctx->stream));
+  {
+    StageTimer t(b->ctx, ST_EIGH);
+    launch_eigh(b->d, b->ctx->stream, true);
+  }
+  return check_launch("eigh");
+}
+
+int sg_density(sg_batch* b) {
+  if (!b) return fail(SG_ERR_INVALID, "null batch");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  {
+    StageTimer t(b->ctx, ST_DENSITY);
+    launch_density(b->d, b->ctx->stream, true);
+  }
+  return check_launch("density");
+}
+
+int sg_energy(sg_batch* b, double* e_el) {
+  if (!b) return fail(SG_ERR_INVALID, "null batch");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  {
+    StageTimer t(b->ctx, ST_ENERGY);
+    launch_energy(b->d, b->ctx->stream, b->d.e_new, true);
+  }
+  int r = check_launch("energy");
+  if (r) return r;
+  if (e_el) {
+    HIP_TRY(hipMemcpyAsync(e_el, b->d.e_new, sizeof(double) * b->d.n_mol, hipMemcpyDeviceToHost,
+                           b->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  }
+  return SG_OK;
+}
+
+int sg_scf_run(sg_batch* b, double threshold, int max_steps, sg_scf_result* res) {
+  if (!b || max_steps < 0 || !(threshold >= 0.0)) return fail(SG_ERR_INVALID, "sg_scf_run: bad arguments");
+  int r;
+  if (!b->have_ints && (r = sg_two_center(b))) return r;
+  if (!b->have_hcore && (r = sg_hcore(b))) return r;
+  if (!b->have_ecc && (r = sg_core_core(b, nullptr))) return r;
+  sg_context* c = b->ctx;
+  hipStream_t s = c->stream;
+  HIP_TRY(hipSetDevice(c->device));
+  // InitialGuess + E0 (scfcalculation.cpp:66-72, 148-165)
+  {
+    StageTimer t(c, ST_GUESS);
+    launch_guess_density(b->d, s);
+  }
+  {
+    StageTimer t(c, ST_FOCK);
+    launch_fock(b->d, s, true);
+  }
+  {
+    StageTimer t(c, ST_ENERGY);
+    launch_energy(b->d, s, b->d.e_new, true);
+  }
+  {
+    StageTimer t(c, ST_CONTROL);
+    launch_scf_init(b->d, s, max_steps);
+  }
+  if ((r = check_launch("scf init"))) return r;
+  int n_active = max_steps > 0 ? b->d.n_mol : 0;
+  int host_active = 0;
+  for (int it = 0; it < max_steps && n_active > 0; ++it) {
+    {
+      StageTimer t(c, ST_EIGH);
+      launch_eigh(b->d, s, false);
+    }
+    {
+      StageTimer t(c, ST_DENSITY);
+      launch_density(b->d, s, false);
+    }
+    {
+      StageTimer t(c, ST_FOCK);
+      launch_fock(b->d, s, false);
+    }
+    {
+      StageTimer t(c, ST_ENERGY);
+      launch_energy(b->d, s, b->d.e_new, false);
+    }
+    {
+      StageTimer t(c, ST_CONTROL);
+      HIP_TRY(hipMemsetAsync(b->d.n_active, 0, sizeof(int), s));
+      launch_scf_update(b->d, s, threshold, max_steps);
+    }
+    if ((r = check_launch("scf step"))) return r;
+    HIP_TRY(hipMemcpyAsync(&host_active, b->d.n_active, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    n_active = host_active;
+  }
+  if (res) {
+    const int n = b->d.n_mol;
+    std::vector<double> e(n);
+    std::vector<int> st(n), cv(n), ss(n);
+    HIP_TRY(hipMemcpyAsync(e.data(), b->d.e_el, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(st.data(), b->d.steps, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(cv.data(), b->d.converged, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(ss.data(), b->d.status, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int m = 0; m < n; ++m) {
+      res[m].e_el = e[m];
+      res[m].e_cc = b->e_cc[m];
+      res[m].e_tot = e[m] + b->e_cc[m];
+      res[m].steps = st[m];
+      res[m].converged = cv[m];
+      res[m].status = ss[m];
+    }
+  }
+  return SG_OK;
+}
+
+long sg_pair_block_count(const sg_batch* b, int mol) {
+  if (!b || mol < 0 || mol >= b->d.n_mol) return -1;
+  return (long)b->blk_count[mol];
+}
+
+int sg_get_pair_blocks(const sg_batch* b, int mol, double* out) {
+  if (!b || !out || mol < 0 || mol >= b->d.n_mol) return fail(SG_ERR_INVALID, "bad arguments");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  HIP_TRY(hipMemcpyAsync(out, b->d.blocks + b->mols[mol].blk0, sizeof(double) * b->blk_count[mol],
+                         hipMemcpyDeviceToHost, b->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  return SG_OK;
+}
+
+static double* matrix_ptr(const sg_batch* b, int which) {
+  return which == SG_HCORE ? b->d.H : which == SG_FOCK ? b->d.F : which == SG_DENSITY ? b->d.P : nullptr;
+}
+
+int sg_get_packed(const sg_batch* b, int mol, int which, double* out) {
+  if (!b || !out || mol < 0 || mol >= b->d.n_mol || !matrix_ptr(b, which))
+    return fail(SG_ERR_INVALID, "bad arguments");
+  const MolDesc& M = b->mols[mol];
+  const int n = M.n_ao;
+  std::vector<double> full((size_t)n * n);
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  HIP_TRY(hipMemcpyAsync(full.data(), matrix_ptr(b, which) + M.mat0, sizeof(double) * full.size(),
+                         hipMemcpyDeviceToHost, b->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j <= i; ++j) out[(size_t)i * (i + 1) / 2 + j] = full[(size_t)i * n + j];
+  return SG_OK;
+}
+
+int sg_set_packed(sg_batch* b, int mol, int which, const double* in) {
+  if (!b || !in || mol < 0 || mol >= b->d.n_mol || !matrix_ptr(b, which))
+    return fail(SG_ERR_INVALID, "bad arguments");
+  const MolDesc& M = b->mols[mol];
+  const int n = M.n_ao;
+  std::vector<double> full((size_t)n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j <= i; ++j) {
+      const double v = in[(size_t)i * (i + 1) / 2 + j];
+      full[(size_t)i * n + j] = v;
+      full[(size_t)j * n + i] = v;
+    }
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  HIP_TRY(hipMemcpyAsync(matrix_ptr(b, which) + M.mat0, full.data(), sizeof(double) * full.size(),
+                         hipMemcpyHostToDevice, b->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  if (which == SG_HCORE) b->have_hcore = true;
+  return SG_OK;
+}
+
+int sg_get_eigen(const sg_batch* b, int mol, double* eigval, double* eigvec) {
+  if (!b || mol < 0 || mol >= b->d.n_mol) return fail(SG_ERR_INVALID, "bad arguments");
+  const MolDesc& M = b->mols[mol];
+  const size_t n = M.n_ao;
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  if (eigval)
+    HIP_TRY(hipMemcpyAsync(eigval, b->d.eig + M.vec0, sizeof(double) * n, hipMemcpyDeviceToHost,
+                           b->ctx->stream));
+  if (eigvec)
+    HIP_TRY(hipMemcpyAsync(eigvec, b->d.C + M.mat0, sizeof(double) * n * n, hipMemcpyDeviceToHost,
+                           b->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  return SG_OK;
+}
+
+}  // extern "C"

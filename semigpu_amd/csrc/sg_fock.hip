@@ -1,0 +1,286 @@
+// sg_fock.hip -- Fock build, guess density, electronic energy and SCF control.
+//
+// Fock build (FockMatrix::ComputeMatrix, fockmatrix.cpp:20-53 with
+// TwoElectronMatrix::ComputeGMatrix_UV, twoelectronmatrix.cpp:14-35).  The
+// reference sums P_ij [(uv|ij) - 1/2 (ui|vj)] over all N^2 AO pairs for every
+// element; every term with u,v or i,j on different atoms is zero
+// (GetValueFromArray returns 0, twocenterintegral.cpp:383-385), so the same
+// sum is evaluated here per atom pair:
+//   u,v on A:        G_uv = sum_B sum_{l in B} (uv|l) w_B(l) - 1/2 sum_{i,j in A} P_ij (ui|vj)
+//   u on A, v on B:  G_uv = -1/2 sum_{i in A, j in B} P_ij (ui|vj)
+// with w_B(l) = P_l (x2 for the off-diagonal AO pairs of B).
+//
+// k_fock: one wavefront per atom A, lanes over the partner atoms B.  Each lane
+// reads block (A,B) once, adds its Coulomb term to a per-lane J_A, and -- for
+// B below A in the molecule -- forms the exchange block K_AB and writes F_AB and
+// F_BA.  J_A is reduced across the wave with a fixed butterfly, so every run
+// (alone or inside any batch) adds the same numbers in the same order: no
+// floating-point atomics anywhere.
+#include "sg_common.h"
+#include "sg_kernels.h"
+
+namespace sg {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int K>
+struct Slot {
+  static constexpr int a = mop_a(K), b = mop_b(K);
+};
+
+// Entry (A slot ka | B slot kb) of the block shared by A and B.
+template <bool HA, bool HB, bool ROW>
+__device__ __forceinline__ double blk_entry(const double* __restrict__ b, int ka, int kb) {
+  if constexpr (HA && HB) return ROW ? b[ka * 10 + kb] : b[kb * 10 + ka];
+  else if constexpr (HA) return b[ka];
+  else if constexpr (HB) return b[kb];
+  else return b[0];
+}
+
+// One partner B for atom A.  ROW: A is the block's row atom (B.local <= A.local),
+// which is also when this lane owns the exchange block K_AB.
+template <bool HA, bool HB, bool ROW>
+__device__ __forceinline__ void partner(const double* __restrict__ b, const double* __restrict__ p,
+                                        int N, int ao_a, int ao_b, double (&J)[10],
+                                        double (&K)[4][4]) {
+  constexpr int NA = HA ? 4 : 1, NB = HB ? 4 : 1;
+  constexpr int SA = HA ? 10 : 1, SB = HB ? 10 : 1;
+  double w[SB];
+#pragma unroll
+  for (int l = 0; l < SB; ++l) {
+    const int x = mop_a(l), y = mop_b(l);
+    const double v = p[(int64_t)(ao_b + x) * N + ao_b + y];
+    w[l] = x == y ? v : 2.0 * v;
+  }
+  double pab[NA][NB];
+  if constexpr (ROW) {
+#pragma unroll
+    for (int x = 0; x < NA; ++x)
+#pragma unroll
+      for (int y = 0; y < NB; ++y) pab[x][y] = p[(int64_t)(ao_a + x) * N + ao_b + y];
+  }
+#pragma unroll
+  for (int k = 0; k < SA; ++k) {
+    const int mu = mop_a(k), nu = mop_b(k);
+#pragma unroll
+    for (int l = 0; l < SB; ++l) {
+      const int b1 = mop_a(l), b2 = mop_b(l);
+      const double val = blk_entry<HA, HB, ROW>(b, k, l);
+      J[k] += val * w[l];
+      if constexpr (ROW) {
+        K[mu][b1] += val * pab[nu][b2];
+        if (mu != nu) K[nu][b1] += val * pab[mu][b2];
+        if (b1 != b2) K[mu][b2] += val * pab[nu][b1];
+        if (mu != nu && b1 != b2) K[nu][b2] += val * pab[mu][b1];
+      }
+    }
+  }
+}
+
+template <bool HA, bool HB, bool ROW>
+__device__ __forceinline__ void write_offdiag(const double* __restrict__ h, double* __restrict__ f,
+                                              int N, int ao_a, int ao_b, const double (&K)[4][4]) {
+  constexpr int NA = HA ? 4 : 1, NB = HB ? 4 : 1;
+#pragma unroll
+  for (int x = 0; x < NA; ++x)
+#pragma unroll
+    for (int y = 0; y < NB; ++y) {
+      const int64_t r = ao_a + x, c = ao_b + y;
+      const double v = h[r * N + c] - 0.5 * K[x][y];
+      f[r * N + c] = v;
+      f[c * N + r] = v;
+    }
+}
+
+template <bool HA>
+__device__ void fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc* __restrict__ atoms,
+                          const double* __restrict__ blk, const double* __restrict__ h,
+                          const double* __restrict__ p, double* __restrict__ f, int lane) {
+  const int N = M.n_ao;
+  double J[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) J[k] = 0.0;
+  double Kaa[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) Kaa[x][y] = 0.0;
+  bool own_diag = false;
+  for (int b0 = 0; b0 < M.n_atoms; b0 += 64) {
+    const int bl = b0 + lane;
+    if (bl >= M.n_atoms) continue;
+    const AtomDesc B = atoms[M.atom0 + bl];
+    const bool row = B.local <= A.local;
+    const double* b = blk + (row ? block_offset(A, B) : block_offset(B, A));
+    if (row) {
+      double K[4][4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) K[x][y] = 0.0;
+      if (B.zn > 1) partner<HA, true, true>(b, p, N, A.ao0, B.ao0, J, K);
+      else partner<HA, false, true>(b, p, N, A.ao0, B.ao0, J, K);
+      if (B.local == A.local) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+          for (int y = 0; y < 4; ++y) Kaa[x][y] = K[x][y];
+        own_diag = true;
+      } else if (B.zn > 1) {
+        write_offdiag<HA, true, true>(h, f, N, A.ao0, B.ao0, K);
+      } else {
+        write_offdiag<HA, false, true>(h, f, N, A.ao0, B.ao0, K);
+      }
+    } else {
+      double K[4][4];
+      if (B.zn > 1) partner<HA, true, false>(b, p, N, A.ao0, B.ao0, J, K);
+      else partner<HA, false, false>(b, p, N, A.ao0, B.ao0, J, K);
+    }
+  }
+  constexpr int SA = HA ? 10 : 1;
+#pragma unroll
+  for (int k = 0; k < SA; ++k) J[k] = wave_sum(J[k]);
+  if (own_diag) {
+    constexpr int NA = HA ? 4 : 1;
+#pragma unroll
+    for (int x = 0; x < NA; ++x)
+#pragma unroll
+      for (int y = 0; y < NA; ++y) {
+        const int64_t r = A.ao0 + x, c = A.ao0 + y;
+        const double g = J[mop_idx(x, y)] - 0.5 * Kaa[x][y];
+        f[r * N + c] = h[r * N + c] + g;
+      }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fock(int n_atoms, const AtomDesc* __restrict__ atoms,
+                                               const MolDesc* __restrict__ mols,
+                                               const double* __restrict__ blocks,
+                                               const double* __restrict__ H,
+                                               const double* __restrict__ P, double* __restrict__ F,
+                                               const int* __restrict__ active,
+                                               const int* __restrict__ status) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ia = blockIdx.x * 4 + wave;
+  if (ia >= n_atoms) return;
+  const AtomDesc A = atoms[ia];
+  if (active && (!active[A.mol] || status[A.mol])) return;
+  const MolDesc M = mols[A.mol];
+  const double* blk = blocks + M.blk0;
+  if (A.zn > 1)
+    fock_atom<true>(A, M, atoms, blk, H + M.mat0, P + M.mat0, F + M.mat0, lane);
+  else
+    fock_atom<false>(A, M, atoms, blk, H + M.mat0, P + M.mat0, F + M.mat0, lane);
+}
+
+// Guess density (densitymatrix.cpp:26-36): P_ii = Z_core / n_AO(atom), 0 elsewhere.
+__global__ void k_guess_density(const MolDesc* __restrict__ mols, const AtomDesc* __restrict__ atoms,
+                                double* __restrict__ P) {
+  const MolDesc M = mols[blockIdx.x];
+  const int N = M.n_ao;
+  double* p = P + M.mat0;
+  for (int64_t t = threadIdx.x; t < (int64_t)N * N; t += blockDim.x) p[t] = 0.0;
+  __syncthreads();
+  for (int a = threadIdx.x; a < M.n_atoms; a += blockDim.x) {
+    const AtomDesc A = atoms[M.atom0 + a];
+    const int na = n_ao_of(A.zn);
+    const double v = (double)core_charge(A.zn) / na;
+    for (int u = 0; u < na; ++u) p[(int64_t)(A.ao0 + u) * N + A.ao0 + u] = v;
+  }
+}
+
+// E_el = 1/2 sum_ij P_ij (H_ij + F_ij) (ElectronicEnergy::ComputeEnergy,
+// electronicenergy.cpp:12-36), one workgroup per molecule, fixed-order tree.
+__global__ void __launch_bounds__(256) k_energy(const MolDesc* __restrict__ mols,
+                                                 const double* __restrict__ H,
+                                                 const double* __restrict__ F,
+                                                 const double* __restrict__ P,
+                                                 double* __restrict__ out,
+                                                 const int* __restrict__ active,
+                                                 const int* __restrict__ status) {
+  const int m = blockIdx.x;
+  if (active && (!active[m] || status[m])) return;
+  const MolDesc M = mols[m];
+  const int64_t nn = (int64_t)M.n_ao * M.n_ao;
+  const double* h = H + M.mat0;
+  const double* f = F + M.mat0;
+  const double* p = P + M.mat0;
+  double s = 0.0;
+  for (int64_t t = threadIdx.x; t < nn; t += blockDim.x) s += p[t] * (h[t] + f[t]);
+  __shared__ double red[4];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[m] = 0.5 * ((red[0] + red[1]) + (red[2] + red[3]));
+}
+
+__global__ void k_scf_init(int n_mol, int max_steps, const double* __restrict__ e0,
+                           double* __restrict__ e_el, int* __restrict__ steps,
+                           int* __restrict__ active, int* __restrict__ converged,
+                           int* __restrict__ status, int* __restrict__ n_active) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m == 0) *n_active = 0;
+  if (m >= n_mol) return;
+  e_el[m] = e0[m];
+  steps[m] = 0;
+  active[m] = max_steps > 0 ? 1 : 0;
+  converged[m] = 0;
+  status[m] = 0;
+}
+
+// Bookkeeping after one SCF step (scfcalculation.cpp:82-146): converged when
+// |E_k - E_(k-1)| < threshold (sameReal, mymath.h:68-75); a failed
+// eigensolve stops the molecule with its previous energy.
+__global__ void k_scf_update(int n_mol, double threshold, int max_steps,
+                             const double* __restrict__ e_new, double* __restrict__ e_el,
+                             int* __restrict__ steps, int* __restrict__ active,
+                             int* __restrict__ converged, const int* __restrict__ status,
+                             int* __restrict__ n_active) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_mol || !active[m]) return;
+  if (status[m]) {
+    active[m] = 0;
+    return;
+  }
+  const double old = e_el[m], e = e_new[m];
+  e_el[m] = e;
+  const int st = steps[m] + 1;
+  steps[m] = st;
+  const int conv = fabs(old - e) < threshold;
+  converged[m] = conv;
+  const int still = !conv && st < max_steps;
+  active[m] = still;
+  if (still) atomicAdd(n_active, 1);
+}
+
+static inline int blocks_for(int64_t n, int t) { return (int)((n + t - 1) / t); }
+
+void launch_guess_density(const BatchDev& b, hipStream_t s) {
+  k_guess_density<<<b.n_mol, 256, 0, s>>>(b.mols, b.atoms, b.P);
+}
+
+void launch_fock(const BatchDev& b, hipStream_t s, bool all) {
+  k_fock<<<blocks_for(b.n_atoms, 4), 256, 0, s>>>(b.n_atoms, b.atoms, b.mols, b.blocks, b.H, b.P, b.F,
+                                                  all ? nullptr : b.active, b.status);
+}
+
+void launch_energy(const BatchDev& b, hipStream_t s, double* out, bool all) {
+  k_energy<<<b.n_mol, 256, 0, s>>>(b.mols, b.H, b.F, b.P, out, all ? nullptr : b.active, b.status);
+}
+
+void launch_scf_init(const BatchDev& b, hipStream_t s, int max_steps) {
+  k_scf_init<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(b.n_mol, max_steps, b.e_new, b.e_el, b.steps,
+                                                      b.active, b.converged, b.status, b.n_active);
+}
+
+void launch_scf_update(const BatchDev& b, hipStream_t s, double threshold, int max_steps) {
+  k_scf_update<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(b.n_mol, threshold, max_steps, b.e_new,
+                                                        b.e_el, b.steps, b.active, b.converged,
+                                                        b.status, b.n_active);
+}
+
+}  // namespace sg

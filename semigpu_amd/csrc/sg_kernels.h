@@ -1,0 +1,57 @@
+// sg_kernels.h -- device batch layout and kernel launchers (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sg_common.h"
+
+namespace sg {
+
+// Device view of a batch of molecules.  All matrices are dense N x N per
+// molecule, row-major (symmetric ones are stored in full); eigenvectors are
+// column-major (C[k*N + i] = component i of eigenvector k), which is the
+// layout the reference hands to dsyev (scfcalculation.cpp:175-179).
+struct BatchDev {
+  int n_mol = 0, n_atoms = 0;
+  int64_t n_mat = 0;     // sum N^2
+  int64_t n_vec = 0;     // sum N
+  int64_t n_blocks = 0;  // sum of pair-block doubles
+  MolDesc* mols = nullptr;
+  AtomDesc* atoms = nullptr;
+  // type-sorted atom-pair lists (row atom, column atom), row local > column local
+  int2* pairs_xx = nullptr; int n_xx = 0;
+  int2* pairs_hx = nullptr; int n_hx = 0;
+  int2* pairs_hh = nullptr; int n_hh = 0;
+  int2* pairs_all = nullptr; int n_pairs_all = 0;
+  int2* diag_tasks = nullptr; int n_diag_tasks = 0;  // (atom, slot)
+  double* blocks = nullptr;  // pair integrals, reference jagged order per molecule
+  double* H = nullptr;
+  double* F = nullptr;
+  double* P = nullptr;
+  double* C = nullptr;       // eigenvectors
+  double* W = nullptr;       // eigensolver workspace (2 * n_mat)
+  double* eig = nullptr;     // eigenvalues (n_vec)
+  double* tau = nullptr;     // Householder scalars (n_vec)
+  double* e_cc = nullptr;    // per molecule
+  double* e_el = nullptr;    // current electronic energy
+  double* e_new = nullptr;   // energy of the step just computed
+  int* steps = nullptr;
+  int* active = nullptr;     // 1 while the SCF of the molecule iterates
+  int* converged = nullptr;
+  int* status = nullptr;     // 0 ok, >0 eigensolver failure
+  int* n_active = nullptr;   // device counter
+  int max_n = 0;             // largest N in the batch
+};
+
+void launch_pair_integrals(const BatchDev& b, hipStream_t s);
+void launch_hcore(const BatchDev& b, hipStream_t s);
+void launch_core_core(const BatchDev& b, hipStream_t s);
+void launch_guess_density(const BatchDev& b, hipStream_t s);
+void launch_fock(const BatchDev& b, hipStream_t s, bool all);
+void launch_energy(const BatchDev& b, hipStream_t s, double* out, bool all);
+void launch_scf_init(const BatchDev& b, hipStream_t s, int max_steps);
+void launch_scf_update(const BatchDev& b, hipStream_t s, double threshold, int max_steps);
+void launch_eigh(const BatchDev& b, hipStream_t s, bool all);
+void launch_density(const BatchDev& b, hipStream_t s, bool all);
+
+}  // namespace sg
