@@ -1,0 +1,148 @@
+"""Parity of the HIP path (through the C ABI) with the reference CPU path.
+
+Bars (also stated in DESIGN.md):
+  pair blocks            bit-identical to the reference
+  Hcore, Fock            |gpu - ref| <= 1e-10 * max|ref|  (normwise relative)
+  energies               |gpu - ref| <= 1e-6 eV at SCF threshold 1e-6 eV, same step count
+  batch vs alone         bit-identical (no floating-point atomics, fixed reduction order)
+Reference values are the committed fixtures written by the compiled reference
+(tests/golden); sizes beyond the fixtures are checked against the oracle.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import atoms_of, golden_names, load_golden
+from semigpu_amd import api, geometry
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+FULL = golden_names(full_only=True)
+LARGE = [n for n in golden_names() if n not in FULL]
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def normwise(a, ref):
+    return float(np.abs(a - ref).max() / max(np.abs(ref).max(), 1e-300))
+
+
+@pytest.mark.parametrize("name", FULL)
+def test_integrals_hcore_fock(engine, name):
+    say("case", name)
+    g = load_golden(name)
+    b = engine.batch([atoms_of(g)])
+    b.two_center()
+    assert np.array_equal(b.pair_blocks(0), g["pair_blocks"])
+    b.hcore()
+    assert normwise(b.packed(0, api.SG_HCORE), g["hcore"]) <= REL
+    b.guess_density()
+    b.fock()
+    assert normwise(b.packed(0, api.SG_FOCK), g["guess_f"]) <= REL
+    b.set_packed(0, api.SG_DENSITY, g["iter1_p"])
+    b.fock()
+    assert normwise(b.packed(0, api.SG_FOCK), g["iter1_f"]) <= REL
+    assert abs(b.core_core()[0] - float(g["e_cc"])) <= 1e-9
+    b.close()
+
+
+@pytest.mark.parametrize("name", LARGE)
+def test_large_sampled(engine, name):
+    say("case", name)
+    g = load_golden(name)
+    b = engine.batch([atoms_of(g)])
+    b.two_center()
+    blk = b.pair_blocks(0)
+    assert np.array_equal(blk[g["pair_blocks_idx"]], g["pair_blocks_val"])
+    b.hcore()
+    h = b.packed(0, api.SG_HCORE)
+    assert np.abs(h[g["hcore_idx"]] - g["hcore_val"]).max() <= REL * np.abs(g["hcore_val"]).max()
+    r = b.scf(threshold=float(g["threshold"]), max_steps=100)[0]
+    assert r["steps"] == int(g["steps"]) and r["converged"]
+    assert abs(r["e_el"] - float(g["e_el"])) <= 1e-6
+    assert abs(r["e_tot"] - float(g["e_tot"])) <= 1e-6
+    b.close()
+
+
+def test_scf_energies_batch(engine):
+    names = FULL
+    say("batch of", len(names))
+    mols = [atoms_of(load_golden(n)) for n in names]
+    b = engine.batch(mols)
+    res = b.scf(threshold=1e-6, max_steps=100)
+    for n, r in zip(names, res):
+        g = load_golden(n)
+        assert r["converged"] == bool(g["converged"]), n
+        assert r["steps"] == int(g["steps"]), n
+        assert abs(r["e_el"] - float(g["e_el"])) <= 1e-6, n
+        assert abs(r["e_cc"] - float(g["e_cc"])) <= 1e-9, n
+        assert abs(r["e_tot"] - float(g["e_tot"])) <= 1e-6, n
+    b.close()
+
+
+def test_shipped_threshold(engine):
+    """Default threshold 5e-3 eV: the reference's own ComputeSCF result."""
+    names = ["ref_04_pyridine", "ref_03_formaldehyde", "ref_02_benzene", "ref_02_styrene"]
+    b = engine.batch([atoms_of(load_golden(n)) for n in names])
+    res = b.scf(threshold=5e-3, max_steps=100)
+    for n, r in zip(names, res):
+        assert abs(r["e_el"] - float(load_golden(n)["shipped_e_el"])) <= 1e-6, n
+    b.close()
+
+
+def test_batch_equals_alone_bitwise(engine):
+    mols = geometry.water_batch(12, seed=77)
+    mols.insert(3, geometry.alkane(7))
+    mols.insert(7, geometry.pyridine())
+    big = engine.batch(mols).scf(threshold=1e-6)
+    for i in (0, 3, 7, len(mols) - 1):
+        alone = engine.batch([mols[i]]).scf(threshold=1e-6)[0]
+        say("alone", i, alone["e_tot"], big[i]["e_tot"])
+        assert alone == big[i]
+
+
+def test_eigensolver_properties(engine):
+    g = load_golden("ref_02_benzene")
+    n = int(g["n_ao"])
+    b = engine.batch([atoms_of(g)])
+    b.two_center()
+    b.hcore()
+    b.set_packed(0, api.SG_DENSITY, g["final_p"])
+    b.fock()
+    b.eigh()
+    w, C = b.eigen(0)
+    import pyoracle as po
+    F = po.packed_to_full(b.packed(0, api.SG_FOCK), n)
+    wr = np.linalg.eigh(F)[0]
+    scale = np.abs(wr).max()
+    assert np.abs(w - wr).max() <= 1e-12 * scale
+    assert np.abs(C.T @ C - np.eye(n)).max() <= 1e-12
+    assert np.abs(F @ C - C * w).max() <= 1e-11 * scale
+    b.close()
+
+
+def test_water_clusters_against_oracle(engine):
+    import pyoracle as po
+    mols = geometry.water_batch(6, seed=5, n_min=2, n_max=10)
+    res = engine.batch(mols).scf(threshold=1e-6)
+    for mol, r in zip(mols, res):
+        o = po.Molecule(mol).scf(threshold=1e-6, mode="nddo")
+        assert r["steps"] == o["steps"]
+        assert abs(r["e_tot"] - o["e_tot"]) <= 1e-6
+
+
+def test_unsupported_element_rejected(engine):
+    with pytest.raises(api.SemiGpuError, match="Z=9"):
+        engine.batch([[(9, 0.0, 0.0, 0.0), (1, 0.0, 0.0, 0.9)]])
+
+
+def test_step_cap_reports_not_converged(engine):
+    b = engine.batch([geometry.pyridine()])
+    r = b.scf(threshold=1e-12, max_steps=2)[0]
+    assert r["steps"] == 2 and not r["converged"] and r["status"] == 0
+    r0 = engine.batch([geometry.pyridine()]).scf(threshold=1e-6, max_steps=0)[0]
+    assert r0["steps"] == 0 and not r0["converged"]
