@@ -255,8 +255,10 @@ int sg_batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, const i
   const size_t o_blk = take(sizeof(double) * blk0);
   const size_t o_H = take(sizeof(double) * mat0), o_F = take(sizeof(double) * mat0),
                o_P = take(sizeof(double) * mat0), o_C = take(sizeof(double) * mat0),
-               o_W = take(sizeof(double) * 2 * mat0);
-  const size_t o_eig = take(sizeof(double) * vec0), o_tau = take(sizeof(double) * vec0);
+               o_W = take(sizeof(double) * 5 * mat0);
+  const size_t o_eig = take(sizeof(double) * vec0), o_tau = take(sizeof(double) * vec0),
+               o_dT = take(sizeof(double) * vec0), o_eT = take(sizeof(double) * vec0),
+               o_tn = take(sizeof(double) * n_mol);
   const size_t o_ecc = take(sizeof(double) * n_mol), o_eel = take(sizeof(double) * n_mol),
                o_enew = take(sizeof(double) * n_mol);
   const size_t o_steps = take(sizeof(int) * n_mol), o_act = take(sizeof(int) * n_mol),
@@ -285,6 +287,9 @@ int sg_batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, const i
   d.W = (double*)(base + o_W);
   d.eig = (double*)(base + o_eig);
   d.tau = (double*)(base + o_tau);
+  d.dT = (double*)(base + o_dT);
+  d.eT = (double*)(base + o_eT);
+  d.tnorm = (double*)(base + o_tn);
   d.e_cc = (double*)(base + o_ecc);
   d.e_el = (double*)(base + o_eel);
   d.e_new = (double*)(base + o_enew);

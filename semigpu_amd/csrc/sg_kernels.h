@@ -29,9 +29,12 @@ struct BatchDev {
   double* F = nullptr;
   double* P = nullptr;
   double* C = nullptr;       // eigenvectors
-  double* W = nullptr;       // eigensolver workspace (2 * n_mat)
-  double* eig = nullptr;     // eigenvalues (n_vec)
+  double* W = nullptr;       // eigensolver workspace (5 * n_mat)
+  double* eig = nullptr;     // eigenvalues (n_vec; the selected ones first)
   double* tau = nullptr;     // Householder scalars (n_vec)
+  double* dT = nullptr;      // tridiagonal diagonal (n_vec)
+  double* eT = nullptr;      // tridiagonal off-diagonal (n_vec)
+  double* tnorm = nullptr;   // Gershgorin bound of each tridiagonal (n_mol)
   double* e_cc = nullptr;    // per molecule
   double* e_el = nullptr;    // current electronic energy
   double* e_new = nullptr;   // energy of the step just computed

@@ -1,27 +1,47 @@
-// sg_linalg.hip -- density matrix and dense symmetric eigensolver, batched.
+// sg_linalg.hip -- batched dense symmetric eigensolver and density matrix.
 //
-// k_density: P = 2 C_occ C_occ^T (DensityMatrix::ComputeElementMatrix,
-//   densitymatrix.cpp:38-45); C column-major, n_occ = floor(n_e / 2).
+// Replaces LAPACK dsyev('V','L') on the Fock matrix
+// (SCFCalculation::GetEigenValVecOfFmatrix, scfcalculation.cpp:169-203) and
+// the density build (DensityMatrix::ComputeElementMatrix,
+// densitymatrix.cpp:38-45).  Four kernels, one workgroup per molecule:
 //
-// k_eigh: replaces LAPACK dsyev('V','L') on the Fock matrix
-//   (SCFCalculation::GetEigenValVecOfFmatrix, scfcalculation.cpp:171-203).
-//   One workgroup per molecule:
-//     1. Householder tridiagonalisation of a copy of F (workspace W), the
-//        reflector of column k kept in row k of W;
-//     2. Q = H_0 ... H_(n-3) accumulated backward into the workspace Z;
-//     3. implicit symmetric QR with Wilkinson shifts on (d, e) -- one lane
-//        computes a sweep's Givens rotations into LDS, then every lane replays
-//        the sweep on its own rows of Z (rows are independent, no barriers
-//        inside a sweep);
-//     4. eigenvalues sorted ascending, eigenvectors written to C.
-//   Golub & Van Loan, Matrix Computations (4th ed.) 8.3.
+//  k_tridiag        Householder reduction to tridiagonal form, blocked as in
+//                   LAPACK dsytrd/dlatrd: a panel of nb reflectors is built with
+//                   its V and W columns in LDS (one matrix-vector product over the
+//                   trailing block per column), then the trailing block gets one
+//                   rank-2nb update.  The matrix is kept in full symmetric
+//                   storage so the product is a row-wise dot (one wavefront per
+//                   row, coalesced, no cross-wave reduction).  Reflector c is
+//                   stored in row c of the work matrix.
+//  k_bisect         eigenvalues of the tridiagonal matrix by bisection on the
+//                   Sturm count, one lane per eigenvalue (only the indices the
+//                   caller selected).
+//  k_invit          eigenvectors of the tridiagonal matrix by inverse iteration
+//                   (3 sweeps, Gaussian elimination with partial pivoting, one
+//                   lane per vector).  Eigenvalues closer than 1e-3*||T|| form a
+//                   cluster; after each sweep a cluster's vectors are
+//                   orthonormalised in order (modified Gram-Schmidt) by one
+//                   wavefront, as LAPACK dstein does.
+//  k_backtransform  Y <- Q Y with the reflectors grouped in compact WY form
+//                   (I - V T V^T), so Y is read twice and written once per panel.
+//
+// The SCF only needs the density P = 2 C_occ C_occ^T, i.e. the invariant
+// subspace of the occupied levels.  It selects whichever side is smaller:
+// the n_occ lowest vectors (P = 2 Y Y^T) or the n - n_occ highest
+// (P = 2 (I - Y Y^T)); sg_eigh selects all n for the full eigendecomposition.
+//
+// Workspace per molecule (5 N^2 doubles at 5*mat0): work matrix A, vectors
+// Y (N x k, row-major), inverse-iteration factors (3 N x min(k,512)).
+// Golub & Van Loan, Matrix Computations (4th ed.) 5.1.7, 8.3, 8.4.
 #include "sg_common.h"
 #include "sg_kernels.h"
 
 namespace sg {
 
-constexpr int kEigThreads = 256;
-constexpr int kEigMaxN = 1024;
+constexpr int kT = 512;                 // threads per eigensolver workgroup
+constexpr int kPanelLds = 96 * 1024;    // bytes of LDS for the V/W panel
+constexpr int kPanelDoubles = kPanelLds / 8;
+constexpr double kEps = 2.220446049250313e-16;
 
 __device__ __forceinline__ double wsum(double v) {
 #pragma unroll
@@ -29,234 +49,609 @@ __device__ __forceinline__ double wsum(double v) {
   return v;
 }
 
-// Block-wide sum, result returned to every thread (fixed order).
+// Block-wide sum in a fixed order; every thread gets the result.
 __device__ double block_sum(double v, double* red) {
   v = wsum(v);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
   __syncthreads();
   if (l == 0) red[w] = v;
   __syncthreads();
   double t = 0.0;
-  const int nw = blockDim.x >> 6;
   for (int i = 0; i < nw; ++i) t += red[i];
   return t;
 }
 
-__global__ void __launch_bounds__(kEigThreads) k_eigh(const MolDesc* __restrict__ mols,
-                                                       const double* __restrict__ F,
-                                                       double* __restrict__ Wbuf,
-                                                       double* __restrict__ C,
-                                                       double* __restrict__ eig,
-                                                       const int* __restrict__ active,
-                                                       int* __restrict__ status) {
-  const int m = blockIdx.x;
-  if (active && (!active[m] || status[m])) return;
-  const MolDesc M = mols[m];
-  const int n = M.n_ao;
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int wave = tid >> 6, lane = tid & 63, nwave = nt >> 6;
-  double* W = Wbuf + 2 * M.mat0;       // n x n, row-major
-  double* Z = W + (int64_t)n * n;      // n x n, column-major
-  const double* f = F + M.mat0;
-  double* c = C + M.mat0;
-  double* w_out = eig + M.vec0;
-
-  __shared__ double d[kEigMaxN], e[kEigMaxN], vs[kEigMaxN], ps[kEigMaxN];
-  __shared__ double tau_s[kEigMaxN];
-  __shared__ double red[kEigThreads / 64];
-  __shared__ int perm[kEigMaxN];
-  __shared__ int s_lo, s_hi, s_stat;
-
-  for (int64_t t = tid; t < (int64_t)n * n; t += nt) W[t] = f[t];
-  __syncthreads();
-
-  // 1. tridiagonalisation
-  for (int k = 0; k + 2 < n; ++k) {
-    const double* rowk = W + (int64_t)k * n;
-    double s2 = 0.0;
-    for (int i = k + 2 + tid; i < n; i += nt) s2 += rowk[i] * rowk[i];
-    s2 = block_sum(s2, red);
-    const double x0 = rowk[k + 1];
-    if (s2 == 0.0) {
-      if (tid == 0) {
-        tau_s[k] = 0.0;
-        e[k] = x0;
-      }
-      __syncthreads();
-      continue;
-    }
-    const double nrm = sqrt(x0 * x0 + s2);
-    const double beta = x0 >= 0.0 ? -nrm : nrm;
-    const double tk = (beta - x0) / beta;
-    const double sc = 1.0 / (x0 - beta);
-    for (int i = k + 1 + tid; i < n; i += nt) {
-      const double v = i == k + 1 ? 1.0 : rowk[i] * sc;
-      vs[i] = v;
-    }
-    if (tid == 0) {
-      tau_s[k] = tk;
-      e[k] = beta;
-    }
-    __syncthreads();
-    // store v in row k (read back in step 2)
-    for (int i = k + 1 + tid; i < n; i += nt) W[(int64_t)k * n + i] = vs[i];
-    // p = tau * W22 v : one wave per row
-    for (int i = k + 1 + wave; i < n; i += nwave) {
-      const double* ri = W + (int64_t)i * n;
-      double s = 0.0;
-      for (int j = k + 1 + lane; j < n; j += 64) s += ri[j] * vs[j];
-      s = wsum(s);
-      if (lane == 0) ps[i] = tk * s;
-    }
-    __syncthreads();
-    double kk = 0.0;
-    for (int i = k + 1 + tid; i < n; i += nt) kk += ps[i] * vs[i];
-    kk = block_sum(kk, red) * (0.5 * tk);
-    for (int i = k + 1 + tid; i < n; i += nt) ps[i] -= kk * vs[i];
-    __syncthreads();
-    const int mm = n - k - 1;
-    for (int64_t t = tid; t < (int64_t)mm * mm; t += nt) {
-      const int i = k + 1 + (int)(t / mm), j = k + 1 + (int)(t % mm);
-      W[(int64_t)i * n + j] -= vs[i] * ps[j] + ps[i] * vs[j];
-    }
-    __syncthreads();
-  }
-  for (int i = tid; i < n; i += nt) d[i] = W[(int64_t)i * n + i];
-  if (tid == 0) {
-    if (n >= 2) e[n - 2] = W[(int64_t)(n - 1) * n + (n - 2)];
-    if (n >= 1) e[n - 1] = 0.0;
-    s_stat = 0;
-  }
-  // 2. Q accumulated backward into Z (column-major)
-  for (int64_t t = tid; t < (int64_t)n * n; t += nt) {
-    const int col = (int)(t / n), row = (int)(t % n);
-    Z[t] = row == col ? 1.0 : 0.0;
-  }
-  __syncthreads();
-  for (int k = n - 3; k >= 0; --k) {
-    const double tk = tau_s[k];
-    if (tk == 0.0) continue;
-    for (int i = k + 1 + tid; i < n; i += nt) vs[i] = W[(int64_t)k * n + i];
-    __syncthreads();
-    for (int j = k + 1 + wave; j < n; j += nwave) {
-      double* zj = Z + (int64_t)j * n;
-      double s = 0.0;
-      for (int i = k + 1 + lane; i < n; i += 64) s += vs[i] * zj[i];
-      s = wsum(s) * tk;
-      for (int i = k + 1 + lane; i < n; i += 64) zj[i] -= s * vs[i];
-    }
-    __syncthreads();
-  }
-  // 3. implicit QR on (d, e); rotations of a sweep in vs (cos) / ps (sin)
-  const double eps = 2.220446049250313e-16;
-  int iters = 0;
-  while (true) {
-    if (tid == 0) {
-      int hi = n - 1;
-      for (int i = 0; i < hi; ++i)
-        if (fabs(e[i]) <= eps * (fabs(d[i]) + fabs(d[i + 1]))) e[i] = 0.0;
-      while (hi > 0 && e[hi - 1] == 0.0) --hi;
-      int lo = hi;
-      if (hi > 0) {
-        lo = hi - 1;
-        while (lo > 0 && e[lo - 1] != 0.0) --lo;
-        if (++iters > 60 * n) {
-          s_stat = hi + 1;
-          hi = 0;
-        } else {
-          const double dl = (d[hi - 1] - d[hi]) * 0.5, b = e[hi - 1];
-          const double den = dl + (dl >= 0.0 ? hypot(dl, b) : -hypot(dl, b));
-          const double mu = d[hi] - b * b / den;
-          double x = d[lo] - mu, y = e[lo];
-          for (int k = lo; k < hi; ++k) {
-            const double r = hypot(x, y);
-            double cs = 1.0, sn = 0.0;
-            if (r != 0.0) {
-              cs = x / r;
-              sn = y / r;
-            }
-            if (k > lo) e[k - 1] = r;
-            const double dk = d[k], dk1 = d[k + 1], ek = e[k];
-            d[k] = cs * cs * dk + 2.0 * cs * sn * ek + sn * sn * dk1;
-            d[k + 1] = sn * sn * dk - 2.0 * cs * sn * ek + cs * cs * dk1;
-            e[k] = cs * sn * (dk1 - dk) + (cs * cs - sn * sn) * ek;
-            if (k + 1 < hi) {
-              y = sn * e[k + 1];
-              e[k + 1] = cs * e[k + 1];
-              x = e[k];
-            }
-            vs[k] = cs;
-            ps[k] = sn;
-          }
-        }
-      }
-      s_lo = lo;
-      s_hi = hi;
-    }
-    __syncthreads();
-    const int lo = s_lo, hi = s_hi;
-    if (hi <= 0) break;
-    for (int i = tid; i < n; i += nt) {
-      double carry = Z[(int64_t)lo * n + i];
-      for (int k = lo; k < hi; ++k) {
-        const double nxt = Z[(int64_t)(k + 1) * n + i];
-        const double cs = vs[k], sn = ps[k];
-        Z[(int64_t)k * n + i] = cs * carry + sn * nxt;
-        carry = -sn * carry + cs * nxt;
-      }
-      Z[(int64_t)hi * n + i] = carry;
-    }
-    __syncthreads();
-  }
-  // 4. ascending order (rank by value, ties by index)
-  for (int i = tid; i < n; i += nt) {
-    const double di = d[i];
-    int r = 0;
-    for (int j = 0; j < n; ++j) {
-      const double dj = d[j];
-      r += (dj < di) || (dj == di && j < i);
-    }
-    perm[i] = r;
-  }
-  __syncthreads();
-  for (int64_t t = tid; t < (int64_t)n * n; t += nt) {
-    const int col = (int)(t / n), row = (int)(t % n);
-    c[(int64_t)perm[col] * n + row] = Z[t];
-  }
-  for (int i = tid; i < n; i += nt) w_out[perm[i]] = d[i];
-  if (tid == 0 && s_stat) status[m] = s_stat;
+__device__ __forceinline__ bool mol_live(const int* active, const int* status, int m) {
+  return !active || (active[m] && !status[m]);
 }
 
-// P = 2 C_occ C_occ^T: one workgroup per (molecule, 16-row strip).
+// Selected eigenvalue range: all, or the smaller of occupied / virtual.
+__device__ __forceinline__ void selection(const MolDesc& M, int want_all, int& lo, int& k) {
+  const int n = M.n_ao, no = M.n_occ;
+  if (want_all) {
+    lo = 0;
+    k = n;
+  } else if (no <= n - no) {
+    lo = 0;
+    k = no;
+  } else {
+    lo = no;
+    k = n - no;
+  }
+}
+
+__device__ __forceinline__ int panel_width(int n) {
+  int nb = kPanelLds / (16 * (n > 0 ? n : 1));
+  return nb < 1 ? 1 : (nb > 16 ? 16 : nb);
+}
+
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kT) k_tridiag(const MolDesc* __restrict__ mols,
+                                                 const double* __restrict__ F,
+                                                 double* __restrict__ work,
+                                                 double* __restrict__ dA, double* __restrict__ eA,
+                                                 double* __restrict__ tauA,
+                                                 const int* __restrict__ active,
+                                                 const int* __restrict__ status) {
+  extern __shared__ double lds[];
+  const int m = blockIdx.x;
+  if (!mol_live(active, status, m)) return;
+  const MolDesc M = mols[m];
+  const int n = M.n_ao, tid = threadIdx.x, nt = blockDim.x;
+  const int wave = tid >> 6, lane = tid & 63, nwave = nt >> 6;
+  const int nb = panel_width(n);
+  double* V = lds;                  // nb columns of length n (column-major)
+  double* Wp = V + (size_t)nb * n;   // nb columns of length n; 2*nb*n <= kPanelDoubles
+  double* x = lds + kPanelDoubles;   // n
+  double* tv = x + n;               // 2 * nb <= 32
+  double* red = tv + 32;            // nwave
+  double* A = work + 5 * M.mat0;
+  double* d = dA + M.vec0;
+  double* e = eA + M.vec0;
+  double* tau = tauA + M.vec0;
+  const double* f = F + M.mat0;
+  for (int64_t t = tid; t < (int64_t)n * n; t += nt) A[t] = f[t];
+  __syncthreads();
+  const int nref = n > 2 ? n - 2 : 0;
+  for (int k0 = 0; k0 < nref; k0 += nb) {
+    const int b = min(nb, nref - k0);
+    for (int t = tid; t < nb * n; t += nt) {
+      V[t] = 0.0;
+      Wp[t] = 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < b; ++j) {
+      const int c = k0 + j;
+      // column c of the partially reduced matrix (rows c..n-1)
+      for (int i = c + tid; i < n; i += nt) {
+        double s = A[(int64_t)c * n + i];
+        for (int q = 0; q < j; ++q)
+          s -= V[q * n + i] * Wp[q * n + c] + Wp[q * n + i] * V[q * n + c];
+        x[i] = s;
+      }
+      __syncthreads();
+      double s2 = 0.0;
+      for (int i = c + 2 + tid; i < n; i += nt) s2 += x[i] * x[i];
+      s2 = block_sum(s2, red);
+      const double alpha = x[c + 1];
+      if (tid == 0) d[c] = x[c];
+      if (s2 == 0.0) {
+        if (tid == 0) {
+          tau[c] = 0.0;
+          e[c] = alpha;
+        }
+        for (int i = c + 1 + tid; i < n; i += nt) A[(int64_t)c * n + i] = 0.0;
+        __syncthreads();
+        continue;
+      }
+      const double beta = -copysign(sqrt(alpha * alpha + s2), alpha);
+      const double tk = (beta - alpha) / beta;
+      const double sc = 1.0 / (alpha - beta);
+      for (int i = c + 1 + tid; i < n; i += nt) {
+        const double v = i == c + 1 ? 1.0 : x[i] * sc;
+        V[j * n + i] = v;
+        A[(int64_t)c * n + i] = v;
+      }
+      if (tid == 0) {
+        tau[c] = tk;
+        e[c] = beta;
+      }
+      __syncthreads();
+      // p = A22 v over the not yet updated trailing block: one wave per row
+      for (int i = c + 1 + wave; i < n; i += nwave) {
+        const double* ri = A + (int64_t)i * n;
+        double s = 0.0;
+        for (int l = c + 1 + lane; l < n; l += 64) s += ri[l] * V[j * n + l];
+        s = wsum(s);
+        if (lane == 0) x[i] = s;
+      }
+      // W^T v and V^T v for the previous columns of the panel
+      for (int q = wave; q < 2 * j; q += nwave) {
+        const double* col = (q < j) ? (Wp + q * n) : (V + (q - j) * n);
+        double s = 0.0;
+        for (int i = c + 1 + lane; i < n; i += 64) s += col[i] * V[j * n + i];
+        s = wsum(s);
+        if (lane == 0) tv[q] = s;
+      }
+      __syncthreads();
+      double pv = 0.0;
+      for (int i = c + 1 + tid; i < n; i += nt) {
+        double s = x[i];
+        for (int q = 0; q < j; ++q) s -= V[q * n + i] * tv[q] + Wp[q * n + i] * tv[j + q];
+        s *= tk;
+        x[i] = s;
+        pv += s * V[j * n + i];
+      }
+      pv = block_sum(pv, red);
+      const double al = -0.5 * tk * pv;
+      for (int i = c + 1 + tid; i < n; i += nt) Wp[j * n + i] = x[i] + al * V[j * n + i];
+      __syncthreads();
+    }
+    // trailing update of the full symmetric block (rows, cols >= k0 + b)
+    const int r = k0 + b, mm = n - r;
+    for (int64_t t = tid; t < (int64_t)mm * mm; t += nt) {
+      const int i = r + (int)(t / mm), l = r + (int)(t % mm);
+      double s = A[(int64_t)i * n + l];
+      for (int q = 0; q < b; ++q) s -= V[q * n + i] * Wp[q * n + l] + Wp[q * n + i] * V[q * n + l];
+      A[(int64_t)i * n + l] = s;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (n >= 2) {
+      d[n - 2] = A[(int64_t)(n - 2) * n + (n - 2)];
+      e[n - 2] = A[(int64_t)(n - 1) * n + (n - 2)];
+    }
+    d[n - 1] = A[(int64_t)(n - 1) * n + (n - 1)];
+    e[n - 1] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sturm count: number of eigenvalues of T below x (LDL^T pivots < 0).
+__device__ __forceinline__ int sturm(const double* d, const double* e2, int n, double x,
+                                     double pivmin) {
+  int cnt = 0;
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  cnt += q < 0.0;
+  for (int k = 1; k < n; ++k) {
+    q = (d[k] - x) - e2[k - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += q < 0.0;
+  }
+  return cnt;
+}
+
+__global__ void __launch_bounds__(256) k_bisect(const MolDesc* __restrict__ mols,
+                                                 const double* __restrict__ dA,
+                                                 const double* __restrict__ eA,
+                                                 double* __restrict__ lamA,
+                                                 double* __restrict__ tnormA, int want_all,
+                                                 const int* __restrict__ active,
+                                                 const int* __restrict__ status) {
+  extern __shared__ double lds[];
+  const int m = blockIdx.y;
+  if (!mol_live(active, status, m)) return;
+  const MolDesc M = mols[m];
+  const int n = M.n_ao;
+  int lo, k;
+  selection(M, want_all, lo, k);
+  if ((int)blockIdx.x * (int)blockDim.x >= k) return;
+  double* d = lds;
+  double* e2 = d + n;
+  double* red = e2 + n;
+  const double* dg = dA + M.vec0;
+  const double* eg = eA + M.vec0;
+  double glo = 1e300, ghi = -1e300, emax = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double di = dg[i];
+    const double el = i > 0 ? fabs(eg[i - 1]) : 0.0, er = i < n - 1 ? fabs(eg[i]) : 0.0;
+    d[i] = di;
+    e2[i] = i < n - 1 ? eg[i] * eg[i] : 0.0;
+    glo = fmin(glo, di - el - er);
+    ghi = fmax(ghi, di + el + er);
+    emax = fmax(emax, e2[i]);
+  }
+  // min / max reductions (exact, order-independent)
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    glo = fmin(glo, __shfl_xor(glo, off, 64));
+    ghi = fmax(ghi, __shfl_xor(ghi, off, 64));
+    emax = fmax(emax, __shfl_xor(emax, off, 64));
+  }
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[3 * w] = glo;
+    red[3 * w + 1] = ghi;
+    red[3 * w + 2] = emax;
+  }
+  __syncthreads();
+  glo = red[0];
+  ghi = red[1];
+  emax = red[2];
+  for (int i = 1; i < nw; ++i) {
+    glo = fmin(glo, red[3 * i]);
+    ghi = fmax(ghi, red[3 * i + 1]);
+    emax = fmax(emax, red[3 * i + 2]);
+  }
+  const double tnorm = fmax(fabs(glo), fabs(ghi));
+  const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax);
+  glo -= 2.0 * kEps * tnorm + 2.0 * pivmin;
+  ghi += 2.0 * kEps * tnorm + 2.0 * pivmin;
+  if (blockIdx.x == 0 && threadIdx.x == 0) tnormA[m] = tnorm;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= k) return;
+  const int idx = lo + j;
+  double a = glo, b = ghi;
+  for (int it = 0; it < 128; ++it) {
+    if (b - a <= 2.0 * kEps * fmax(fabs(a), fabs(b)) + pivmin) break;
+    const double mid = 0.5 * (a + b);
+    if (sturm(d, e2, n, mid, pivmin) > idx) b = mid;
+    else a = mid;
+  }
+  lamA[M.vec0 + j] = 0.5 * (a + b);
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic start vector entries in [-1, 1).
+__device__ __forceinline__ double start_entry(int vec, int row) {
+  uint64_t z = ((uint64_t)(uint32_t)vec << 32) ^ (uint64_t)(uint32_t)row;
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+// Solves (T - lam I) y = y in place (y column `col` of Y, row stride ldy) by
+// Gaussian elimination with partial pivoting; fac holds 3 doubles per row for
+// this lane (stride fs between rows).  Returns the 2-norm of the solution.
+__device__ double tri_solve(const double* __restrict__ d, const double* __restrict__ e, int n,
+                            double lam, double tiny, double* __restrict__ Y, int ldy, int col,
+                            double* __restrict__ fac, int fs) {
+  double ca = d[0] - lam, cb = n > 1 ? e[0] : 0.0;
+  double y0 = Y[col];
+  for (int i = 0; i + 1 < n; ++i) {
+    const double sub = e[i], na = d[i + 1] - lam, nb = i + 2 < n ? e[i + 1] : 0.0;
+    const double y1 = Y[(int64_t)(i + 1) * ldy + col];
+    double* fr = fac + (int64_t)i * 3 * fs;
+    if (fabs(ca) >= fabs(sub)) {
+      const double mlt = ca != 0.0 ? sub / ca : 0.0;
+      fr[0] = ca;
+      fr[fs] = cb;
+      fr[2 * fs] = 0.0;
+      Y[(int64_t)i * ldy + col] = y0;
+      y0 = y1 - mlt * y0;
+      ca = na - mlt * cb;
+      cb = nb;
+    } else {
+      const double mlt = ca / sub;
+      fr[0] = sub;
+      fr[fs] = na;
+      fr[2 * fs] = nb;
+      Y[(int64_t)i * ldy + col] = y1;
+      y0 = y0 - mlt * y1;
+      ca = cb - mlt * na;
+      cb = -mlt * nb;
+    }
+  }
+  double* fr = fac + (int64_t)(n - 1) * 3 * fs;
+  fr[0] = ca;
+  fr[fs] = 0.0;
+  fr[2 * fs] = 0.0;
+  Y[(int64_t)(n - 1) * ldy + col] = y0;
+  double x1 = 0.0, x2 = 0.0, nrm = 0.0;
+  for (int i = n - 1; i >= 0; --i) {
+    const double* g = fac + (int64_t)i * 3 * fs;
+    double u0 = g[0];
+    if (fabs(u0) < tiny) u0 = u0 >= 0.0 ? tiny : -tiny;
+    const double xi = (Y[(int64_t)i * ldy + col] - g[fs] * x1 - g[2 * fs] * x2) / u0;
+    Y[(int64_t)i * ldy + col] = xi;
+    x2 = x1;
+    x1 = xi;
+    nrm += xi * xi;
+  }
+  return sqrt(nrm);
+}
+
+__global__ void __launch_bounds__(kT) k_invit(const MolDesc* __restrict__ mols,
+                                               const double* __restrict__ dA,
+                                               const double* __restrict__ eA,
+                                               const double* __restrict__ lamA,
+                                               const double* __restrict__ tnormA,
+                                               double* __restrict__ work, int want_all,
+                                               const int* __restrict__ active,
+                                               const int* __restrict__ status) {
+  extern __shared__ double lds[];
+  const int m = blockIdx.x;
+  if (!mol_live(active, status, m)) return;
+  const MolDesc M = mols[m];
+  const int n = M.n_ao, tid = threadIdx.x, nt = blockDim.x;
+  const int wave = tid >> 6, lane = tid & 63, nwave = nt >> 6;
+  int lo, k;
+  selection(M, want_all, lo, k);
+  if (k == 0) return;
+  double* d = lds;
+  double* e = d + n;
+  double* sh = e + n;                     // shifts, k
+  int* cstart = (int*)(sh + k);           // first vector of each vector's cluster, k
+  int* clist = cstart + k;                // cluster starts, k + 1
+  __shared__ int n_clusters;
+  const double* lam = lamA + M.vec0;
+  const double tnorm = tnormA[m];
+  const double tiny = kEps * fmax(tnorm, 1e-300);
+  const int L = k < nt ? k : nt;          // lanes in use (factor slots)
+  double* Y = work + 5 * M.mat0 + (int64_t)n * n;
+  double* fac = Y + (int64_t)n * n;
+  for (int i = tid; i < n; i += nt) {
+    d[i] = dA[M.vec0 + i];
+    e[i] = eA[M.vec0 + i];
+  }
+  if (tid == 0) {
+    // clusters: consecutive eigenvalues closer than 1e-3 ||T|| (dstein's ORTOL);
+    // equal shifts inside a cluster are separated by a few ulps
+    const double ortol = 1e-3 * tnorm;
+    int nc = 0;
+    for (int j = 0; j < k; ++j) {
+      double s = lam[j];
+      if (j == 0 || lam[j] - lam[j - 1] >= ortol) {
+        clist[nc++] = j;
+      } else {
+        const double pert = 10.0 * kEps * fmax(fabs(s), tnorm);
+        if (s - sh[j - 1] < pert) s = sh[j - 1] + pert;
+      }
+      sh[j] = s;
+      cstart[j] = clist[nc - 1];
+    }
+    clist[nc] = k;
+    n_clusters = nc;
+  }
+  __syncthreads();
+  for (int64_t t = tid; t < (int64_t)n * k; t += nt) {
+    const int row = (int)(t / k), col = (int)(t % k);
+    Y[t] = start_entry(lo + col, row);
+  }
+  __syncthreads();
+  for (int sweep = 0; sweep < 3; ++sweep) {
+    for (int j = tid; j < k; j += nt) {
+      // scale the right-hand side to unit max-norm, then solve
+      double mx = 0.0;
+      for (int i = 0; i < n; ++i) mx = fmax(mx, fabs(Y[(int64_t)i * k + j]));
+      const double s = mx > 0.0 ? 1.0 / mx : 1.0;
+      for (int i = 0; i < n; ++i) Y[(int64_t)i * k + j] *= s;
+      const double nrm = tri_solve(d, e, n, sh[j], tiny, Y, k, j, fac + (j % L), L);
+      const bool single = cstart[j] == j && (j + 1 == k || cstart[j + 1] != j);
+      if (single) {
+        const double inv = 1.0 / nrm;
+        for (int i = 0; i < n; ++i) Y[(int64_t)i * k + j] *= inv;
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+    // clusters of two or more: modified Gram-Schmidt in order, one wave each
+    for (int c = wave; c < n_clusters; c += nwave) {
+      const int c0 = clist[c], c1 = clist[c + 1];
+      if (c1 - c0 < 2) continue;
+      for (int j = c0; j < c1; ++j) {
+        for (int q = c0; q < j; ++q) {
+          double s = 0.0;
+          for (int i = lane; i < n; i += 64) s += Y[(int64_t)i * k + q] * Y[(int64_t)i * k + j];
+          s = wsum(s);
+          for (int i = lane; i < n; i += 64) Y[(int64_t)i * k + j] -= s * Y[(int64_t)i * k + q];
+          __threadfence_block();
+          __builtin_amdgcn_wave_barrier();
+        }
+        double s = 0.0;
+        for (int i = lane; i < n; i += 64) {
+          const double v = Y[(int64_t)i * k + j];
+          s += v * v;
+        }
+        const double inv = 1.0 / sqrt(wsum(s));
+        for (int i = lane; i < n; i += 64) Y[(int64_t)i * k + j] *= inv;
+        __threadfence_block();
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict__ mols,
+                                                       double* __restrict__ work,
+                                                       const double* __restrict__ tauA,
+                                                       int want_all,
+                                                       const int* __restrict__ active,
+                                                       const int* __restrict__ status) {
+  extern __shared__ double lds[];
+  const int m = blockIdx.x;
+  if (!mol_live(active, status, m)) return;
+  const MolDesc M = mols[m];
+  const int n = M.n_ao, tid = threadIdx.x, nt = blockDim.x;
+  const int wave = tid >> 6, lane = tid & 63, nwave = nt >> 6;
+  int lo, k;
+  selection(M, want_all, lo, k);
+  const int nref = n > 2 ? n - 2 : 0;
+  if (k == 0 || nref == 0) return;
+  const int nb = panel_width(n);
+  const double* A = work + 5 * M.mat0;
+  double* Y = work + 5 * M.mat0 + (int64_t)n * n;
+  const double* tau = tauA + M.vec0;
+  double* Vp = lds;                     // nb x n panel (column q = reflector k0+q)
+  double* T = lds + kPanelDoubles / 2;  // nb x nb upper triangular (stride nb)
+  double* G = T + 256;                  // nb x nb Gram matrix V^T V
+  const int npanel = (nref + nb - 1) / nb;
+  for (int pnl = npanel - 1; pnl >= 0; --pnl) {
+    const int k0 = pnl * nb, b = min(nb, nref - k0);
+    for (int t = tid; t < nb * n; t += nt) {
+      const int q = t / n, i = t % n;
+      double v = 0.0;
+      if (q < b) {
+        const int c = k0 + q;
+        v = i < c + 1 ? 0.0 : A[(int64_t)c * n + i];
+      }
+      Vp[t] = v;
+    }
+    __syncthreads();
+    for (int pq = wave; pq < b * b; pq += nwave) {
+      const int p = pq / b, q = pq % b;
+      if (p < q) {
+        double s = 0.0;
+        for (int i = k0 + 1 + lane; i < n; i += 64) s += Vp[p * n + i] * Vp[q * n + i];
+        s = wsum(s);
+        if (lane == 0) G[p * nb + q] = s;
+      }
+    }
+    __syncthreads();
+    for (int t = tid; t < 256; t += nt) T[t] = 0.0;
+    __syncthreads();
+    if (tid == 0) {
+      // forward compact-WY factor: H_k0 ... H_(k0+b-1) = I - V T V^T
+      for (int i = 0; i < b; ++i) {
+        const double ti = tau[k0 + i];
+        T[i * nb + i] = ti;
+        for (int r = 0; r < i; ++r) {
+          double s = 0.0;
+          for (int q = r; q < i; ++q) s += T[r * nb + q] * G[q * nb + i];
+          T[r * nb + i] = -ti * s;
+        }
+      }
+    }
+    __syncthreads();
+    for (int col = tid; col < k; col += nt) {
+      double w1[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) w1[q] = 0.0;
+      for (int i = k0 + 1; i < n; ++i) {
+        const double y = Y[(int64_t)i * k + col];
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+          if (q < b) w1[q] += Vp[q * n + i] * y;
+      }
+      double w2[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+          if (r < b && q >= r && q < b) s += T[r * nb + q] * w1[q];
+        w2[r] = s;
+      }
+      for (int i = k0 + 1; i < n; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+          if (q < b) s += Vp[q * n + i] * w2[q];
+        Y[(int64_t)i * k + col] -= s;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// P from the selected vectors Y (N x k, row-major): 2 Y Y^T (occupied side)
+// or 2 (I - Y Y^T) (virtual side).  16 x 16 output tiles through LDS.
 __global__ void __launch_bounds__(256) k_density(const MolDesc* __restrict__ mols,
-                                                  const double* __restrict__ C,
+                                                  const double* __restrict__ work,
                                                   double* __restrict__ P,
                                                   const int* __restrict__ active,
-                                                  const int* __restrict__ status, int strips) {
+                                                  const int* __restrict__ status) {
   const int m = blockIdx.y;
-  if (active && (!active[m] || status[m])) return;
+  if (!mol_live(active, status, m)) return;
   const MolDesc M = mols[m];
+  const int n = M.n_ao;
+  int lo, k;
+  selection(M, 0, lo, k);
+  const int tiles = (n + 15) / 16;
+  __shared__ double a[16][17], bt[16][17];
+  const double* Y = work + 5 * M.mat0 + (int64_t)n * n;
+  double* p = P + M.mat0;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  for (int tile = blockIdx.x; tile < tiles * tiles; tile += gridDim.x) {
+    const int i0 = (tile / tiles) * 16, j0 = (tile % tiles) * 16;
+    double s = 0.0;
+    for (int c0 = 0; c0 < k; c0 += 16) {
+      const int ra = i0 + ty, rb = j0 + ty, cc = c0 + tx;
+      a[ty][tx] = (ra < n && cc < k) ? Y[(int64_t)ra * k + cc] : 0.0;
+      bt[ty][tx] = (rb < n && cc < k) ? Y[(int64_t)rb * k + cc] : 0.0;
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) s += a[ty][q] * bt[tx][q];
+      __syncthreads();
+    }
+    const int i = i0 + ty, j = j0 + tx;
+    if (i < n && j < n) {
+      const double v = lo == 0 ? 2.0 * s : 2.0 * ((i == j ? 1.0 : 0.0) - s);
+      p[(int64_t)i * n + j] = v;
+    }
+  }
+}
+
+// P = 2 C_occ C_occ^T from the column-major eigenvectors of a full
+// decomposition (sg_eigh followed by sg_density).
+__global__ void __launch_bounds__(256) k_density_from_c(const MolDesc* __restrict__ mols,
+                                                         const double* __restrict__ C,
+                                                         double* __restrict__ P, int strips) {
+  const MolDesc M = mols[blockIdx.y];
   const int n = M.n_ao, no = M.n_occ;
   const double* c = C + M.mat0;
   double* p = P + M.mat0;
-  for (int row = blockIdx.x; row < n; row += strips) {
+  for (int row = blockIdx.x; row < n; row += strips)
     for (int col = threadIdx.x; col < n; col += blockDim.x) {
       double s = 0.0;
       for (int k = 0; k < no; ++k) s += c[(int64_t)k * n + row] * c[(int64_t)k * n + col];
       p[(int64_t)row * n + col] = s * 2.0;
     }
+}
+
+// Full eigendecomposition output for sg_eigh: C column-major, ascending values.
+__global__ void k_write_eigvec(const MolDesc* __restrict__ mols, const double* __restrict__ work,
+                               double* __restrict__ C, const int* __restrict__ active,
+                               const int* __restrict__ status) {
+  const int m = blockIdx.y;
+  if (!mol_live(active, status, m)) return;
+  const MolDesc M = mols[m];
+  const int n = M.n_ao;
+  const double* Y = work + 5 * M.mat0 + (int64_t)n * n;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)n * n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int col = (int)(t / n), row = (int)(t % n);
+    C[M.mat0 + t] = Y[(int64_t)row * n + col];
   }
 }
 
+static size_t tridiag_lds(int max_n) {
+  return sizeof(double) * ((size_t)kPanelDoubles + max_n + 32 + 16);
+}
+
 void launch_eigh(const BatchDev& b, hipStream_t s, bool all) {
-  k_eigh<<<b.n_mol, kEigThreads, 0, s>>>(b.mols, b.F, b.W, b.C, b.eig, all ? nullptr : b.active,
-                                         b.status);
+  const int* act = all ? nullptr : b.active;
+  const int want_all = all ? 1 : 0;
+  const int N = b.max_n;
+  k_tridiag<<<b.n_mol, kT, tridiag_lds(N), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status);
+  dim3 gb((N + 255) / 256, b.n_mol);
+  k_bisect<<<gb, 256, sizeof(double) * (2 * N + 32), s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm,
+                                                          want_all, act, b.status);
+  const size_t inv_lds = sizeof(double) * (3 * (size_t)N) + sizeof(int) * (2 * (size_t)N + 2);
+  k_invit<<<b.n_mol, kT, inv_lds, s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm, b.W, want_all, act,
+                                       b.status);
+  const size_t bt_lds = sizeof(double) * ((size_t)kPanelDoubles / 2 + 2 * 256);
+  k_backtransform<<<b.n_mol, kT, bt_lds, s>>>(b.mols, b.W, b.tau, want_all, act, b.status);
+  if (all) {
+    dim3 gw(64, b.n_mol);
+    k_write_eigvec<<<gw, 256, 0, s>>>(b.mols, b.W, b.C, act, b.status);
+  }
 }
 
 void launch_density(const BatchDev& b, hipStream_t s, bool all) {
-  const int strips = 64;
-  dim3 grid(strips, b.n_mol);
-  k_density<<<grid, 256, 0, s>>>(b.mols, b.C, b.P, all ? nullptr : b.active, b.status, strips);
+  dim3 grid(64, b.n_mol);
+  if (all)
+    k_density_from_c<<<grid, 256, 0, s>>>(b.mols, b.C, b.P, 64);
+  else
+    k_density<<<grid, 256, 0, s>>>(b.mols, b.W, b.P, b.active, b.status);
 }
 
 }  // namespace sg
