@@ -39,7 +39,8 @@
 namespace sg {
 
 constexpr int kT = 512;                 // threads per eigensolver workgroup
-constexpr int kPanelLds = 96 * 1024;    // bytes of LDS for the V/W panel
+constexpr int kPanelLds = 48 * 1024;    // bytes of LDS for the V/W panel
+constexpr int kNbMax = 8;               // widest reflector panel
 constexpr int kPanelDoubles = kPanelLds / 8;
 constexpr double kEps = 2.220446049250313e-16;
 
@@ -82,11 +83,15 @@ __device__ __forceinline__ void selection(const MolDesc& M, int want_all, int& l
 
 __device__ __forceinline__ int panel_width(int n) {
   int nb = kPanelLds / (16 * (n > 0 ? n : 1));
-  return nb < 1 ? 1 : (nb > 16 ? 16 : nb);
+  return nb < 1 ? 1 : (nb > kNbMax ? kNbMax : nb);
 }
 
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kT) k_tridiag(const MolDesc* __restrict__ mols,
+// Lower-triangle storage: only A[i][l], l <= i, is read and updated; reflector
+// c is kept in the (otherwise unused) upper part of row c.  TMAX bounds the
+// number of 64-column chunks of a row (n <= 64*TMAX).
+template <int TMAX>
+__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_tridiag(const MolDesc* __restrict__ mols,
                                                  const double* __restrict__ F,
                                                  double* __restrict__ work,
                                                  double* __restrict__ dA, double* __restrict__ eA,
@@ -100,11 +105,12 @@ __global__ void __launch_bounds__(kT) k_tridiag(const MolDesc* __restrict__ mols
   const int n = M.n_ao, tid = threadIdx.x, nt = blockDim.x;
   const int wave = tid >> 6, lane = tid & 63, nwave = nt >> 6;
   const int nb = panel_width(n);
-  double* V = lds;                  // nb columns of length n (column-major)
-  double* Wp = V + (size_t)nb * n;   // nb columns of length n; 2*nb*n <= kPanelDoubles
-  double* x = lds + kPanelDoubles;   // n
-  double* tv = x + n;               // 2 * nb <= 32
-  double* red = tv + 32;            // nwave
+  double* V = lds;                   // nb columns of length n (column-major)
+  double* Wp = V + (size_t)nb * n;    // nb columns of length n; 2*nb*n <= kPanelDoubles
+  double* x = lds + kPanelDoubles;    // n
+  double* tv = x + n;                // 2 * nb <= 16
+  double* red = tv + 16;             // nwave
+  double* qb = red + 16;             // nwave x n partial transposed products
   double* A = work + 5 * M.mat0;
   double* d = dA + M.vec0;
   double* e = eA + M.vec0;
@@ -122,9 +128,9 @@ __global__ void __launch_bounds__(kT) k_tridiag(const MolDesc* __restrict__ mols
     __syncthreads();
     for (int j = 0; j < b; ++j) {
       const int c = k0 + j;
-      // column c of the partially reduced matrix (rows c..n-1)
+      // column c of the partially reduced matrix (rows c..n-1, lower part)
       for (int i = c + tid; i < n; i += nt) {
-        double s = A[(int64_t)c * n + i];
+        double s = A[(int64_t)i * n + c];
         for (int q = 0; q < j; ++q)
           s -= V[q * n + i] * Wp[q * n + c] + Wp[q * n + i] * V[q * n + c];
         x[i] = s;
@@ -157,27 +163,70 @@ __global__ void __launch_bounds__(kT) k_tridiag(const MolDesc* __restrict__ mols
         e[c] = beta;
       }
       __syncthreads();
-      // p = A22 v over the not yet updated trailing block: one wave per row
-      for (int i = c + 1 + wave; i < n; i += nwave) {
-        const double* ri = A + (int64_t)i * n;
-        double s = 0.0;
-        for (int l = c + 1 + lane; l < n; l += 64) s += ri[l] * V[j * n + l];
-        s = wsum(s);
-        if (lane == 0) x[i] = s;
+      // p = A22 v from the lower triangle of the trailing block.  A wave takes
+      // 8 consecutive rows: row parts are dotted with v, and each row also adds
+      // A[i][l] v_i to lane-owned accumulators for column l < i (the upper half).
+      {
+        const int m1 = c + 1;
+        const int T = (n - m1 + 63) >> 6;
+        const double* vj = V + j * n;
+        double q[TMAX];
+#pragma unroll
+        for (int t = 0; t < TMAX; ++t) q[t] = 0.0;
+        for (int i0 = m1 + wave * 8; i0 < n; i0 += nwave * 8) {
+          double acc[8], vi[8];
+#pragma unroll
+          for (int r = 0; r < 8; ++r) {
+            acc[r] = 0.0;
+            vi[r] = i0 + r < n ? vj[i0 + r] : 0.0;
+          }
+          const int tmax = (min(i0 + 7, n - 1) - m1) >> 6;
+#pragma unroll
+          for (int t = 0; t < TMAX; ++t) {
+            if (t <= tmax) {
+              const int l = m1 + lane + 64 * t;
+              const double vl = l < n ? vj[l] : 0.0;
+              double a[8];
+#pragma unroll
+              for (int r = 0; r < 8; ++r) {
+                const int i = i0 + r;
+                a[r] = (i < n && l <= i) ? A[(int64_t)i * n + l] : 0.0;
+              }
+              double qs = 0.0;
+#pragma unroll
+              for (int r = 0; r < 8; ++r) {
+                acc[r] += a[r] * vl;
+                if (l < i0 + r) qs += a[r] * vi[r];
+              }
+              q[t] += qs;
+            }
+          }
+#pragma unroll
+          for (int r = 0; r < 8; ++r) {
+            const double sr = wsum(acc[r]);
+            if (lane == r && i0 + r < n) x[i0 + r] = sr;
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < TMAX; ++t) {
+          const int l = m1 + lane + 64 * t;
+          if (t < T && l < n) qb[wave * n + l] = q[t];
+        }
       }
       // W^T v and V^T v for the previous columns of the panel
-      for (int q = wave; q < 2 * j; q += nwave) {
-        const double* col = (q < j) ? (Wp + q * n) : (V + (q - j) * n);
+      for (int qq = wave; qq < 2 * j; qq += nwave) {
+        const double* col = (qq < j) ? (Wp + qq * n) : (V + (qq - j) * n);
         double s = 0.0;
         for (int i = c + 1 + lane; i < n; i += 64) s += col[i] * V[j * n + i];
         s = wsum(s);
-        if (lane == 0) tv[q] = s;
+        if (lane == 0) tv[qq] = s;
       }
       __syncthreads();
       double pv = 0.0;
       for (int i = c + 1 + tid; i < n; i += nt) {
         double s = x[i];
-        for (int q = 0; q < j; ++q) s -= V[q * n + i] * tv[q] + Wp[q * n + i] * tv[j + q];
+        for (int w = 0; w < nwave; ++w) s += qb[w * n + i];
+        for (int qq = 0; qq < j; ++qq) s -= V[qq * n + i] * tv[qq] + Wp[qq * n + i] * tv[j + qq];
         s *= tk;
         x[i] = s;
         pv += s * V[j * n + i];
@@ -187,13 +236,36 @@ __global__ void __launch_bounds__(kT) k_tridiag(const MolDesc* __restrict__ mols
       for (int i = c + 1 + tid; i < n; i += nt) Wp[j * n + i] = x[i] + al * V[j * n + i];
       __syncthreads();
     }
-    // trailing update of the full symmetric block (rows, cols >= k0 + b)
-    const int r = k0 + b, mm = n - r;
-    for (int64_t t = tid; t < (int64_t)mm * mm; t += nt) {
-      const int i = r + (int)(t / mm), l = r + (int)(t % mm);
-      double s = A[(int64_t)i * n + l];
-      for (int q = 0; q < b; ++q) s -= V[q * n + i] * Wp[q * n + l] + Wp[q * n + i] * V[q * n + l];
-      A[(int64_t)i * n + l] = s;
+    // trailing update of the lower triangle (rows >= k0 + b, columns r..i):
+    // one wave per row, lanes over columns, 4 column chunks per pass
+    const int r = k0 + b;
+    for (int i = r + wave; i < n; i += nwave) {
+      double vi[kNbMax], wi[kNbMax];
+#pragma unroll
+      for (int qq = 0; qq < kNbMax; ++qq) {
+        vi[qq] = qq < b ? V[qq * n + i] : 0.0;
+        wi[qq] = qq < b ? Wp[qq * n + i] : 0.0;
+      }
+      double* ai = A + (int64_t)i * n;
+      for (int l0 = r + lane; l0 <= i; l0 += 256) {
+        double a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int l = l0 + 64 * u;
+          a[u] = l <= i ? ai[l] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int l = l0 + 64 * u;
+          if (l <= i) {
+            double sacc = a[u];
+#pragma unroll
+            for (int qq = 0; qq < kNbMax; ++qq)
+              if (qq < b) sacc -= vi[qq] * Wp[qq * n + l] + wi[qq] * V[qq * n + l];
+            ai[l] = sacc;
+          }
+        }
+      }
     }
     __syncthreads();
   }
@@ -304,58 +376,90 @@ __device__ __forceinline__ double start_entry(int vec, int row) {
   return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
-// Solves (T - lam I) y = y in place (y column `col` of Y, row stride ldy) by
+// Solves (T - lam I) y = y in place (y = column `col` of Y, row stride ldy) by
 // Gaussian elimination with partial pivoting; fac holds 3 doubles per row for
-// this lane (stride fs between rows).  Returns the 2-norm of the solution.
+// this lane (stride fs between components).  Loads are issued ahead of
+// the recurrence (4 rows).  Returns the 2-norm of the solution.  The right-hand side
+// is not rescaled: pivots below `tiny` are raised to it, so each solve grows
+// the vector by at most ~1/tiny and three sweeps stay far from overflow.
 __device__ double tri_solve(const double* __restrict__ d, const double* __restrict__ e, int n,
                             double lam, double tiny, double* __restrict__ Y, int ldy, int col,
                             double* __restrict__ fac, int fs) {
   double ca = d[0] - lam, cb = n > 1 ? e[0] : 0.0;
   double y0 = Y[col];
-  for (int i = 0; i + 1 < n; ++i) {
-    const double sub = e[i], na = d[i + 1] - lam, nb = i + 2 < n ? e[i + 1] : 0.0;
-    const double y1 = Y[(int64_t)(i + 1) * ldy + col];
-    double* fr = fac + (int64_t)i * 3 * fs;
-    if (fabs(ca) >= fabs(sub)) {
-      const double mlt = ca != 0.0 ? sub / ca : 0.0;
-      fr[0] = ca;
-      fr[fs] = cb;
-      fr[2 * fs] = 0.0;
-      Y[(int64_t)i * ldy + col] = y0;
-      y0 = y1 - mlt * y0;
-      ca = na - mlt * cb;
-      cb = nb;
-    } else {
-      const double mlt = ca / sub;
-      fr[0] = sub;
-      fr[fs] = na;
-      fr[2 * fs] = nb;
-      Y[(int64_t)i * ldy + col] = y1;
-      y0 = y0 - mlt * y1;
-      ca = cb - mlt * na;
-      cb = -mlt * nb;
+  for (int i0 = 0; i0 < n - 1; i0 += 4) {
+    double yb[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + 1 + r;
+      yb[r] = i < n ? Y[(int64_t)i * ldy + col] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + r;
+      if (i < n - 1) {
+        const double sub = e[i], na = d[i + 1] - lam, nb = i + 2 < n ? e[i + 1] : 0.0;
+        const double y1 = yb[r];
+        double* fr = fac + (int64_t)i * 3 * fs;
+        if (fabs(ca) >= fabs(sub)) {
+          const double mlt = ca != 0.0 ? sub / ca : 0.0;
+          fr[0] = ca;
+          fr[fs] = cb;
+          fr[2 * fs] = 0.0;
+          Y[(int64_t)i * ldy + col] = y0;
+          y0 = y1 - mlt * y0;
+          ca = na - mlt * cb;
+          cb = nb;
+        } else {
+          const double mlt = ca / sub;
+          fr[0] = sub;
+          fr[fs] = na;
+          fr[2 * fs] = nb;
+          Y[(int64_t)i * ldy + col] = y1;
+          y0 = y0 - mlt * y1;
+          ca = cb - mlt * na;
+          cb = -mlt * nb;
+        }
+      }
     }
   }
-  double* fr = fac + (int64_t)(n - 1) * 3 * fs;
-  fr[0] = ca;
-  fr[fs] = 0.0;
-  fr[2 * fs] = 0.0;
-  Y[(int64_t)(n - 1) * ldy + col] = y0;
+  {
+    double* fr = fac + (int64_t)(n - 1) * 3 * fs;
+    fr[0] = ca;
+    fr[fs] = 0.0;
+    fr[2 * fs] = 0.0;
+    Y[(int64_t)(n - 1) * ldy + col] = y0;
+  }
   double x1 = 0.0, x2 = 0.0, nrm = 0.0;
-  for (int i = n - 1; i >= 0; --i) {
-    const double* g = fac + (int64_t)i * 3 * fs;
-    double u0 = g[0];
-    if (fabs(u0) < tiny) u0 = u0 >= 0.0 ? tiny : -tiny;
-    const double xi = (Y[(int64_t)i * ldy + col] - g[fs] * x1 - g[2 * fs] * x2) / u0;
-    Y[(int64_t)i * ldy + col] = xi;
-    x2 = x1;
-    x1 = xi;
-    nrm += xi * xi;
+  for (int i0 = n - 1; i0 >= 0; i0 -= 4) {
+    double g0[4], g1[4], g2[4], yv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 - r >= 0 ? i0 - r : 0;
+      const double* g = fac + (int64_t)i * 3 * fs;
+      g0[r] = g[0];
+      g1[r] = g[fs];
+      g2[r] = g[2 * fs];
+      yv[r] = Y[(int64_t)i * ldy + col];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 - r;
+      if (i >= 0) {
+        double u0 = g0[r];
+        if (fabs(u0) < tiny) u0 = u0 >= 0.0 ? tiny : -tiny;
+        const double xi = (yv[r] - g1[r] * x1 - g2[r] * x2) / u0;
+        Y[(int64_t)i * ldy + col] = xi;
+        x2 = x1;
+        x1 = xi;
+        nrm += xi * xi;
+      }
+    }
   }
   return sqrt(nrm);
 }
 
-__global__ void __launch_bounds__(kT) k_invit(const MolDesc* __restrict__ mols,
+__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_invit(const MolDesc* __restrict__ mols,
                                                const double* __restrict__ dA,
                                                const double* __restrict__ eA,
                                                const double* __restrict__ lamA,
@@ -389,9 +493,12 @@ __global__ void __launch_bounds__(kT) k_invit(const MolDesc* __restrict__ mols,
     e[i] = eA[M.vec0 + i];
   }
   if (tid == 0) {
-    // clusters: consecutive eigenvalues closer than 1e-3 ||T|| (dstein's ORTOL);
-    // equal shifts inside a cluster are separated by a few ulps
-    const double ortol = 1e-3 * tnorm;
+    // clusters: consecutive eigenvalues closer than ortol.  The full
+    // decomposition uses dstein's 1e-3 ||T||; the SCF path, which only needs
+    // the subspace, uses 1e-5 ||T|| (vectors of eigenvalues further apart than
+    // that are orthogonal to ~eps ||T|| / gap <= 1e-11 without reorthogonalising).
+    // Equal shifts inside a cluster are separated by a few ulps.
+    const double ortol = (want_all ? 1e-3 : 1e-5) * tnorm;
     int nc = 0;
     for (int j = 0; j < k; ++j) {
       double s = lam[j];
@@ -415,16 +522,18 @@ __global__ void __launch_bounds__(kT) k_invit(const MolDesc* __restrict__ mols,
   __syncthreads();
   for (int sweep = 0; sweep < 3; ++sweep) {
     for (int j = tid; j < k; j += nt) {
-      // scale the right-hand side to unit max-norm, then solve
-      double mx = 0.0;
-      for (int i = 0; i < n; ++i) mx = fmax(mx, fabs(Y[(int64_t)i * k + j]));
-      const double s = mx > 0.0 ? 1.0 / mx : 1.0;
-      for (int i = 0; i < n; ++i) Y[(int64_t)i * k + j] *= s;
       const double nrm = tri_solve(d, e, n, sh[j], tiny, Y, k, j, fac + (j % L), L);
       const bool single = cstart[j] == j && (j + 1 == k || cstart[j + 1] != j);
-      if (single) {
+      if (single && sweep == 2) {
         const double inv = 1.0 / nrm;
-        for (int i = 0; i < n; ++i) Y[(int64_t)i * k + j] *= inv;
+        for (int i0 = 0; i0 < n; i0 += 8) {
+          double v[8];
+#pragma unroll
+          for (int r = 0; r < 8; ++r) v[r] = i0 + r < n ? Y[(int64_t)(i0 + r) * k + j] : 0.0;
+#pragma unroll
+          for (int r = 0; r < 8; ++r)
+            if (i0 + r < n) Y[(int64_t)(i0 + r) * k + j] = v[r] * inv;
+        }
       }
     }
     __threadfence_block();
@@ -481,7 +590,7 @@ __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict_
   const double* tau = tauA + M.vec0;
   double* Vp = lds;                     // nb x n panel (column q = reflector k0+q)
   double* T = lds + kPanelDoubles / 2;  // nb x nb upper triangular (stride nb)
-  double* G = T + 256;                  // nb x nb Gram matrix V^T V
+  double* G = T + kNbMax * kNbMax;      // nb x nb Gram matrix V^T V
   const int npanel = (nref + nb - 1) / nb;
   for (int pnl = npanel - 1; pnl >= 0; --pnl) {
     const int k0 = pnl * nb, b = min(nb, nref - k0);
@@ -505,7 +614,7 @@ __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict_
       }
     }
     __syncthreads();
-    for (int t = tid; t < 256; t += nt) T[t] = 0.0;
+    for (int t = tid; t < kNbMax * kNbMax; t += nt) T[t] = 0.0;
     __syncthreads();
     if (tid == 0) {
       // forward compact-WY factor: H_k0 ... H_(k0+b-1) = I - V T V^T
@@ -521,30 +630,46 @@ __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict_
     }
     __syncthreads();
     for (int col = tid; col < k; col += nt) {
-      double w1[16];
+      double w1[kNbMax];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) w1[q] = 0.0;
-      for (int i = k0 + 1; i < n; ++i) {
-        const double y = Y[(int64_t)i * k + col];
+      for (int q = 0; q < kNbMax; ++q) w1[q] = 0.0;
+      for (int i0 = k0 + 1; i0 < n; i0 += 4) {
+        double yv[4];
 #pragma unroll
-        for (int q = 0; q < 16; ++q)
-          if (q < b) w1[q] += Vp[q * n + i] * y;
+        for (int u = 0; u < 4; ++u) yv[u] = i0 + u < n ? Y[(int64_t)(i0 + u) * k + col] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int i = i0 + u < n ? i0 + u : k0 + 1;
+          const double y = yv[u];
+#pragma unroll
+          for (int q = 0; q < kNbMax; ++q)
+            if (q < b) w1[q] += Vp[q * n + i] * y;
+        }
       }
-      double w2[16];
+      double w2[kNbMax];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
+      for (int r = 0; r < kNbMax; ++r) {
         double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < 16; ++q)
+        for (int q = 0; q < kNbMax; ++q)
           if (r < b && q >= r && q < b) s += T[r * nb + q] * w1[q];
         w2[r] = s;
       }
-      for (int i = k0 + 1; i < n; ++i) {
-        double s = 0.0;
+      for (int i0 = k0 + 1; i0 < n; i0 += 4) {
+        double yv[4];
 #pragma unroll
-        for (int q = 0; q < 16; ++q)
-          if (q < b) s += Vp[q * n + i] * w2[q];
-        Y[(int64_t)i * k + col] -= s;
+        for (int u = 0; u < 4; ++u) yv[u] = i0 + u < n ? Y[(int64_t)(i0 + u) * k + col] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int i = i0 + u;
+          if (i < n) {
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < kNbMax; ++q)
+              if (q < b) s += Vp[q * n + i] * w2[q];
+            Y[(int64_t)i * k + col] = yv[u] - s;
+          }
+        }
       }
     }
     __syncthreads();
@@ -624,21 +749,24 @@ __global__ void k_write_eigvec(const MolDesc* __restrict__ mols, const double* _
 }
 
 static size_t tridiag_lds(int max_n) {
-  return sizeof(double) * ((size_t)kPanelDoubles + max_n + 32 + 16);
+  return sizeof(double) * ((size_t)kPanelDoubles + max_n + 32 + (size_t)(kT / 64) * max_n);
 }
 
 void launch_eigh(const BatchDev& b, hipStream_t s, bool all) {
   const int* act = all ? nullptr : b.active;
   const int want_all = all ? 1 : 0;
   const int N = b.max_n;
-  k_tridiag<<<b.n_mol, kT, tridiag_lds(N), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status);
+  if (N <= 384)
+    k_tridiag<6><<<b.n_mol, kT, tridiag_lds(N), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status);
+  else
+    k_tridiag<16><<<b.n_mol, kT, tridiag_lds(N), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status);
   dim3 gb((N + 255) / 256, b.n_mol);
   k_bisect<<<gb, 256, sizeof(double) * (2 * N + 32), s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm,
                                                           want_all, act, b.status);
   const size_t inv_lds = sizeof(double) * (3 * (size_t)N) + sizeof(int) * (2 * (size_t)N + 2);
   k_invit<<<b.n_mol, kT, inv_lds, s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm, b.W, want_all, act,
                                        b.status);
-  const size_t bt_lds = sizeof(double) * ((size_t)kPanelDoubles / 2 + 2 * 256);
+  const size_t bt_lds = sizeof(double) * ((size_t)kPanelDoubles / 2 + 2 * kNbMax * kNbMax);
   k_backtransform<<<b.n_mol, kT, bt_lds, s>>>(b.mols, b.W, b.tau, want_all, act, b.status);
   if (all) {
     dim3 gw(64, b.n_mol);
