@@ -85,6 +85,17 @@ int sg_energy(sg_batch* b, double* e_el);
  * the reference.  results[n_mol]. */
 int sg_scf_run(sg_batch* b, double threshold_ev, int max_steps, sg_scf_result* results);
 
+/* Same SCF with a precision policy (no counterpart in the reference, which is
+ * FP64 throughout).  SG_POLICY_FP64 is sg_scf_run.  SG_POLICY_MIXED runs the
+ * Fock contractions and the density product in FP32 (density on the matrix
+ * cores) until a molecule's |dE| < switch_ev (1e-3 eV suggested), then in FP64
+ * to convergence; integrals, eigensolver and energies stay FP64, and
+ * convergence is only tested between two FP64 steps. */
+#define SG_POLICY_FP64 0
+#define SG_POLICY_MIXED 1
+int sg_scf_run_policy(sg_batch* b, double threshold_ev, int max_steps, int policy,
+                      double switch_ev, sg_scf_result* results);
+
 /* Host copies for the reference's host-side views. */
 long sg_pair_block_count(const sg_batch* b, int mol);
 /* double**** all2CenterIntegral contents in the reference jagged order

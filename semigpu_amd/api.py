@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 from typing import List, Sequence
 
 import numpy as np
@@ -28,6 +29,7 @@ EXPORTS = (
     "sg_last_error", "sg_version", "sg_ctx_create", "sg_ctx_destroy", "sg_batch_create",
     "sg_batch_destroy", "sg_batch_molecule_info", "sg_two_center", "sg_hcore", "sg_core_core",
     "sg_guess_density", "sg_fock", "sg_eigh", "sg_density", "sg_energy", "sg_scf_run",
+    "sg_scf_run_policy",
     "sg_pair_block_count", "sg_get_pair_blocks", "sg_get_packed", "sg_set_packed",
     "sg_get_eigen", "sg_timing_enable", "sg_timing_read", "sg_timing_reset",
 )
@@ -69,6 +71,8 @@ def load_library(path: str = LIB_PATH):
     L.sg_core_core.argtypes = [_vp, _dp]
     L.sg_energy.argtypes = [_vp, _dp]
     L.sg_scf_run.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ScfResult)]
+    L.sg_scf_run_policy.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_double, ctypes.POINTER(ScfResult)]
     L.sg_pair_block_count.argtypes = [_vp, ctypes.c_int]
     L.sg_pair_block_count.restype = ctypes.c_long
     L.sg_get_pair_blocks.argtypes = [_vp, ctypes.c_int, _dp]
@@ -104,9 +108,13 @@ class Engine:
         _check(L.sg_ctx_create(device, ctypes.byref(h)))
         self.h = h
         self.L = L
+        self._batches = weakref.WeakSet()
 
     def close(self):
+        """Destroys the context; batches still alive are destroyed first."""
         if self.h:
+            for b in list(self._batches):
+                b.close()
             self.L.sg_ctx_destroy(self.h)
             self.h = None
 
@@ -154,6 +162,7 @@ class Batch:
         _check(self.L.sg_batch_create(engine.h, self.n_mol, _i(self.offsets), _i(self.Z),
                                       _d(self.xyz), ctypes.byref(h)))
         self.h = h
+        engine._batches.add(self)
         self.info = []
         for m in range(self.n_mol):
             na, nao, nocc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -204,9 +213,12 @@ class Batch:
         _check(self.L.sg_energy(self.h, _d(out)))
         return out
 
-    def scf(self, threshold: float = 5e-3, max_steps: int = 100):
+    def scf(self, threshold: float = 5e-3, max_steps: int = 100, policy: str = "fp64",
+            switch_ev: float = 1e-3):
+        """SCF for every molecule; policy "fp64" (the reference) or "mixed"."""
         res = (ScfResult * self.n_mol)()
-        _check(self.L.sg_scf_run(self.h, threshold, max_steps, res))
+        pol = {"fp64": 0, "mixed": 1}[policy]
+        _check(self.L.sg_scf_run_policy(self.h, threshold, max_steps, pol, switch_ev, res))
         return [{"e_el": r.e_el, "e_cc": r.e_cc, "e_tot": r.e_tot, "steps": r.steps,
                  "converged": bool(r.converged), "status": r.status} for r in res]
 

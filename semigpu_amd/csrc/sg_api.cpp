@@ -48,7 +48,7 @@ struct sg_batch {
   std::vector<MolDesc> mols;
   std::vector<AtomDesc> atoms;
   std::vector<int64_t> blk_count;  // doubles of pair blocks per molecule
-  bool have_ints = false, have_hcore = false, have_ecc = false;
+  bool have_ints = false, have_hcore = false, have_ecc = false, have_fp32 = false;
   std::vector<double> e_cc;
 };
 
@@ -252,7 +252,7 @@ int sg_batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, const i
   const size_t o_xx = take(sizeof(int2) * pxx.size()), o_hx = take(sizeof(int2) * phx.size()),
                o_hh = take(sizeof(int2) * phh.size()), o_all = take(sizeof(int2) * pall.size()),
                o_diag = take(sizeof(int2) * diag.size());
-  const size_t o_blk = take(sizeof(double) * blk0);
+  const size_t o_blk = take(sizeof(double) * blk0), o_blk32 = take(sizeof(float) * blk0);
   const size_t o_H = take(sizeof(double) * mat0), o_F = take(sizeof(double) * mat0),
                o_P = take(sizeof(double) * mat0), o_C = take(sizeof(double) * mat0),
                o_W = take(sizeof(double) * 5 * mat0);
@@ -263,7 +263,8 @@ int sg_batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, const i
                o_enew = take(sizeof(double) * n_mol);
   const size_t o_steps = take(sizeof(int) * n_mol), o_act = take(sizeof(int) * n_mol),
                o_conv = take(sizeof(int) * n_mol), o_stat = take(sizeof(int) * n_mol),
-               o_nact = take(sizeof(int) * 64);
+               o_nact = take(sizeof(int) * 64), o_prec = take(sizeof(int) * n_mol),
+               o_pprec = take(sizeof(int) * n_mol);
   (void)hipSetDevice(ctx->device);
   hipError_t e = hipMalloc(&b->dev_mem, off);
   if (e != hipSuccess) {
@@ -280,6 +281,7 @@ int sg_batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, const i
   d.pairs_all = (int2*)(base + o_all);
   d.diag_tasks = (int2*)(base + o_diag);
   d.blocks = (double*)(base + o_blk);
+  d.blocks32 = (float*)(base + o_blk32);
   d.H = (double*)(base + o_H);
   d.F = (double*)(base + o_F);
   d.P = (double*)(base + o_P);
@@ -298,6 +300,8 @@ int sg_batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, const i
   d.converged = (int*)(base + o_conv);
   d.status = (int*)(base + o_stat);
   d.n_active = (int*)(base + o_nact);
+  d.prec = (int*)(base + o_prec);
+  d.prev_prec = (int*)(base + o_pprec);
   hipStream_t s = ctx->stream;
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
@@ -313,6 +317,8 @@ int sg_batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, const i
     if (e == hipSuccess) e = hipMemsetAsync(z, 0, sizeof(double) * mat0, s);
   if (e == hipSuccess) e = hipMemsetAsync(d.status, 0, sizeof(int) * n_mol, s);
   if (e == hipSuccess) e = hipMemsetAsync(d.active, 0, sizeof(int) * n_mol, s);
+  if (e == hipSuccess) e = hipMemsetAsync(d.prec, 0, sizeof(int) * n_mol, s);
+  if (e == hipSuccess) e = hipMemsetAsync(d.prev_prec, 0, sizeof(int) * n_mol, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     (void)hipFree(b->dev_mem);
@@ -351,6 +357,7 @@ int sg_two_center(sg_batch* b) {
   int r = check_launch("two-center integrals");
   if (r) return r;
   b->have_ints = true;
+  b->have_fp32 = false;
   return SG_OK;
 }
 
@@ -446,7 +453,15 @@ int sg_energy(sg_batch* b, double* e_el) {
 }
 
 int sg_scf_run(sg_batch* b, double threshold, int max_steps, sg_scf_result* res) {
-  if (!b || max_steps < 0 || !(threshold >= 0.0)) return fail(SG_ERR_INVALID, "sg_scf_run: bad arguments");
+  return sg_scf_run_policy(b, threshold, max_steps, SG_POLICY_FP64, 1e-3, res);
+}
+
+int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, double switch_ev,
+                      sg_scf_result* res) {
+  if (!b || max_steps < 0 || !(threshold >= 0.0) || (policy != SG_POLICY_FP64 &&
+                                                     policy != SG_POLICY_MIXED) ||
+      !(switch_ev >= 0.0))
+    return fail(SG_ERR_INVALID, "sg_scf_run: bad arguments");
   int r;
   if (!b->have_ints && (r = sg_two_center(b))) return r;
   if (!b->have_hcore && (r = sg_hcore(b))) return r;
@@ -454,7 +469,12 @@ int sg_scf_run(sg_batch* b, double threshold, int max_steps, sg_scf_result* res)
   sg_context* c = b->ctx;
   hipStream_t s = c->stream;
   HIP_TRY(hipSetDevice(c->device));
-  // InitialGuess + E0 (scfcalculation.cpp:66-72, 148-165)
+  if (policy == SG_POLICY_MIXED && !b->have_fp32) {
+    StageTimer t(c, ST_TWO_CENTER);
+    launch_blocks_to_fp32(b->d, s);
+    b->have_fp32 = true;
+  }
+  // InitialGuess + E0 (scfcalculation.cpp:66-72, 148-165), in FP64
   {
     StageTimer t(c, ST_GUESS);
     launch_guess_density(b->d, s);
@@ -469,11 +489,13 @@ int sg_scf_run(sg_batch* b, double threshold, int max_steps, sg_scf_result* res)
   }
   {
     StageTimer t(c, ST_CONTROL);
-    launch_scf_init(b->d, s, max_steps);
+    launch_set_prec(b->d, s, policy == SG_POLICY_MIXED ? 1 : 0);
+    launch_scf_init(b->d, s, max_steps, policy);
   }
   if ((r = check_launch("scf init"))) return r;
   int n_active = max_steps > 0 ? b->d.n_mol : 0;
   int host_active = 0;
+  const int fp32_cap = 40;
   for (int it = 0; it < max_steps && n_active > 0; ++it) {
     {
       StageTimer t(c, ST_EIGH);
@@ -494,7 +516,7 @@ int sg_scf_run(sg_batch* b, double threshold, int max_steps, sg_scf_result* res)
     {
       StageTimer t(c, ST_CONTROL);
       HIP_TRY(hipMemsetAsync(b->d.n_active, 0, sizeof(int), s));
-      launch_scf_update(b->d, s, threshold, max_steps);
+      launch_scf_update(b->d, s, threshold, max_steps, switch_ev, fp32_cap);
     }
     if ((r = check_launch("scf step"))) return r;
     HIP_TRY(hipMemcpyAsync(&host_active, b->d.n_active, sizeof(int), hipMemcpyDeviceToHost, s));

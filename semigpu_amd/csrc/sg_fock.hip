@@ -33,8 +33,8 @@ struct Slot {
 };
 
 // Entry (A slot ka | B slot kb) of the block shared by A and B.
-template <bool HA, bool HB, bool ROW>
-__device__ __forceinline__ double blk_entry(const double* __restrict__ b, int ka, int kb) {
+template <bool HA, bool HB, bool ROW, typename S>
+__device__ __forceinline__ S blk_entry(const S* __restrict__ b, int ka, int kb) {
   if constexpr (HA && HB) return ROW ? b[ka * 10 + kb] : b[kb * 10 + ka];
   else if constexpr (HA) return b[ka];
   else if constexpr (HB) return b[kb];
@@ -42,26 +42,27 @@ __device__ __forceinline__ double blk_entry(const double* __restrict__ b, int ka
 }
 
 // One partner B for atom A.  ROW: A is the block's row atom (B.local <= A.local),
-// which is also when this lane owns the exchange block K_AB.
-template <bool HA, bool HB, bool ROW>
-__device__ __forceinline__ void partner(const double* __restrict__ b, const double* __restrict__ p,
-                                        int N, int ao_a, int ao_b, double (&J)[10],
-                                        double (&K)[4][4]) {
+// which is also when this lane owns the exchange block K_AB.  T is the
+// arithmetic type of the contraction (double, or float in the FP32 phase of the
+// mixed-precision policy, where S = float reads the FP32 copy of the blocks).
+template <bool HA, bool HB, bool ROW, typename T, typename S>
+__device__ __forceinline__ void partner(const S* __restrict__ b, const double* __restrict__ p,
+                                        int N, int ao_a, int ao_b, T (&J)[10], T (&K)[4][4]) {
   constexpr int NA = HA ? 4 : 1, NB = HB ? 4 : 1;
   constexpr int SA = HA ? 10 : 1, SB = HB ? 10 : 1;
-  double w[SB];
+  T w[SB];
 #pragma unroll
   for (int l = 0; l < SB; ++l) {
     const int x = mop_a(l), y = mop_b(l);
-    const double v = p[(int64_t)(ao_b + x) * N + ao_b + y];
-    w[l] = x == y ? v : 2.0 * v;
+    const T v = (T)p[(int64_t)(ao_b + x) * N + ao_b + y];
+    w[l] = x == y ? v : (T)2 * v;
   }
-  double pab[NA][NB];
+  T pab[NA][NB];
   if constexpr (ROW) {
 #pragma unroll
     for (int x = 0; x < NA; ++x)
 #pragma unroll
-      for (int y = 0; y < NB; ++y) pab[x][y] = p[(int64_t)(ao_a + x) * N + ao_b + y];
+      for (int y = 0; y < NB; ++y) pab[x][y] = (T)p[(int64_t)(ao_a + x) * N + ao_b + y];
   }
 #pragma unroll
   for (int k = 0; k < SA; ++k) {
@@ -69,7 +70,7 @@ __device__ __forceinline__ void partner(const double* __restrict__ b, const doub
 #pragma unroll
     for (int l = 0; l < SB; ++l) {
       const int b1 = mop_a(l), b2 = mop_b(l);
-      const double val = blk_entry<HA, HB, ROW>(b, k, l);
+      const T val = (T)blk_entry<HA, HB, ROW>(b, k, l);
       J[k] += val * w[l];
       if constexpr (ROW) {
         K[mu][b1] += val * pab[nu][b2];
@@ -81,47 +82,47 @@ __device__ __forceinline__ void partner(const double* __restrict__ b, const doub
   }
 }
 
-template <bool HA, bool HB, bool ROW>
+template <bool HA, bool HB, bool ROW, typename T>
 __device__ __forceinline__ void write_offdiag(const double* __restrict__ h, double* __restrict__ f,
-                                              int N, int ao_a, int ao_b, const double (&K)[4][4]) {
+                                              int N, int ao_a, int ao_b, const T (&K)[4][4]) {
   constexpr int NA = HA ? 4 : 1, NB = HB ? 4 : 1;
 #pragma unroll
   for (int x = 0; x < NA; ++x)
 #pragma unroll
     for (int y = 0; y < NB; ++y) {
       const int64_t r = ao_a + x, c = ao_b + y;
-      const double v = h[r * N + c] - 0.5 * K[x][y];
+      const double v = h[r * N + c] - 0.5 * (double)K[x][y];
       f[r * N + c] = v;
       f[c * N + r] = v;
     }
 }
 
-template <bool HA>
+template <bool HA, typename T, typename S>
 __device__ void fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc* __restrict__ atoms,
-                          const double* __restrict__ blk, const double* __restrict__ h,
+                          const S* __restrict__ blk, const double* __restrict__ h,
                           const double* __restrict__ p, double* __restrict__ f, int lane) {
   const int N = M.n_ao;
-  double J[10];
+  T J[10];
 #pragma unroll
-  for (int k = 0; k < 10; ++k) J[k] = 0.0;
-  double Kaa[4][4];
+  for (int k = 0; k < 10; ++k) J[k] = (T)0;
+  T Kaa[4][4];
 #pragma unroll
   for (int x = 0; x < 4; ++x)
 #pragma unroll
-    for (int y = 0; y < 4; ++y) Kaa[x][y] = 0.0;
+    for (int y = 0; y < 4; ++y) Kaa[x][y] = (T)0;
   bool own_diag = false;
   for (int b0 = 0; b0 < M.n_atoms; b0 += 64) {
     const int bl = b0 + lane;
     if (bl >= M.n_atoms) continue;
     const AtomDesc B = atoms[M.atom0 + bl];
     const bool row = B.local <= A.local;
-    const double* b = blk + (row ? block_offset(A, B) : block_offset(B, A));
+    const S* b = blk + (row ? block_offset(A, B) : block_offset(B, A));
     if (row) {
-      double K[4][4];
+      T K[4][4];
 #pragma unroll
       for (int x = 0; x < 4; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y) K[x][y] = 0.0;
+        for (int y = 0; y < 4; ++y) K[x][y] = (T)0;
       if (B.zn > 1) partner<HA, true, true>(b, p, N, A.ao0, B.ao0, J, K);
       else partner<HA, false, true>(b, p, N, A.ao0, B.ao0, J, K);
       if (B.local == A.local) {
@@ -136,14 +137,15 @@ __device__ void fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc* _
         write_offdiag<HA, false, true>(h, f, N, A.ao0, B.ao0, K);
       }
     } else {
-      double K[4][4];
+      T K[4][4];
       if (B.zn > 1) partner<HA, true, false>(b, p, N, A.ao0, B.ao0, J, K);
       else partner<HA, false, false>(b, p, N, A.ao0, B.ao0, J, K);
     }
   }
   constexpr int SA = HA ? 10 : 1;
+  double Jd[10];
 #pragma unroll
-  for (int k = 0; k < SA; ++k) J[k] = wave_sum(J[k]);
+  for (int k = 0; k < SA; ++k) Jd[k] = wave_sum((double)J[k]);
   if (own_diag) {
     constexpr int NA = HA ? 4 : 1;
 #pragma unroll
@@ -151,7 +153,7 @@ __device__ void fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc* _
 #pragma unroll
       for (int y = 0; y < NA; ++y) {
         const int64_t r = A.ao0 + x, c = A.ao0 + y;
-        const double g = J[mop_idx(x, y)] - 0.5 * Kaa[x][y];
+        const double g = Jd[mop_idx(x, y)] - 0.5 * (double)Kaa[x][y];
         f[r * N + c] = h[r * N + c] + g;
       }
   }
@@ -160,21 +162,36 @@ __device__ void fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc* _
 __global__ void __launch_bounds__(256) k_fock(int n_atoms, const AtomDesc* __restrict__ atoms,
                                                const MolDesc* __restrict__ mols,
                                                const double* __restrict__ blocks,
+                                               const float* __restrict__ blocks32,
                                                const double* __restrict__ H,
                                                const double* __restrict__ P, double* __restrict__ F,
                                                const int* __restrict__ active,
-                                               const int* __restrict__ status) {
+                                               const int* __restrict__ status,
+                                               const int* __restrict__ prec) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ia = blockIdx.x * 4 + wave;
   if (ia >= n_atoms) return;
   const AtomDesc A = atoms[ia];
   if (active && (!active[A.mol] || status[A.mol])) return;
   const MolDesc M = mols[A.mol];
-  const double* blk = blocks + M.blk0;
-  if (A.zn > 1)
-    fock_atom<true>(A, M, atoms, blk, H + M.mat0, P + M.mat0, F + M.mat0, lane);
-  else
-    fock_atom<false>(A, M, atoms, blk, H + M.mat0, P + M.mat0, F + M.mat0, lane);
+  const double* h = H + M.mat0;
+  const double* p = P + M.mat0;
+  double* f = F + M.mat0;
+  if (prec && prec[A.mol]) {
+    const float* blk = blocks32 + M.blk0;
+    if (A.zn > 1) fock_atom<true, float>(A, M, atoms, blk, h, p, f, lane);
+    else fock_atom<false, float>(A, M, atoms, blk, h, p, f, lane);
+  } else {
+    const double* blk = blocks + M.blk0;
+    if (A.zn > 1) fock_atom<true, double>(A, M, atoms, blk, h, p, f, lane);
+    else fock_atom<false, double>(A, M, atoms, blk, h, p, f, lane);
+  }
+}
+
+__global__ void k_blocks_to_fp32(int64_t n, const double* __restrict__ src, float* __restrict__ dst) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
 }
 
 // Guess density (densitymatrix.cpp:26-36): P_ii = Z_core / n_AO(atom), 0 elsewhere.
@@ -218,10 +235,11 @@ __global__ void __launch_bounds__(256) k_energy(const MolDesc* __restrict__ mols
   if (threadIdx.x == 0) out[m] = 0.5 * ((red[0] + red[1]) + (red[2] + red[3]));
 }
 
-__global__ void k_scf_init(int n_mol, int max_steps, const double* __restrict__ e0,
+__global__ void k_scf_init(int n_mol, int max_steps, int policy, const double* __restrict__ e0,
                            double* __restrict__ e_el, int* __restrict__ steps,
                            int* __restrict__ active, int* __restrict__ converged,
-                           int* __restrict__ status, int* __restrict__ n_active) {
+                           int* __restrict__ status, int* __restrict__ n_active,
+                           int* __restrict__ prec, int* __restrict__ prev_prec) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m == 0) *n_active = 0;
   if (m >= n_mol) return;
@@ -230,16 +248,27 @@ __global__ void k_scf_init(int n_mol, int max_steps, const double* __restrict__ 
   active[m] = max_steps > 0 ? 1 : 0;
   converged[m] = 0;
   status[m] = 0;
+  prev_prec[m] = prec[m];
+  (void)policy;
+}
+
+__global__ void k_set_prec(int n_mol, int v, int* __restrict__ prec) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < n_mol) prec[m] = v;
 }
 
 // Bookkeeping after one SCF step (scfcalculation.cpp:82-146): converged when
-// |E_k - E_(k-1)| < threshold (sameReal, mymath.h:68-75); a failed
-// eigensolve stops the molecule with its previous energy.
-__global__ void k_scf_update(int n_mol, double threshold, int max_steps,
-                             const double* __restrict__ e_new, double* __restrict__ e_el,
-                             int* __restrict__ steps, int* __restrict__ active,
-                             int* __restrict__ converged, const int* __restrict__ status,
-                             int* __restrict__ n_active) {
+// |E_k - E_(k-1)| < threshold (sameReal, mymath.h:68-75); a failed eigensolve
+// stops the molecule with its previous energy.  Mixed precision: a step run
+// with FP32 contractions never counts as converged; the molecule switches to
+// FP64 once |dE| < switch_ev (or after fp32_cap FP32 steps), and convergence
+// is tested only between two FP64 steps.
+__global__ void k_scf_update(int n_mol, double threshold, int max_steps, double switch_ev,
+                             int fp32_cap, const double* __restrict__ e_new,
+                             double* __restrict__ e_el, int* __restrict__ steps,
+                             int* __restrict__ active, int* __restrict__ converged,
+                             const int* __restrict__ status, int* __restrict__ n_active,
+                             int* __restrict__ prec, int* __restrict__ prev_prec) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n_mol || !active[m]) return;
   if (status[m]) {
@@ -250,7 +279,11 @@ __global__ void k_scf_update(int n_mol, double threshold, int max_steps,
   e_el[m] = e;
   const int st = steps[m] + 1;
   steps[m] = st;
-  const int conv = fabs(old - e) < threshold;
+  const double de = fabs(old - e);
+  const int step_prec = prec[m];
+  const int conv = step_prec == 0 && prev_prec[m] == 0 && de < threshold;
+  prev_prec[m] = step_prec;
+  if (step_prec && (de < switch_ev || st >= fp32_cap)) prec[m] = 0;
   converged[m] = conv;
   const int still = !conv && st < max_steps;
   active[m] = still;
@@ -264,23 +297,35 @@ void launch_guess_density(const BatchDev& b, hipStream_t s) {
 }
 
 void launch_fock(const BatchDev& b, hipStream_t s, bool all) {
-  k_fock<<<blocks_for(b.n_atoms, 4), 256, 0, s>>>(b.n_atoms, b.atoms, b.mols, b.blocks, b.H, b.P, b.F,
-                                                  all ? nullptr : b.active, b.status);
+  k_fock<<<blocks_for(b.n_atoms, 4), 256, 0, s>>>(b.n_atoms, b.atoms, b.mols, b.blocks, b.blocks32,
+                                                  b.H, b.P, b.F, all ? nullptr : b.active, b.status,
+                                                  all ? nullptr : b.prec);
+}
+
+void launch_blocks_to_fp32(const BatchDev& b, hipStream_t s) {
+  k_blocks_to_fp32<<<2048, 256, 0, s>>>(b.n_blocks, b.blocks, b.blocks32);
 }
 
 void launch_energy(const BatchDev& b, hipStream_t s, double* out, bool all) {
   k_energy<<<b.n_mol, 256, 0, s>>>(b.mols, b.H, b.F, b.P, out, all ? nullptr : b.active, b.status);
 }
 
-void launch_scf_init(const BatchDev& b, hipStream_t s, int max_steps) {
-  k_scf_init<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(b.n_mol, max_steps, b.e_new, b.e_el, b.steps,
-                                                      b.active, b.converged, b.status, b.n_active);
+void launch_scf_init(const BatchDev& b, hipStream_t s, int max_steps, int policy) {
+  k_scf_init<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(b.n_mol, max_steps, policy, b.e_new, b.e_el,
+                                                      b.steps, b.active, b.converged, b.status,
+                                                      b.n_active, b.prec, b.prev_prec);
 }
 
-void launch_scf_update(const BatchDev& b, hipStream_t s, double threshold, int max_steps) {
-  k_scf_update<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(b.n_mol, threshold, max_steps, b.e_new,
-                                                        b.e_el, b.steps, b.active, b.converged,
-                                                        b.status, b.n_active);
+void launch_set_prec(const BatchDev& b, hipStream_t s, int v) {
+  k_set_prec<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(b.n_mol, v, b.prec);
+}
+
+void launch_scf_update(const BatchDev& b, hipStream_t s, double threshold, int max_steps,
+                       double switch_ev, int fp32_cap) {
+  k_scf_update<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(b.n_mol, threshold, max_steps, switch_ev,
+                                                        fp32_cap, b.e_new, b.e_el, b.steps,
+                                                        b.active, b.converged, b.status, b.n_active,
+                                                        b.prec, b.prev_prec);
 }
 
 }  // namespace sg

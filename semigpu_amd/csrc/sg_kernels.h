@@ -25,6 +25,7 @@ struct BatchDev {
   int2* pairs_all = nullptr; int n_pairs_all = 0;
   int2* diag_tasks = nullptr; int n_diag_tasks = 0;  // (atom, slot)
   double* blocks = nullptr;  // pair integrals, reference jagged order per molecule
+  float* blocks32 = nullptr; // FP32 copy of the pair integrals (mixed-precision steps)
   double* H = nullptr;
   double* F = nullptr;
   double* P = nullptr;
@@ -43,6 +44,8 @@ struct BatchDev {
   int* converged = nullptr;
   int* status = nullptr;     // 0 ok, >0 eigensolver failure
   int* n_active = nullptr;   // device counter
+  int* prec = nullptr;       // precision of the next SCF step: 1 = FP32 contractions, 0 = FP64
+  int* prev_prec = nullptr;  // precision of the step just done
   int max_n = 0;             // largest N in the batch
 };
 
@@ -52,8 +55,11 @@ void launch_core_core(const BatchDev& b, hipStream_t s);
 void launch_guess_density(const BatchDev& b, hipStream_t s);
 void launch_fock(const BatchDev& b, hipStream_t s, bool all);
 void launch_energy(const BatchDev& b, hipStream_t s, double* out, bool all);
-void launch_scf_init(const BatchDev& b, hipStream_t s, int max_steps);
-void launch_scf_update(const BatchDev& b, hipStream_t s, double threshold, int max_steps);
+void launch_scf_init(const BatchDev& b, hipStream_t s, int max_steps, int policy);
+void launch_scf_update(const BatchDev& b, hipStream_t s, double threshold, int max_steps,
+                       double switch_ev, int fp32_cap);
+void launch_blocks_to_fp32(const BatchDev& b, hipStream_t s);
+void launch_set_prec(const BatchDev& b, hipStream_t s, int v);
 void launch_eigh(const BatchDev& b, hipStream_t s, bool all);
 void launch_density(const BatchDev& b, hipStream_t s, bool all);
 

@@ -678,40 +678,88 @@ __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict_
 
 // ---------------------------------------------------------------------------
 // P from the selected vectors Y (N x k, row-major): 2 Y Y^T (occupied side)
-// or 2 (I - Y Y^T) (virtual side).  16 x 16 output tiles through LDS.
+// or 2 (I - Y Y^T) (virtual side), on the matrix cores.  One wavefront per
+// 16 x 16 tile of P, k in steps of 4 with v_mfma_f64_16x16x4f64 (FP64 steps)
+// or v_mfma_f32_16x16x4f32 (FP32 steps of the mixed-precision policy).
+// Operand layout of the 16x16x4 instructions: lane l supplies A[l%16][l/16]
+// and B[l/16][l%16].  Result register r of lane l holds D[i][l%16] with
+// i = 4*(l/16) + r for the f32 instruction and i = 4*r + l/16 for the f64 one
+// (measured on gfx950).
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * r + (lane >> 4); }
+};
+template <>
+struct Mfma16<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <typename T>
+__device__ __forceinline__ void density_tile(const double* __restrict__ Y, int n, int k, int lo,
+                                             int i0, int j0, int lane, double* __restrict__ p) {
+  typename Mfma16<T>::v4 acc = {0, 0, 0, 0};
+  const int ra = i0 + (lane & 15), rb = j0 + (lane & 15), kk = lane >> 4;
+  const bool oka = ra < n, okb = rb < n;
+  const double* ya = Y + (int64_t)(oka ? ra : 0) * k;
+  const double* yb = Y + (int64_t)(okb ? rb : 0) * k;
+  int c = 0;
+  for (; c + 16 <= k; c += 16) {
+    T a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[u] = oka ? (T)ya[c + 4 * u + kk] : (T)0;
+      b[u] = okb ? (T)yb[c + 4 * u + kk] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc = Mfma16<T>::op(a[u], b[u], acc);
+  }
+  for (; c < k; c += 4) {
+    const int cc = c + kk;
+    const T a = (oka && cc < k) ? (T)ya[cc] : (T)0;
+    const T b = (okb && cc < k) ? (T)yb[cc] : (T)0;
+    acc = Mfma16<T>::op(a, b, acc);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + Mfma16<T>::row(lane, r), j = j0 + (lane & 15);
+    if (i < n && j < n) {
+      const double s = (double)acc[r];
+      p[(int64_t)i * n + j] = lo == 0 ? 2.0 * s : 2.0 * ((i == j ? 1.0 : 0.0) - s);
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) k_density(const MolDesc* __restrict__ mols,
                                                   const double* __restrict__ work,
                                                   double* __restrict__ P,
                                                   const int* __restrict__ active,
-                                                  const int* __restrict__ status) {
+                                                  const int* __restrict__ status,
+                                                  const int* __restrict__ prec) {
   const int m = blockIdx.y;
   if (!mol_live(active, status, m)) return;
   const MolDesc M = mols[m];
   const int n = M.n_ao;
   int lo, k;
   selection(M, 0, lo, k);
-  const int tiles = (n + 15) / 16;
-  __shared__ double a[16][17], bt[16][17];
   const double* Y = work + 5 * M.mat0 + (int64_t)n * n;
   double* p = P + M.mat0;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  for (int tile = blockIdx.x; tile < tiles * tiles; tile += gridDim.x) {
-    const int i0 = (tile / tiles) * 16, j0 = (tile % tiles) * 16;
-    double s = 0.0;
-    for (int c0 = 0; c0 < k; c0 += 16) {
-      const int ra = i0 + ty, rb = j0 + ty, cc = c0 + tx;
-      a[ty][tx] = (ra < n && cc < k) ? Y[(int64_t)ra * k + cc] : 0.0;
-      bt[ty][tx] = (rb < n && cc < k) ? Y[(int64_t)rb * k + cc] : 0.0;
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < 16; ++q) s += a[ty][q] * bt[tx][q];
-      __syncthreads();
-    }
-    const int i = i0 + ty, j = j0 + tx;
-    if (i < n && j < n) {
-      const double v = lo == 0 ? 2.0 * s : 2.0 * ((i == j ? 1.0 : 0.0) - s);
-      p[(int64_t)i * n + j] = v;
-    }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int t16 = (n + 15) >> 4;
+  const bool fp32 = prec && prec[m];
+  for (int tile = blockIdx.x * 4 + wave; tile < t16 * t16; tile += gridDim.x * 4) {
+    const int i0 = (tile / t16) * 16, j0 = (tile % t16) * 16;
+    if (fp32) density_tile<float>(Y, n, k, lo, i0, j0, lane, p);
+    else density_tile<double>(Y, n, k, lo, i0, j0, lane, p);
   }
 }
 
@@ -779,7 +827,7 @@ void launch_density(const BatchDev& b, hipStream_t s, bool all) {
   if (all)
     k_density_from_c<<<grid, 256, 0, s>>>(b.mols, b.C, b.P, 64);
   else
-    k_density<<<grid, 256, 0, s>>>(b.mols, b.W, b.P, b.active, b.status);
+    k_density<<<grid, 256, 0, s>>>(b.mols, b.W, b.P, b.active, b.status, b.prec);
 }
 
 }  // namespace sg

@@ -146,3 +146,26 @@ def test_step_cap_reports_not_converged(engine):
     assert r["steps"] == 2 and not r["converged"] and r["status"] == 0
     r0 = engine.batch([geometry.pyridine()]).scf(threshold=1e-6, max_steps=0)[0]
     assert r0["steps"] == 0 and not r0["converged"]
+
+
+def test_mixed_policy_within_1e4(engine):
+    """FP32 contractions until |dE| < 1e-3 eV, then FP64: |E - E_ref| <= 1e-4 eV."""
+    names = FULL + LARGE
+    mols = [atoms_of(load_golden(n)) for n in names]
+    res = engine.batch(mols).scf(threshold=1e-6, max_steps=100, policy="mixed")
+    for n, r in zip(names, res):
+        g = load_golden(n)
+        say("mixed", n, r["steps"], r["e_tot"] - float(g["e_tot"]))
+        assert r["converged"], n
+        assert abs(r["e_tot"] - float(g["e_tot"])) <= 1e-4, n
+
+
+def test_mixed_policy_water_batch(engine):
+    mols = geometry.water_batch(64, seed=11)
+    b = engine.batch(mols)
+    ref = b.scf(threshold=1e-6, policy="fp64")
+    mix = b.scf(threshold=1e-6, policy="mixed")
+    worst = max(abs(a["e_tot"] - c["e_tot"]) for a, c in zip(ref, mix))
+    say("mixed vs fp64 worst |dE|", worst)
+    assert all(r["converged"] for r in mix)
+    assert worst <= 1e-4
