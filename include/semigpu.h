@@ -107,6 +107,42 @@ int sg_set_packed(sg_batch* b, int mol, int which, const double* in);
 /* SCFCalculation::eigenVal / eigenVec (eigenvectors column-major, n_ao*n_ao). */
 int sg_get_eigen(const sg_batch* b, int mol, double* eigval, double* eigvec);
 
+/* Continuous-batching SCF server (no counterpart in the reference, whose main()
+ * runs one molecule per process, semiGPU.cc:30-110).  Requests (one molecule
+ * each) queue in arrival order; every sg_server_step() admits queued molecules
+ * while their HBM footprint fits the server's pools (first fit) and a slot is
+ * free, sets them up (integrals, Hcore, core-core, guess, E_0), runs ONE SCF
+ * iteration for every molecule in flight, and retires the molecules that
+ * converged, reached max_steps or failed.  State stays in HBM between steps.
+ * A molecule's result does not depend on what else is in flight.
+ *   hbm_bytes  device memory the server may hold (one allocation)
+ *   max_slots  molecules in flight at most;  max_atoms  atoms in flight at most
+ *   policy     SG_POLICY_FP64 / SG_POLICY_MIXED (switch_ev as for sg_scf_run_policy)
+ * Times are seconds on the server's clock (sg_server_clock, 0 at creation). */
+typedef struct sg_server sg_server;
+typedef struct {
+  long long request_id;
+  double e_el, e_cc, e_tot;      /* eV, as sg_scf_result */
+  int steps, converged, status;
+  int n_ao;
+  double t_submit, t_admit, t_done;
+} sg_served;
+int sg_server_create(sg_context* ctx, double hbm_bytes, int max_slots, int max_atoms,
+                     double threshold_ev, int max_steps, int policy, double switch_ev,
+                     sg_server** out);
+int sg_server_destroy(sg_server* srv);
+/* Queues one molecule (Angstrom); rejects elements outside H, C, N, O and
+ * molecules larger than the server's pools. */
+int sg_server_submit(sg_server* srv, long long request_id, int n_atoms, const int* Z,
+                     const double* xyz_ang);
+/* One admission + SCF iteration + retirement round; reports the molecules in
+ * flight and still queued afterwards. */
+int sg_server_step(sg_server* srv, int* n_inflight, int* n_queued);
+/* Moves up to max_out finished results (completion order) into out[];
+ * returns how many, or a negative error code. */
+int sg_server_poll(sg_server* srv, sg_served* out, int max_out);
+double sg_server_clock(const sg_server* srv);
+
 /* Per-stage GPU time (HIP events on the context stream), milliseconds summed
  * since the last reset; stage order: two_center, hcore, guess, fock, eigh,
  * density, energy, control.  counts[] gets the number of launches timed. */

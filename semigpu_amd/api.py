@@ -32,6 +32,8 @@ EXPORTS = (
     "sg_scf_run_policy",
     "sg_pair_block_count", "sg_get_pair_blocks", "sg_get_packed", "sg_set_packed",
     "sg_get_eigen", "sg_timing_enable", "sg_timing_read", "sg_timing_reset",
+    "sg_server_create", "sg_server_destroy", "sg_server_submit", "sg_server_step",
+    "sg_server_poll", "sg_server_clock",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -42,6 +44,14 @@ _vp = ctypes.c_void_p
 class ScfResult(ctypes.Structure):
     _fields_ = [("e_el", ctypes.c_double), ("e_cc", ctypes.c_double), ("e_tot", ctypes.c_double),
                 ("steps", ctypes.c_int), ("converged", ctypes.c_int), ("status", ctypes.c_int)]
+
+
+class Served(ctypes.Structure):
+    _fields_ = [("request_id", ctypes.c_longlong), ("e_el", ctypes.c_double),
+                ("e_cc", ctypes.c_double), ("e_tot", ctypes.c_double), ("steps", ctypes.c_int),
+                ("converged", ctypes.c_int), ("status", ctypes.c_int), ("n_ao", ctypes.c_int),
+                ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double)]
 
 
 class SemiGpuError(RuntimeError):
@@ -82,6 +92,15 @@ def load_library(path: str = LIB_PATH):
     L.sg_timing_enable.argtypes = [_vp, ctypes.c_int]
     L.sg_timing_read.argtypes = [_vp, _dp, _ip]
     L.sg_timing_reset.argtypes = [_vp]
+    L.sg_server_create.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                   ctypes.POINTER(_vp)]
+    L.sg_server_destroy.argtypes = [_vp]
+    L.sg_server_submit.argtypes = [_vp, ctypes.c_longlong, ctypes.c_int, _ip, _dp]
+    L.sg_server_step.argtypes = [_vp, _ip, _ip]
+    L.sg_server_poll.argtypes = [_vp, ctypes.POINTER(Served), ctypes.c_int]
+    L.sg_server_clock.argtypes = [_vp]
+    L.sg_server_clock.restype = ctypes.c_double
     _lib = L
     return L
 
@@ -109,12 +128,15 @@ class Engine:
         self.h = h
         self.L = L
         self._batches = weakref.WeakSet()
+        self._servers = weakref.WeakSet()
 
     def close(self):
         """Destroys the context; batches still alive are destroyed first."""
         if self.h:
             for b in list(self._batches):
                 b.close()
+            for srv in list(self._servers):
+                srv.close()
             self.L.sg_ctx_destroy(self.h)
             self.h = None
 
@@ -126,6 +148,9 @@ class Engine:
 
     def batch(self, molecules: Sequence[Sequence[Sequence[float]]]) -> "Batch":
         return Batch(self, molecules)
+
+    def server(self, **kw) -> "Server":
+        return Server(self, **kw)
 
     def timing(self, on: bool = True):
         _check(self.L.sg_timing_enable(self.h, 1 if on else 0))
@@ -245,3 +270,65 @@ class Batch:
         c = np.zeros(n * n)
         _check(self.L.sg_get_eigen(self.h, m, _d(w), _d(c)))
         return w, c.reshape(n, n).T.copy()  # columns are eigenvectors
+
+
+class Server:
+    """Continuous-batching SCF server (sg_server): submit molecules, step, poll."""
+
+    def __init__(self, engine: Engine, hbm_bytes: float = 8e9, max_slots: int = 4096,
+                 max_atoms: int = 1 << 18, threshold: float = 5e-3, max_steps: int = 100,
+                 policy: str = "fp64", switch_ev: float = 1e-3):
+        self.engine = engine
+        self.L = engine.L
+        h = _vp()
+        pol = {"fp64": 0, "mixed": 1}[policy]
+        _check(self.L.sg_server_create(engine.h, float(hbm_bytes), max_slots, max_atoms,
+                                       threshold, max_steps, pol, switch_ev, ctypes.byref(h)))
+        self.h = h
+        engine._servers.add(self)
+
+    def close(self):
+        if self.h:
+            self.L.sg_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, request_id: int, molecule) -> None:
+        Z = np.asarray([int(a[0]) for a in molecule], dtype=np.int32)
+        xyz = np.asarray([[float(c) for c in a[1:4]] for a in molecule], dtype=np.float64).ravel()
+        _check(self.L.sg_server_submit(self.h, request_id, len(Z), _i(Z), _d(xyz)))
+
+    def step(self):
+        """One admission + iteration + retirement round -> (in flight, queued)."""
+        n_in, n_q = ctypes.c_int(), ctypes.c_int()
+        _check(self.L.sg_server_step(self.h, ctypes.byref(n_in), ctypes.byref(n_q)))
+        return n_in.value, n_q.value
+
+    def poll(self, max_out: int = 4096):
+        buf = (Served * max_out)()
+        n = self.L.sg_server_poll(self.h, buf, max_out)
+        if n < 0:
+            _check(n)
+        return [{"id": r.request_id, "e_el": r.e_el, "e_cc": r.e_cc, "e_tot": r.e_tot,
+                 "steps": r.steps, "converged": bool(r.converged), "status": r.status,
+                 "n_ao": r.n_ao, "t_submit": r.t_submit, "t_admit": r.t_admit,
+                 "t_done": r.t_done} for r in buf[:n]]
+
+    def clock(self) -> float:
+        return self.L.sg_server_clock(self.h)
+
+    def drain(self, max_iters: int = 100000):
+        """Steps until nothing is queued or in flight; returns every result."""
+        out = []
+        for _ in range(max_iters):
+            n_in, n_q = self.step()
+            out.extend(self.poll())
+            if n_in == 0 and n_q == 0:
+                break
+        out.extend(self.poll())
+        return out

@@ -27,6 +27,7 @@ UNITS = {
     "sg_fock.hip": [],
     "sg_linalg.hip": [],
     "sg_api.cpp": ["-x", "hip"],
+    "sg_server.cpp": ["-x", "hip"],
 }
 
 

@@ -13,33 +13,28 @@
 #include <string>
 #include <vector>
 
-#include "../../include/semigpu.h"
-#include "sg_kernels.h"
+#include "sg_host.h"
 
 using namespace sg;
 
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string& msg) {
+namespace sg {
+
+int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
 
-#define HIP_TRY(expr)                                                                   \
-  do {                                                                                  \
-    hipError_t _e = (expr);                                                             \
-    if (_e != hipSuccess)                                                               \
-      return fail(SG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));        \
-  } while (0)
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(SG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return SG_OK;
+}
 
-struct sg_context {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  bool timing = false;
-  double ms[SG_N_STAGES] = {0};
-  int counts[SG_N_STAGES] = {0};
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
-};
+bool supported_element(int z) { return z == 1 || z == 6 || z == 7 || z == 8; }
+
+}  // namespace sg
 
 struct sg_batch {
   sg_context* ctx = nullptr;
@@ -51,39 +46,6 @@ struct sg_batch {
   bool have_ints = false, have_hcore = false, have_ecc = false, have_fp32 = false;
   std::vector<double> e_cc;
 };
-
-enum { ST_TWO_CENTER = 0, ST_HCORE, ST_GUESS, ST_FOCK, ST_EIGH, ST_DENSITY, ST_ENERGY, ST_CONTROL };
-
-namespace {
-
-struct StageTimer {
-  sg_context* c;
-  int stage;
-  hipEvent_t a = nullptr, b = nullptr;
-  StageTimer(sg_context* ctx, int st) : c(ctx), stage(st) {
-    if (c->timing) {
-      (void)hipEventCreate(&a);
-      (void)hipEventCreate(&b);
-      (void)hipEventRecord(a, c->stream);
-    }
-  }
-  ~StageTimer() {
-    if (c->timing) {
-      (void)hipEventRecord(b, c->stream);
-      c->pending.push_back({stage, {a, b}});
-    }
-  }
-};
-
-int check_launch(const char* what) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(SG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-  return SG_OK;
-}
-
-bool supported(int z) { return z == 1 || z == 6 || z == 7 || z == 8; }
-
-}  // namespace
 
 extern "C" {
 
@@ -165,7 +127,7 @@ int sg_batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, const i
       return fail(SG_ERR_INVALID, "molecule " + std::to_string(m) + " has no atoms");
   const int n_atoms = atom_offsets[n_mol];
   for (int i = 0; i < n_atoms; ++i) {
-    if (!supported(Z[i]))
+    if (!supported_element(Z[i]))
       return fail(SG_ERR_UNSUPPORTED, "atom " + std::to_string(i) + ": element Z=" +
                                           std::to_string(Z[i]) + " has no MNDO parameters here");
     for (int k = 0; k < 3; ++k)
@@ -224,9 +186,10 @@ int sg_batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, const i
     blk0 += row;
     max_n = std::max(max_n, ao);
   }
-  if (max_n > 1024) {
+  if (max_n > kMaxBasis) {
     delete b;
-    return fail(SG_ERR_INVALID, "molecules above 1024 basis functions are not supported yet");
+    return fail(SG_ERR_INVALID, "molecules above " + std::to_string(kMaxBasis) +
+                                    " basis functions are not supported");
   }
   BatchDev& d = b->d;
   d.n_mol = n_mol;
@@ -357,7 +320,7 @@ int sg_two_center(sg_batch* b) {
   int r = check_launch("two-center integrals");
   if (r) return r;
   b->have_ints = true;
-  b->have_fp32 = false;
+  b->have_fp32 = true;  // the integral kernels write the FP32 copy as well
   return SG_OK;
 }
 
@@ -469,7 +432,7 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
   sg_context* c = b->ctx;
   hipStream_t s = c->stream;
   HIP_TRY(hipSetDevice(c->device));
-  if (policy == SG_POLICY_MIXED && !b->have_fp32) {
+  if (policy == SG_POLICY_MIXED && !b->have_fp32) {  // integrals set by the stage API only
     StageTimer t(c, ST_TWO_CENTER);
     launch_blocks_to_fp32(b->d, s);
     b->have_fp32 = true;

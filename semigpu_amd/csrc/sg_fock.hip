@@ -196,7 +196,8 @@ __global__ void k_blocks_to_fp32(int64_t n, const double* __restrict__ src, floa
 
 // Guess density (densitymatrix.cpp:26-36): P_ii = Z_core / n_AO(atom), 0 elsewhere.
 __global__ void k_guess_density(const MolDesc* __restrict__ mols, const AtomDesc* __restrict__ atoms,
-                                double* __restrict__ P) {
+                                double* __restrict__ P, const int* __restrict__ mask) {
+  if (mask && !mask[blockIdx.x]) return;
   const MolDesc M = mols[blockIdx.x];
   const int N = M.n_ao;
   double* p = P + M.mat0;
@@ -239,10 +240,12 @@ __global__ void k_scf_init(int n_mol, int max_steps, int policy, const double* _
                            double* __restrict__ e_el, int* __restrict__ steps,
                            int* __restrict__ active, int* __restrict__ converged,
                            int* __restrict__ status, int* __restrict__ n_active,
-                           int* __restrict__ prec, int* __restrict__ prev_prec) {
+                           int* __restrict__ prec, int* __restrict__ prev_prec,
+                           const int* __restrict__ mask) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m == 0) *n_active = 0;
+  if (m == 0 && !mask) *n_active = 0;
   if (m >= n_mol) return;
+  if (mask && !mask[m]) return;
   e_el[m] = e0[m];
   steps[m] = 0;
   active[m] = max_steps > 0 ? 1 : 0;
@@ -252,9 +255,9 @@ __global__ void k_scf_init(int n_mol, int max_steps, int policy, const double* _
   (void)policy;
 }
 
-__global__ void k_set_prec(int n_mol, int v, int* __restrict__ prec) {
+__global__ void k_set_prec(int n_mol, int v, int* __restrict__ prec, const int* __restrict__ mask) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m < n_mol) prec[m] = v;
+  if (m < n_mol && (!mask || mask[m])) prec[m] = v;
 }
 
 // Bookkeeping after one SCF step (scfcalculation.cpp:82-146): converged when
@@ -290,15 +293,34 @@ __global__ void k_scf_update(int n_mol, double threshold, int max_steps, double 
   if (still) atomicAdd(n_active, 1);
 }
 
-static inline int blocks_for(int64_t n, int t) { return (int)((n + t - 1) / t); }
-
-void launch_guess_density(const BatchDev& b, hipStream_t s) {
-  k_guess_density<<<b.n_mol, 256, 0, s>>>(b.mols, b.atoms, b.P);
+// Zeroes H and F of the masked molecules, so that a molecule placed in ranges
+// an earlier molecule used starts from the same state as in a fresh batch.
+__global__ void k_clear_hf(const MolDesc* __restrict__ mols, double* __restrict__ H,
+                           double* __restrict__ F, const int* __restrict__ mask) {
+  if (!mask[blockIdx.y]) return;
+  const MolDesc M = mols[blockIdx.y];
+  const int64_t nn = (int64_t)M.n_ao * M.n_ao;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nn;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    H[M.mat0 + t] = 0.0;
+    F[M.mat0 + t] = 0.0;
+  }
 }
 
-void launch_fock(const BatchDev& b, hipStream_t s, bool all) {
+static inline int blocks_for(int64_t n, int t) { return (int)((n + t - 1) / t); }
+
+void launch_clear_hf(const BatchDev& b, hipStream_t s, const int* mask) {
+  if (b.n_mol) k_clear_hf<<<dim3(8, b.n_mol), 256, 0, s>>>(b.mols, b.H, b.F, mask);
+}
+
+void launch_guess_density(const BatchDev& b, hipStream_t s, const int* mask) {
+  k_guess_density<<<b.n_mol, 256, 0, s>>>(b.mols, b.atoms, b.P, mask);
+}
+
+void launch_fock(const BatchDev& b, hipStream_t s, bool all, const int* mask) {
   k_fock<<<blocks_for(b.n_atoms, 4), 256, 0, s>>>(b.n_atoms, b.atoms, b.mols, b.blocks, b.blocks32,
-                                                  b.H, b.P, b.F, all ? nullptr : b.active, b.status,
+                                                  b.H, b.P, b.F,
+                                                  mask ? mask : (all ? nullptr : b.active), b.status,
                                                   all ? nullptr : b.prec);
 }
 
@@ -306,18 +328,20 @@ void launch_blocks_to_fp32(const BatchDev& b, hipStream_t s) {
   k_blocks_to_fp32<<<2048, 256, 0, s>>>(b.n_blocks, b.blocks, b.blocks32);
 }
 
-void launch_energy(const BatchDev& b, hipStream_t s, double* out, bool all) {
-  k_energy<<<b.n_mol, 256, 0, s>>>(b.mols, b.H, b.F, b.P, out, all ? nullptr : b.active, b.status);
+void launch_energy(const BatchDev& b, hipStream_t s, double* out, bool all, const int* mask) {
+  k_energy<<<b.n_mol, 256, 0, s>>>(b.mols, b.H, b.F, b.P, out,
+                                   mask ? mask : (all ? nullptr : b.active), b.status);
 }
 
-void launch_scf_init(const BatchDev& b, hipStream_t s, int max_steps, int policy) {
+void launch_scf_init(const BatchDev& b, hipStream_t s, int max_steps, int policy,
+                     const int* mask) {
   k_scf_init<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(b.n_mol, max_steps, policy, b.e_new, b.e_el,
                                                       b.steps, b.active, b.converged, b.status,
-                                                      b.n_active, b.prec, b.prev_prec);
+                                                      b.n_active, b.prec, b.prev_prec, mask);
 }
 
-void launch_set_prec(const BatchDev& b, hipStream_t s, int v) {
-  k_set_prec<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(b.n_mol, v, b.prec);
+void launch_set_prec(const BatchDev& b, hipStream_t s, int v, const int* mask) {
+  k_set_prec<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(b.n_mol, v, b.prec, mask);
 }
 
 void launch_scf_update(const BatchDev& b, hipStream_t s, double threshold, int max_steps,

@@ -334,14 +334,17 @@ __device__ __forceinline__ double xx_entry(const LocalXX& v, const Frame& F) {
 
 template <int E>
 struct XXFill {
-  __device__ static __forceinline__ void run(const LocalXX& v, const Frame& F, double* out) {
-    XXFill<E - 1>::run(v, F, out);
-    out[E - 1] = xx_entry<(E - 1) / 10, (E - 1) % 10>(v, F);
+  __device__ static __forceinline__ void run(const LocalXX& v, const Frame& F, double* out,
+                                             float* out32) {
+    XXFill<E - 1>::run(v, F, out, out32);
+    const double x = xx_entry<(E - 1) / 10, (E - 1) % 10>(v, F);
+    out[E - 1] = x;
+    if (out32) out32[E - 1] = (float)x;
   }
 };
 template <>
 struct XXFill<0> {
-  __device__ static __forceinline__ void run(const LocalXX&, const Frame&, double*) {}
+  __device__ static __forceinline__ void run(const LocalXX&, const Frame&, double*, float*) {}
 };
 
 // ---------------------------------------------------------------------------
@@ -350,7 +353,8 @@ struct XXFill<0> {
 __global__ void __launch_bounds__(64) k_pair_xx(const int2* __restrict__ pairs, int n,
                                                  const AtomDesc* __restrict__ atoms,
                                                  const MolDesc* __restrict__ mols,
-                                                 double* __restrict__ blocks) {
+                                                 double* __restrict__ blocks,
+                                                 float* __restrict__ blocks32) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int2 pr = pairs[i];
@@ -358,8 +362,8 @@ __global__ void __launch_bounds__(64) k_pair_xx(const int2* __restrict__ pairs, 
   const double R = dist3(A.x, A.y, A.z, C.x, C.y, C.z);
   const Frame F = local_frame(A, C, R);
   const LocalXX v = local_xx(R, A.zn, C.zn);
-  double* out = blocks + mols[A.mol].blk0 + block_offset(A, C);
-  XXFill<100>::run(v, F, out);
+  const int64_t o = mols[A.mol].blk0 + block_offset(A, C);
+  XXFill<100>::run(v, F, blocks + o, blocks32 ? blocks32 + o : nullptr);
 }
 
 // (H H | X pair) blocks: the H orbital is always the first pair and the frame
@@ -367,7 +371,8 @@ __global__ void __launch_bounds__(64) k_pair_xx(const int2* __restrict__ pairs, 
 __global__ void __launch_bounds__(256) k_pair_hx(const int2* __restrict__ pairs, int n,
                                                   const AtomDesc* __restrict__ atoms,
                                                   const MolDesc* __restrict__ mols,
-                                                  double* __restrict__ blocks) {
+                                                  double* __restrict__ blocks,
+                                                  float* __restrict__ blocks32) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int2 pr = pairs[i];
@@ -382,8 +387,11 @@ __global__ void __launch_bounds__(256) k_pair_hx(const int2* __restrict__ pairs,
   const double qUz = mp_qUz(R, h, x);
   const double vpi = qq + mp_qQpipi(R, h, x);
   const double vz = qq + mp_qQzz(R, h, x);
-  double* out = blocks + mols[Ar.mol].blk0 + block_offset(Ar, Ac);
+  const int64_t o = mols[Ar.mol].blk0 + block_offset(Ar, Ac);
+  double* out = blocks + o;
+  float* o32 = blocks32 ? blocks32 + o : nullptr;
   out[0] = qq * kHartreeToEv;
+  if (o32) o32[0] = (float)out[0];
 #pragma unroll
   for (int k = 1; k < 10; ++k) {
     const int t2 = mop_a(k), t3 = mop_b(k);
@@ -396,19 +404,24 @@ __global__ void __launch_bounds__(256) k_pair_hx(const int2* __restrict__ pairs,
       r = a + b;
     }
     out[k] = r * kHartreeToEv;
+    if (o32) o32[k] = (float)out[k];
   }
 }
 
 __global__ void __launch_bounds__(256) k_pair_hh(const int2* __restrict__ pairs, int n,
                                                   const AtomDesc* __restrict__ atoms,
                                                   const MolDesc* __restrict__ mols,
-                                                  double* __restrict__ blocks) {
+                                                  double* __restrict__ blocks,
+                                                  float* __restrict__ blocks32) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int2 pr = pairs[i];
   const AtomDesc A = atoms[pr.x], C = atoms[pr.y];
   const double R = dist3(A.x, A.y, A.z, C.x, C.y, C.z);
-  blocks[mols[A.mol].blk0 + block_offset(A, C)] = mp_qq(R, A.zn, C.zn) * kHartreeToEv;
+  const int64_t o = mols[A.mol].blk0 + block_offset(A, C);
+  const double v = mp_qq(R, A.zn, C.zn) * kHartreeToEv;
+  blocks[o] = v;
+  if (blocks32) blocks32[o] = (float)v;
 }
 
 // One-center blocks (SelfIntegral*, :738-778); zero where the reference
@@ -428,18 +441,22 @@ __device__ double one_center(int z, int s0, int s1, int s2, int s3) {
 }
 
 __global__ void k_one_center(int n_atoms, const AtomDesc* __restrict__ atoms,
-                             const MolDesc* __restrict__ mols, double* __restrict__ blocks) {
+                             const MolDesc* __restrict__ mols, double* __restrict__ blocks,
+                             float* __restrict__ blocks32, const int* __restrict__ mask) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_atoms) return;
   const AtomDesc A = atoms[i];
-  double* out = blocks + mols[A.mol].blk0 + block_offset(A, A);
-  if (A.zn == 1) {
-    out[0] = c_par.gss[1];
-    return;
+  if (mask && !mask[A.mol]) return;
+  const int64_t o = mols[A.mol].blk0 + block_offset(A, A);
+  double* out = blocks + o;
+  const int cnt = A.zn == 1 ? 1 : 100;
+  for (int t = 0; t < cnt; ++t) {
+    const int k = t / 10, l = t % 10;
+    const double v = A.zn == 1 ? c_par.gss[1]
+                               : one_center(A.zn, mop_a(k), mop_b(k), mop_a(l), mop_b(l));
+    out[t] = v;
+    if (blocks32) blocks32[o + t] = (float)v;
   }
-  for (int k = 0; k < 10; ++k)
-    for (int l = 0; l < 10; ++l)
-      out[k * 10 + l] = one_center(A.zn, mop_a(k), mop_b(k), mop_a(l), mop_b(l));
 }
 
 // ---------------------------------------------------------------------------
@@ -559,9 +576,10 @@ __global__ void k_hcore_diag(const int2* __restrict__ tasks, int n,
 // Core-core repulsion per molecule (CoreCoreRepulsion::ComputeRepulsion).
 __global__ void k_core_core(int n_mol, const AtomDesc* __restrict__ atoms,
                             const MolDesc* __restrict__ mols, const double* __restrict__ blocks,
-                            double* __restrict__ e_cc) {
+                            double* __restrict__ e_cc, const int* __restrict__ mask) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n_mol) return;
+  if (mask && !mask[m]) return;
   const MolDesc M = mols[m];
   const double* blk = blocks + M.blk0;
   double e = 0.0;
@@ -592,11 +610,18 @@ __global__ void k_core_core(int n_mol, const AtomDesc* __restrict__ atoms,
 // ---------------------------------------------------------------------------
 static inline int blocks_for(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
-void launch_pair_integrals(const BatchDev& b, hipStream_t s) {
-  if (b.n_xx) k_pair_xx<<<blocks_for(b.n_xx, 64), 64, 0, s>>>(b.pairs_xx, b.n_xx, b.atoms, b.mols, b.blocks);
-  if (b.n_hx) k_pair_hx<<<blocks_for(b.n_hx, 256), 256, 0, s>>>(b.pairs_hx, b.n_hx, b.atoms, b.mols, b.blocks);
-  if (b.n_hh) k_pair_hh<<<blocks_for(b.n_hh, 256), 256, 0, s>>>(b.pairs_hh, b.n_hh, b.atoms, b.mols, b.blocks);
-  k_one_center<<<blocks_for(b.n_atoms, 128), 128, 0, s>>>(b.n_atoms, b.atoms, b.mols, b.blocks);
+void launch_pair_integrals(const BatchDev& b, hipStream_t s, const int* mask) {
+  if (b.n_xx)
+    k_pair_xx<<<blocks_for(b.n_xx, 64), 64, 0, s>>>(b.pairs_xx, b.n_xx, b.atoms, b.mols, b.blocks,
+                                                   b.blocks32);
+  if (b.n_hx)
+    k_pair_hx<<<blocks_for(b.n_hx, 256), 256, 0, s>>>(b.pairs_hx, b.n_hx, b.atoms, b.mols, b.blocks,
+                                                     b.blocks32);
+  if (b.n_hh)
+    k_pair_hh<<<blocks_for(b.n_hh, 256), 256, 0, s>>>(b.pairs_hh, b.n_hh, b.atoms, b.mols, b.blocks,
+                                                     b.blocks32);
+  k_one_center<<<blocks_for(b.n_atoms, 128), 128, 0, s>>>(b.n_atoms, b.atoms, b.mols, b.blocks,
+                                                          b.blocks32, mask);
 }
 
 void launch_hcore(const BatchDev& b, hipStream_t s) {
@@ -607,8 +632,9 @@ void launch_hcore(const BatchDev& b, hipStream_t s) {
                                                                b.mols, b.blocks, b.H);
 }
 
-void launch_core_core(const BatchDev& b, hipStream_t s) {
-  k_core_core<<<blocks_for(b.n_mol, 64), 64, 0, s>>>(b.n_mol, b.atoms, b.mols, b.blocks, b.e_cc);
+void launch_core_core(const BatchDev& b, hipStream_t s, const int* mask) {
+  k_core_core<<<blocks_for(b.n_mol, 64), 64, 0, s>>>(b.n_mol, b.atoms, b.mols, b.blocks, b.e_cc,
+                                                     mask);
 }
 
 }  // namespace sg

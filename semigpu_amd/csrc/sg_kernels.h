@@ -7,6 +7,10 @@
 
 namespace sg {
 
+// Largest basis the eigensolver handles in one workgroup (LDS-resident
+// tridiagonal, 20 x 64-column row chunks in the reduction).
+constexpr int kMaxBasis = 1280;
+
 // Device view of a batch of molecules.  All matrices are dense N x N per
 // molecule, row-major (symmetric ones are stored in full); eigenvectors are
 // column-major (C[k*N + i] = component i of eigenvector k), which is the
@@ -49,17 +53,22 @@ struct BatchDev {
   int max_n = 0;             // largest N in the batch
 };
 
-void launch_pair_integrals(const BatchDev& b, hipStream_t s);
+// `mask` (optional, per molecule) restricts a launch to the molecules whose
+// entry is non-zero; it overrides the SCF's own `active` flags.
+void launch_pair_integrals(const BatchDev& b, hipStream_t s, const int* mask = nullptr);
 void launch_hcore(const BatchDev& b, hipStream_t s);
-void launch_core_core(const BatchDev& b, hipStream_t s);
-void launch_guess_density(const BatchDev& b, hipStream_t s);
-void launch_fock(const BatchDev& b, hipStream_t s, bool all);
-void launch_energy(const BatchDev& b, hipStream_t s, double* out, bool all);
-void launch_scf_init(const BatchDev& b, hipStream_t s, int max_steps, int policy);
+void launch_core_core(const BatchDev& b, hipStream_t s, const int* mask = nullptr);
+void launch_guess_density(const BatchDev& b, hipStream_t s, const int* mask = nullptr);
+void launch_fock(const BatchDev& b, hipStream_t s, bool all, const int* mask = nullptr);
+void launch_energy(const BatchDev& b, hipStream_t s, double* out, bool all,
+                   const int* mask = nullptr);
+void launch_scf_init(const BatchDev& b, hipStream_t s, int max_steps, int policy,
+                     const int* mask = nullptr);
 void launch_scf_update(const BatchDev& b, hipStream_t s, double threshold, int max_steps,
                        double switch_ev, int fp32_cap);
 void launch_blocks_to_fp32(const BatchDev& b, hipStream_t s);
-void launch_set_prec(const BatchDev& b, hipStream_t s, int v);
+void launch_set_prec(const BatchDev& b, hipStream_t s, int v, const int* mask = nullptr);
+void launch_clear_hf(const BatchDev& b, hipStream_t s, const int* mask);
 void launch_eigh(const BatchDev& b, hipStream_t s, bool all);
 void launch_density(const BatchDev& b, hipStream_t s, bool all);
 

@@ -807,7 +807,7 @@ void launch_eigh(const BatchDev& b, hipStream_t s, bool all) {
   if (N <= 384)
     k_tridiag<6><<<b.n_mol, kT, tridiag_lds(N), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status);
   else
-    k_tridiag<16><<<b.n_mol, kT, tridiag_lds(N), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status);
+    k_tridiag<20><<<b.n_mol, kT, tridiag_lds(N), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status);
   dim3 gb((N + 255) / 256, b.n_mol);
   k_bisect<<<gb, 256, sizeof(double) * (2 * N + 32), s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm,
                                                           want_all, act, b.status);

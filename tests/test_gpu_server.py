@@ -1,0 +1,99 @@
+"""Continuous-batching server (sg_server) through the C ABI.
+
+A molecule's result must not depend on what else is in flight: every result
+from the server is compared bit for bit with the same molecule run alone as a
+batch of one, and the reference fixtures are met to the SCF bars
+(|E - E_ref| <= 1e-6 eV at threshold 1e-6 eV, same step count).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import atoms_of, golden_names, load_golden
+from semigpu_amd import api, geometry
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("e_el", "e_cc", "e_tot", "steps", "converged", "status")
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stream():
+    """Mixed sizes: small water clusters, a few large ones, alkanes, pyridine."""
+    mols = geometry.water_batch(24, seed=314, n_min=2, n_max=24)
+    mols[5:5] = [geometry.water_cluster(60, geometry.SplitMix64(9))]
+    mols[11:11] = [geometry.alkane(9)]
+    mols[17:17] = [geometry.water_cluster(48, geometry.SplitMix64(10)), geometry.pyridine()]
+    return mols
+
+
+def test_server_equals_alone_bitwise(engine):
+    mols = stream()
+    # small pools: the large clusters wait for space, ranges and slots are reused
+    srv = engine.server(hbm_bytes=48e6, max_slots=6, max_atoms=600, threshold=1e-6)
+    for i, m in enumerate(mols):
+        srv.submit(1000 + i, m)
+    out = srv.drain()
+    srv.close()
+    assert sorted(r["id"] for r in out) == [1000 + i for i in range(len(mols))]
+    by_id = {r["id"]: r for r in out}
+    for i, m in enumerate(mols):
+        r = by_id[1000 + i]
+        assert r["t_submit"] <= r["t_admit"] <= r["t_done"]
+        alone = engine.batch([m]).scf(threshold=1e-6)[0]
+        say("server", i, len(m), r["steps"], r["e_tot"], alone["e_tot"])
+        assert {k: r[k] for k in KEYS} == alone, i
+
+
+def test_server_mixed_policy_equals_alone(engine):
+    mols = geometry.water_batch(16, seed=2718, n_min=2, n_max=32)
+    srv = engine.server(hbm_bytes=64e6, max_slots=5, max_atoms=400, threshold=1e-6,
+                        policy="mixed")
+    for i, m in enumerate(mols):
+        srv.submit(i, m)
+    out = {r["id"]: r for r in srv.drain()}
+    srv.close()
+    for i, m in enumerate(mols):
+        alone = engine.batch([m]).scf(threshold=1e-6, policy="mixed")[0]
+        assert {k: out[i][k] for k in KEYS} == alone, i
+
+
+def test_server_fixtures(engine):
+    names = golden_names()
+    srv = engine.server(hbm_bytes=256e6, max_slots=16, max_atoms=4096, threshold=1e-6)
+    for i, n in enumerate(names):
+        srv.submit(i, atoms_of(load_golden(n)))
+    out = {r["id"]: r for r in srv.drain()}
+    srv.close()
+    for i, n in enumerate(names):
+        g = load_golden(n)
+        r = out[i]
+        assert r["steps"] == int(g["steps"]), n
+        assert r["converged"] == bool(g["converged"]), n
+        assert abs(r["e_tot"] - float(g["e_tot"])) <= 1e-6, n
+
+
+def test_server_rejects(engine):
+    srv = engine.server(hbm_bytes=16e6, max_slots=4, max_atoms=256)
+    with pytest.raises(api.SemiGpuError, match="Z=9"):
+        srv.submit(1, [(9, 0.0, 0.0, 0.0), (1, 0.0, 0.0, 0.9)])
+    with pytest.raises(api.SemiGpuError, match="capacity"):
+        srv.submit(2, geometry.water_cluster(100, geometry.SplitMix64(1)))  # 300 atoms > max_atoms
+    assert srv.step() == (0, 0)
+    assert srv.poll() == []
+    srv.close()
+
+
+def test_server_step_cap(engine):
+    srv = engine.server(hbm_bytes=16e6, max_slots=4, max_atoms=256, threshold=1e-12,
+                        max_steps=3)
+    srv.submit(7, geometry.pyridine())
+    out = srv.drain()
+    srv.close()
+    assert len(out) == 1 and out[0]["steps"] == 3 and not out[0]["converged"]
+    alone = engine.batch([geometry.pyridine()]).scf(threshold=1e-12, max_steps=3)[0]
+    assert {k: out[0][k] for k in KEYS} == alone
