@@ -144,9 +144,10 @@ int sg_server_poll(sg_server* srv, sg_served* out, int max_out);
 double sg_server_clock(const sg_server* srv);
 
 /* Per-stage GPU time (HIP events on the context stream), milliseconds summed
- * since the last reset; stage order: two_center, hcore, guess, fock, eigh,
- * density, energy, control.  counts[] gets the number of launches timed. */
-#define SG_N_STAGES 8
+ * since the last reset; stage order: two_center, hcore, guess, fock, tridiag,
+ * bisect, invit, backtransform (the eigensolver's four kernels), density,
+ * energy, control.  counts[] gets the number of launches timed. */
+#define SG_N_STAGES 11
 int sg_timing_enable(sg_context* ctx, int on);
 int sg_timing_read(sg_context* ctx, double* ms, int* counts);
 int sg_timing_reset(sg_context* ctx);

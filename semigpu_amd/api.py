@@ -22,7 +22,9 @@ LIB_PATH = os.path.join(HERE, "libsemigpu.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include", "semigpu.h")
 
 SG_HCORE, SG_FOCK, SG_DENSITY = 0, 1, 2
-STAGES = ("two_center", "hcore", "guess", "fock", "eigh", "density", "energy", "control")
+STAGES = ("two_center", "hcore", "guess", "fock", "tridiag", "bisect", "invit", "backtransform",
+          "density", "energy", "control")
+EIGH_STAGES = ("tridiag", "bisect", "invit", "backtransform")
 
 # Every function declared in include/semigpu.h (checked by tests/test_abi.py).
 EXPORTS = (
@@ -156,8 +158,8 @@ class Engine:
         _check(self.L.sg_timing_enable(self.h, 1 if on else 0))
 
     def timing_read(self):
-        ms = np.zeros(8)
-        cnt = np.zeros(8, dtype=np.int32)
+        ms = np.zeros(len(STAGES))
+        cnt = np.zeros(len(STAGES), dtype=np.int32)
         _check(self.L.sg_timing_read(self.h, _d(ms), _i(cnt)))
         return {s: (float(ms[k]), int(cnt[k])) for k, s in enumerate(STAGES)}
 

@@ -45,6 +45,9 @@ struct sg_batch {
   std::vector<int64_t> blk_count;  // doubles of pair blocks per molecule
   bool have_ints = false, have_hcore = false, have_ecc = false, have_fp32 = false;
   std::vector<double> e_cc;
+  hipGraphExec_t iter_exec = nullptr;  // captured SCF iteration
+  double g_threshold = -1.0, g_switch = -1.0;
+  int g_max_steps = -1;
 };
 
 extern "C" {
@@ -297,6 +300,7 @@ int sg_batch_destroy(sg_batch* b) {
   if (!b) return SG_OK;
   (void)hipSetDevice(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
+  if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
   if (b->dev_mem) (void)hipFree(b->dev_mem);
   delete b;
   return SG_OK;
@@ -382,8 +386,7 @@ int sg_eigh(sg_batch* b) {
   HIP_TRY(hipSetDevice(b->ctx->device));
   HIP_TRY(hipMemsetAsync(b->d.status, 0, sizeof(int) * b->d.n_mol, b->ctx->stream));
   {
-    StageTimer t(b->ctx, ST_EIGH);
-    launch_eigh(b->d, b->ctx->stream, true);
+    timed_eigh(b->ctx, b->d, b->ctx->stream, true);
   }
   return check_launch("eigh");
 }
@@ -459,28 +462,36 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
   int n_active = max_steps > 0 ? b->d.n_mol : 0;
   int host_active = 0;
   const int fp32_cap = 40;
+  // Without stage timing, one iteration is captured once as a HIP graph and
+  // replayed; its kernel arguments depend on the batch and on these settings.
+  const bool use_graph = !c->timing;
+  if (use_graph && (!b->iter_exec || b->g_threshold != threshold || b->g_max_steps != max_steps ||
+                    b->g_switch != switch_ev)) {
+    if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
+    b->iter_exec = nullptr;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    scf_iteration(c, b->d, s, threshold, max_steps, switch_ev, fp32_cap);
+    HIP_TRY(hipStreamEndCapture(s, &g));
+    if ((r = check_launch("scf step capture"))) {
+      (void)hipGraphDestroy(g);
+      return r;
+    }
+    hipError_t ge = hipGraphInstantiate(&b->iter_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ge != hipSuccess) {
+      b->iter_exec = nullptr;
+      return fail(SG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ge));
+    }
+    b->g_threshold = threshold;
+    b->g_max_steps = max_steps;
+    b->g_switch = switch_ev;
+  }
   for (int it = 0; it < max_steps && n_active > 0; ++it) {
-    {
-      StageTimer t(c, ST_EIGH);
-      launch_eigh(b->d, s, false);
-    }
-    {
-      StageTimer t(c, ST_DENSITY);
-      launch_density(b->d, s, false);
-    }
-    {
-      StageTimer t(c, ST_FOCK);
-      launch_fock(b->d, s, false);
-    }
-    {
-      StageTimer t(c, ST_ENERGY);
-      launch_energy(b->d, s, b->d.e_new, false);
-    }
-    {
-      StageTimer t(c, ST_CONTROL);
-      HIP_TRY(hipMemsetAsync(b->d.n_active, 0, sizeof(int), s));
-      launch_scf_update(b->d, s, threshold, max_steps, switch_ev, fp32_cap);
-    }
+    if (use_graph)
+      HIP_TRY(hipGraphLaunch(b->iter_exec, s));
+    else
+      scf_iteration(c, b->d, s, threshold, max_steps, switch_ev, fp32_cap);
     if ((r = check_launch("scf step"))) return r;
     HIP_TRY(hipMemcpyAsync(&host_active, b->d.n_active, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));

@@ -309,6 +309,10 @@ __global__ void k_clear_hf(const MolDesc* __restrict__ mols, double* __restrict_
 
 static inline int blocks_for(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
+__global__ void k_reset_count(int* __restrict__ n) { *n = 0; }
+
+void launch_reset_count(const BatchDev& b, hipStream_t s) { k_reset_count<<<1, 64, 0, s>>>(b.n_active); }
+
 void launch_clear_hf(const BatchDev& b, hipStream_t s, const int* mask) {
   if (b.n_mol) k_clear_hf<<<dim3(8, b.n_mol), 256, 0, s>>>(b.mols, b.H, b.F, mask);
 }

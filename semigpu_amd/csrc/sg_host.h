@@ -20,7 +20,11 @@ int fail(int code, const std::string& msg);
       return ::sg::fail(SG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
   } while (0)
 
-enum { ST_TWO_CENTER = 0, ST_HCORE, ST_GUESS, ST_FOCK, ST_EIGH, ST_DENSITY, ST_ENERGY, ST_CONTROL };
+enum {
+  ST_TWO_CENTER = 0, ST_HCORE, ST_GUESS, ST_FOCK, ST_TRIDIAG, ST_BISECT, ST_INVIT, ST_BACK,
+  ST_DENSITY, ST_ENERGY, ST_CONTROL
+};
+static_assert(ST_CONTROL + 1 == SG_N_STAGES, "stage list and SG_N_STAGES disagree");
 
 int check_launch(const char* what);
 bool supported_element(int z);
@@ -57,5 +61,52 @@ struct StageTimer {
     }
   }
 };
+
+// One SCF iteration of the active molecules (scfcalculation.cpp:82-146):
+// eigensolve, density, Fock, energy, convergence bookkeeping.
+inline void scf_iteration(sg_context* c, const BatchDev& d, hipStream_t s, double threshold,
+                          int max_steps, double switch_ev, int fp32_cap);
+
+// The eigensolver's four kernels, each timed as its own stage.
+inline void timed_eigh(sg_context* c, const BatchDev& d, hipStream_t s, bool all) {
+  {
+    StageTimer t(c, ST_TRIDIAG);
+    launch_eigh_tridiag(d, s, all);
+  }
+  {
+    StageTimer t(c, ST_BISECT);
+    launch_eigh_bisect(d, s, all);
+  }
+  {
+    StageTimer t(c, ST_INVIT);
+    launch_eigh_invit(d, s, all);
+  }
+  {
+    StageTimer t(c, ST_BACK);
+    launch_eigh_back(d, s, all);
+  }
+}
+
+inline void scf_iteration(sg_context* c, const BatchDev& d, hipStream_t s, double threshold,
+                          int max_steps, double switch_ev, int fp32_cap) {
+  timed_eigh(c, d, s, false);
+  {
+    StageTimer t(c, ST_DENSITY);
+    launch_density(d, s, false);
+  }
+  {
+    StageTimer t(c, ST_FOCK);
+    launch_fock(d, s, false);
+  }
+  {
+    StageTimer t(c, ST_ENERGY);
+    launch_energy(d, s, d.e_new, false);
+  }
+  {
+    StageTimer t(c, ST_CONTROL);
+    launch_reset_count(d, s);
+    launch_scf_update(d, s, threshold, max_steps, switch_ev, fp32_cap);
+  }
+}
 
 }  // namespace sg

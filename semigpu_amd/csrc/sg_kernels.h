@@ -64,11 +64,21 @@ void launch_energy(const BatchDev& b, hipStream_t s, double* out, bool all,
                    const int* mask = nullptr);
 void launch_scf_init(const BatchDev& b, hipStream_t s, int max_steps, int policy,
                      const int* mask = nullptr);
+// Zeroes the device counter of molecules still iterating (a kernel rather than
+// a memset, so that the SCF iteration can be captured in a HIP graph).
+void launch_reset_count(const BatchDev& b, hipStream_t s);
 void launch_scf_update(const BatchDev& b, hipStream_t s, double threshold, int max_steps,
                        double switch_ev, int fp32_cap);
 void launch_blocks_to_fp32(const BatchDev& b, hipStream_t s);
 void launch_set_prec(const BatchDev& b, hipStream_t s, int v, const int* mask = nullptr);
 void launch_clear_hf(const BatchDev& b, hipStream_t s, const int* mask);
+// Eigensolver of F (all = every eigenpair, else the selected subspace of the
+// active molecules): tridiagonalisation, bisection, inverse iteration,
+// back-transformation; launch_eigh runs the four in order.
+void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all);
+void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all);
+void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all);
+void launch_eigh_back(const BatchDev& b, hipStream_t s, bool all);
 void launch_eigh(const BatchDev& b, hipStream_t s, bool all);
 void launch_density(const BatchDev& b, hipStream_t s, bool all);
 

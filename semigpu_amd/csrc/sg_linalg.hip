@@ -800,26 +800,45 @@ static size_t tridiag_lds(int max_n) {
   return sizeof(double) * ((size_t)kPanelDoubles + max_n + 32 + (size_t)(kT / 64) * max_n);
 }
 
-void launch_eigh(const BatchDev& b, hipStream_t s, bool all) {
+void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all) {
   const int* act = all ? nullptr : b.active;
-  const int want_all = all ? 1 : 0;
   const int N = b.max_n;
   if (N <= 384)
     k_tridiag<6><<<b.n_mol, kT, tridiag_lds(N), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status);
   else
     k_tridiag<20><<<b.n_mol, kT, tridiag_lds(N), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status);
+}
+
+void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all) {
+  const int N = b.max_n;
   dim3 gb((N + 255) / 256, b.n_mol);
   k_bisect<<<gb, 256, sizeof(double) * (2 * N + 32), s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm,
-                                                          want_all, act, b.status);
+                                                          all ? 1 : 0, all ? nullptr : b.active,
+                                                          b.status);
+}
+
+void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all) {
+  const int N = b.max_n;
   const size_t inv_lds = sizeof(double) * (3 * (size_t)N) + sizeof(int) * (2 * (size_t)N + 2);
-  k_invit<<<b.n_mol, kT, inv_lds, s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm, b.W, want_all, act,
-                                       b.status);
+  k_invit<<<b.n_mol, kT, inv_lds, s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm, b.W, all ? 1 : 0,
+                                       all ? nullptr : b.active, b.status);
+}
+
+void launch_eigh_back(const BatchDev& b, hipStream_t s, bool all) {
+  const int* act = all ? nullptr : b.active;
   const size_t bt_lds = sizeof(double) * ((size_t)kPanelDoubles / 2 + 2 * kNbMax * kNbMax);
-  k_backtransform<<<b.n_mol, kT, bt_lds, s>>>(b.mols, b.W, b.tau, want_all, act, b.status);
+  k_backtransform<<<b.n_mol, kT, bt_lds, s>>>(b.mols, b.W, b.tau, all ? 1 : 0, act, b.status);
   if (all) {
     dim3 gw(64, b.n_mol);
     k_write_eigvec<<<gw, 256, 0, s>>>(b.mols, b.W, b.C, act, b.status);
   }
+}
+
+void launch_eigh(const BatchDev& b, hipStream_t s, bool all) {
+  launch_eigh_tridiag(b, s, all);
+  launch_eigh_bisect(b, s, all);
+  launch_eigh_invit(b, s, all);
+  launch_eigh_back(b, s, all);
 }
 
 void launch_density(const BatchDev& b, hipStream_t s, bool all) {

@@ -432,27 +432,7 @@ int sg_server_step(sg_server* S, int* n_inflight, int* n_queued) {
   }
   // 2. one SCF iteration for every molecule in flight
   if (hw > 0) {
-    {
-      StageTimer t(c, ST_EIGH);
-      launch_eigh(it, s, false);
-    }
-    {
-      StageTimer t(c, ST_DENSITY);
-      launch_density(it, s, false);
-    }
-    {
-      StageTimer t(c, ST_FOCK);
-      launch_fock(it, s, false);
-    }
-    {
-      StageTimer t(c, ST_ENERGY);
-      launch_energy(it, s, it.e_new, false);
-    }
-    {
-      StageTimer t(c, ST_CONTROL);
-      HIP_TRY(hipMemsetAsync(it.n_active, 0, sizeof(int), s));
-      launch_scf_update(it, s, S->threshold, S->max_steps, S->switch_ev, 40);
-    }
+    scf_iteration(c, it, s, S->threshold, S->max_steps, S->switch_ev, 40);
     int r = check_launch("server iteration");
     if (r) return r;
     std::vector<int> act(hw), st(hw), cv(hw), ss(hw);

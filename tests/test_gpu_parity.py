@@ -169,3 +169,52 @@ def test_mixed_policy_water_batch(engine):
     say("mixed vs fp64 worst |dE|", worst)
     assert all(r["converged"] for r in mix)
     assert worst <= 1e-4
+
+
+def test_graph_replay_equals_eager(engine):
+    """The captured SCF iteration (replayed once per step, with eager setup
+    work and host synchronisation between replays) gives the eager results."""
+    mols = geometry.water_batch(24, seed=99, n_min=2, n_max=30)
+    b = engine.batch(mols)
+    for policy in ("fp64", "mixed"):
+        g1 = b.scf(threshold=1e-6, policy=policy)
+        engine.timing(True)
+        eager = b.scf(threshold=1e-6, policy=policy)
+        engine.timing(False)
+        engine.timing_reset()
+        g2 = b.scf(threshold=1e-6, policy=policy)
+        assert g1 == eager == g2, policy
+    b.close()
+
+
+def test_large_cluster_against_oracle(engine):
+    """N = 600 takes the long-row tridiagonalisation path (N > 384)."""
+    import pyoracle as po
+    mol = geometry.water_cluster(100, geometry.SplitMix64(4242))
+    r = engine.batch([mol]).scf(threshold=1e-6)[0]
+    o = po.Molecule(mol).scf(threshold=1e-6, mode="nddo")
+    say("N=600", r["steps"], o["steps"], r["e_tot"] - o["e_tot"])
+    assert r["converged"] and r["steps"] == o["steps"]
+    assert abs(r["e_tot"] - o["e_tot"]) <= 1e-6
+
+
+def test_eigensolver_largest_basis(engine):
+    """N = 1200 (the largest cluster of the serving stream) against numpy."""
+    mol = geometry.water_cluster(200, geometry.SplitMix64(77))
+    b = engine.batch([mol])
+    n = b.n_ao(0)
+    assert n == 1200
+    b.two_center()
+    b.hcore()
+    b.guess_density()
+    b.fock()
+    b.eigh()
+    w, C = b.eigen(0)
+    import pyoracle as po
+    F = po.packed_to_full(b.packed(0, api.SG_FOCK), n)
+    wr = np.linalg.eigh(F)[0]
+    scale = np.abs(wr).max()
+    assert np.abs(w - wr).max() <= 1e-12 * scale
+    assert np.abs(C.T @ C - np.eye(n)).max() <= 1e-12
+    assert np.abs(F @ C - C * w).max() <= 1e-11 * scale
+    b.close()
