@@ -57,6 +57,10 @@ int sg_ctx_destroy(sg_context* ctx);
  * SCFCalculation::AllocSCFData (scfcalculation.cpp:28-46). */
 int sg_batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, const int* Z,
                     const double* xyz_ang, sg_batch** out);
+/* Same, with coordinates already in bohr (the AtomicOrbital::coordinates of
+ * atomicOrbitals.h:24, as the reference's kernels see them). */
+int sg_batch_create_bohr(sg_context* ctx, int n_mol, const int* atom_offsets, const int* Z,
+                         const double* xyz_bohr, sg_batch** out);
 int sg_batch_destroy(sg_batch* b);
 int sg_batch_molecule_info(const sg_batch* b, int mol, int* n_atoms, int* n_ao, int* n_occ);
 
@@ -96,16 +100,27 @@ int sg_scf_run(sg_batch* b, double threshold_ev, int max_steps, sg_scf_result* r
 int sg_scf_run_policy(sg_batch* b, double threshold_ev, int max_steps, int policy,
                       double switch_ev, sg_scf_result* results);
 
+/* Per-molecule eigensolver status of the last sg_eigh / SCF step
+ * (status[n_mol]; > 0 where dsyev would report info > 0). */
+int sg_get_status(const sg_batch* b, int* status);
+
 /* Host copies for the reference's host-side views. */
 long sg_pair_block_count(const sg_batch* b, int mol);
 /* double**** all2CenterIntegral contents in the reference jagged order
  * [i][j<=i][k][l] (twocenterintegral.cpp:55-107); read by GetValueFromArray. */
 int sg_get_pair_blocks(const sg_batch* b, int mol, double* out);
+/* The reverse copy (TwoCenterIntegral::To_device, twocenterintegral.cpp:23-53):
+ * host pair blocks become the device's (FP64 and the FP32 copy). */
+int sg_set_pair_blocks(sg_batch* b, int mol, const double* in);
 /* BaseMatrix::matrixHold_ of Hcore / FockMatrix / DensityMatrix (packed). */
 int sg_get_packed(const sg_batch* b, int mol, int which, double* out);
 int sg_set_packed(sg_batch* b, int mol, int which, const double* in);
 /* SCFCalculation::eigenVal / eigenVec (eigenvectors column-major, n_ao*n_ao). */
 int sg_get_eigen(const sg_batch* b, int mol, double* eigval, double* eigvec);
+/* Uploads eigenvalues and/or eigenvectors (either may be NULL), so that
+ * sg_density builds P from a host eigenVec as DensityMatrix does
+ * (densitymatrix.h:13, the eigenVec pointer it aliases). */
+int sg_set_eigen(sg_batch* b, int mol, const double* eigval, const double* eigvec);
 
 /* Continuous-batching SCF server (no counterpart in the reference, whose main()
  * runs one molecule per process, semiGPU.cc:30-110).  Requests (one molecule

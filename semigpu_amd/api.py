@@ -33,7 +33,8 @@ EXPORTS = (
     "sg_guess_density", "sg_fock", "sg_eigh", "sg_density", "sg_energy", "sg_scf_run",
     "sg_scf_run_policy",
     "sg_pair_block_count", "sg_get_pair_blocks", "sg_get_packed", "sg_set_packed",
-    "sg_get_eigen", "sg_timing_enable", "sg_timing_read", "sg_timing_reset",
+    "sg_get_eigen", "sg_set_eigen", "sg_batch_create_bohr", "sg_set_pair_blocks",
+    "sg_get_status", "sg_timing_enable", "sg_timing_read", "sg_timing_reset",
     "sg_server_create", "sg_server_destroy", "sg_server_submit", "sg_server_step",
     "sg_server_poll", "sg_server_clock",
 )
@@ -91,6 +92,10 @@ def load_library(path: str = LIB_PATH):
     L.sg_get_packed.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp]
     L.sg_set_packed.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp]
     L.sg_get_eigen.argtypes = [_vp, ctypes.c_int, _dp, _dp]
+    L.sg_set_eigen.argtypes = [_vp, ctypes.c_int, _dp, _dp]
+    L.sg_set_pair_blocks.argtypes = [_vp, ctypes.c_int, _dp]
+    L.sg_get_status.argtypes = [_vp, _ip]
+    L.sg_batch_create_bohr.argtypes = [_vp, ctypes.c_int, _ip, _ip, _dp, ctypes.POINTER(_vp)]
     L.sg_timing_enable.argtypes = [_vp, ctypes.c_int]
     L.sg_timing_read.argtypes = [_vp, _dp, _ip]
     L.sg_timing_reset.argtypes = [_vp]

@@ -120,8 +120,22 @@ int sg_timing_reset(sg_context* c) {
   return r;
 }
 
+static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, const int* Z,
+                        const double* xyz_ang, double to_bohr, sg_batch** out);
+
 int sg_batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, const int* Z,
                     const double* xyz_ang, sg_batch** out) {
+  // AtomicOrbital::SetCoordinates: Angstrom -> bohr (atomicOrbitals.cpp:40-44)
+  return batch_create(ctx, n_mol, atom_offsets, Z, xyz_ang, 1.8897259886, out);
+}
+
+int sg_batch_create_bohr(sg_context* ctx, int n_mol, const int* atom_offsets, const int* Z,
+                         const double* xyz_bohr, sg_batch** out) {
+  return batch_create(ctx, n_mol, atom_offsets, Z, xyz_bohr, 1.0, out);
+}
+
+static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, const int* Z,
+                        const double* xyz_ang, double to_bohr, sg_batch** out) {
   if (!ctx || !out || n_mol <= 0 || !atom_offsets || !Z || !xyz_ang)
     return fail(SG_ERR_INVALID, "sg_batch_create: bad arguments");
   if (atom_offsets[0] != 0) return fail(SG_ERR_INVALID, "atom_offsets[0] must be 0");
@@ -155,10 +169,9 @@ int sg_batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, const i
       const int gi = M.atom0 + a;
       AtomDesc& A = b->atoms[gi];
       A.zn = Z[gi];
-      // AtomicOrbital::SetCoordinates: Angstrom -> bohr (atomicOrbitals.cpp:40-44)
-      A.x = xyz_ang[3 * gi + 0] * 1.8897259886;
-      A.y = xyz_ang[3 * gi + 1] * 1.8897259886;
-      A.z = xyz_ang[3 * gi + 2] * 1.8897259886;
+      A.x = xyz_ang[3 * gi + 0] * to_bohr;
+      A.y = xyz_ang[3 * gi + 1] * to_bohr;
+      A.z = xyz_ang[3 * gi + 2] * to_bohr;
       A.mol = m;
       A.local = a;
       A.ao0 = ao;
@@ -581,6 +594,45 @@ int sg_get_eigen(const sg_batch* b, int mol, double* eigval, double* eigvec) {
                            b->ctx->stream));
   if (eigvec)
     HIP_TRY(hipMemcpyAsync(eigvec, b->d.C + M.mat0, sizeof(double) * n * n, hipMemcpyDeviceToHost,
+                           b->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  return SG_OK;
+}
+
+int sg_set_pair_blocks(sg_batch* b, int mol, const double* in) {
+  if (!b || !in || mol < 0 || mol >= b->d.n_mol) return fail(SG_ERR_INVALID, "bad arguments");
+  const int64_t n = b->blk_count[mol];
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  std::vector<float> f(n);
+  for (int64_t i = 0; i < n; ++i) f[i] = (float)in[i];
+  HIP_TRY(hipMemcpyAsync(b->d.blocks + b->mols[mol].blk0, in, sizeof(double) * n,
+                         hipMemcpyHostToDevice, b->ctx->stream));
+  HIP_TRY(hipMemcpyAsync(b->d.blocks32 + b->mols[mol].blk0, f.data(), sizeof(float) * n,
+                         hipMemcpyHostToDevice, b->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  b->have_ints = true;
+  return SG_OK;
+}
+
+int sg_get_status(const sg_batch* b, int* status) {
+  if (!b || !status) return fail(SG_ERR_INVALID, "bad arguments");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  HIP_TRY(hipMemcpyAsync(status, b->d.status, sizeof(int) * b->d.n_mol, hipMemcpyDeviceToHost,
+                         b->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  return SG_OK;
+}
+
+int sg_set_eigen(sg_batch* b, int mol, const double* eigval, const double* eigvec) {
+  if (!b || mol < 0 || mol >= b->d.n_mol) return fail(SG_ERR_INVALID, "bad arguments");
+  const MolDesc& M = b->mols[mol];
+  const size_t n = M.n_ao;
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  if (eigval)
+    HIP_TRY(hipMemcpyAsync(b->d.eig + M.vec0, eigval, sizeof(double) * n, hipMemcpyHostToDevice,
+                           b->ctx->stream));
+  if (eigvec)
+    HIP_TRY(hipMemcpyAsync(b->d.C + M.mat0, eigvec, sizeof(double) * n * n, hipMemcpyHostToDevice,
                            b->ctx->stream));
   HIP_TRY(hipStreamSynchronize(b->ctx->stream));
   return SG_OK;
