@@ -156,9 +156,10 @@ def serve(eng, rate: float, seconds: float, threshold: float, policy: str, seed:
 def calib_copy(nbytes: int = 1 << 30):
     """One plain device copy of known size, the scale for the PMC byte counters."""
     import torch
-    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
-    y.copy_(x)
+    torch.cuda.synchronize()
+    torch.mul(x, 2.0, out=y)  # one elementwise kernel: reads and writes nbytes each
     torch.cuda.synchronize()
     print("copy kernel: %d bytes read, %d bytes written" % (nbytes, nbytes), file=sys.stderr,
           flush=True)

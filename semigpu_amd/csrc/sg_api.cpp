@@ -243,7 +243,7 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   const size_t o_steps = take(sizeof(int) * n_mol), o_act = take(sizeof(int) * n_mol),
                o_conv = take(sizeof(int) * n_mol), o_stat = take(sizeof(int) * n_mol),
                o_nact = take(sizeof(int) * 64), o_prec = take(sizeof(int) * n_mol),
-               o_pprec = take(sizeof(int) * n_mol);
+               o_pprec = take(sizeof(int) * n_mol), o_order = take(sizeof(int) * n_mol);
   (void)hipSetDevice(ctx->device);
   hipError_t e = hipMalloc(&b->dev_mem, off);
   if (e != hipSuccess) {
@@ -281,6 +281,8 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   d.n_active = (int*)(base + o_nact);
   d.prec = (int*)(base + o_prec);
   d.prev_prec = (int*)(base + o_pprec);
+  int* order = (int*)(base + o_order);
+  d.order = order;
   hipStream_t s = ctx->stream;
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
@@ -292,6 +294,12 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   if (e == hipSuccess) e = up(d.pairs_hh, phh.data(), sizeof(int2) * phh.size());
   if (e == hipSuccess) e = up(d.pairs_all, pall.data(), sizeof(int2) * pall.size());
   if (e == hipSuccess) e = up(d.diag_tasks, diag.data(), sizeof(int2) * diag.size());
+  // largest molecules first in the per-molecule launches
+  std::vector<int> ord(n_mol);
+  for (int m = 0; m < n_mol; ++m) ord[m] = m;
+  std::stable_sort(ord.begin(), ord.end(),
+                   [&](int x, int y) { return b->mols[x].n_ao > b->mols[y].n_ao; });
+  if (e == hipSuccess) e = up(order, ord.data(), sizeof(int) * n_mol);
   for (double* z : {d.H, d.F, d.P, d.C})
     if (e == hipSuccess) e = hipMemsetAsync(z, 0, sizeof(double) * mat0, s);
   if (e == hipSuccess) e = hipMemsetAsync(d.status, 0, sizeof(int) * n_mol, s);

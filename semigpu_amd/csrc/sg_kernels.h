@@ -51,6 +51,10 @@ struct BatchDev {
   int* prec = nullptr;       // precision of the next SCF step: 1 = FP32 contractions, 0 = FP64
   int* prev_prec = nullptr;  // precision of the step just done
   int max_n = 0;             // largest N in the batch
+  // Molecule indices by decreasing N (or null for 0..n_mol-1): the per-molecule
+  // eigensolver and density launches take their workgroups in this order, so
+  // the longest jobs start first and the launch tail is short.
+  const int* order = nullptr;
 };
 
 // `mask` (optional, per molecule) restricts a launch to the molecules whose

@@ -8,17 +8,17 @@
 //  k_tridiag        Householder reduction to tridiagonal form, blocked as in
 //                   LAPACK dsytrd/dlatrd: a panel of nb reflectors is built with
 //                   its V and W columns in LDS (one matrix-vector product over the
-//                   trailing block per column), then the trailing block gets one
-//                   rank-2nb update.  The matrix is kept in full symmetric
-//                   storage so the product is a row-wise dot (one wavefront per
-//                   row, coalesced, no cross-wave reduction).  Reflector c is
-//                   stored in row c of the work matrix.
+//                   lower triangle of the trailing block per column, 8 rows per
+//                   wavefront), then the trailing block gets one rank-2nb update
+//                   on the matrix cores.  Reflector c is stored in the upper part
+//                   of row c of the work matrix.
 //  k_bisect         eigenvalues of the tridiagonal matrix by bisection on the
 //                   Sturm count, one lane per eigenvalue (only the indices the
 //                   caller selected).
 //  k_invit          eigenvectors of the tridiagonal matrix by inverse iteration
 //                   (3 sweeps, Gaussian elimination with partial pivoting, one
-//                   lane per vector).  Eigenvalues closer than 1e-3*||T|| form a
+//                   lane per vector).  Eigenvalues closer than 1e-3*||T|| (full
+//                   decomposition) or 1e-5*||T|| (SCF subspace) form a
 //                   cluster; after each sweep a cluster's vectors are
 //                   orthonormalised in order (modified Gram-Schmidt) by one
 //                   wavefront, as LAPACK dstein does.
@@ -86,6 +86,30 @@ __device__ __forceinline__ int panel_width(int n) {
   return nb < 1 ? 1 : (nb > kNbMax ? kNbMax : nb);
 }
 
+// Operand layout of the 16x16x4 matrix-core instructions: lane l supplies
+// A[l%16][l/16] and B[l/16][l%16].  Result register r of lane l holds
+// D[i][l%16] with i = 4*(l/16) + r for the f32 instruction and i = 4*r + l/16
+// for the f64 one (measured on gfx950).
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * r + (lane >> 4); }
+};
+template <>
+struct Mfma16<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+
 // ---------------------------------------------------------------------------
 // Lower-triangle storage: only A[i][l], l <= i, is read and updated; reflector
 // c is kept in the (otherwise unused) upper part of row c.  TMAX bounds the
@@ -97,9 +121,10 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_
                                                  double* __restrict__ dA, double* __restrict__ eA,
                                                  double* __restrict__ tauA,
                                                  const int* __restrict__ active,
-                                                 const int* __restrict__ status) {
+                                                 const int* __restrict__ status,
+                                                 const int* __restrict__ order) {
   extern __shared__ double lds[];
-  const int m = blockIdx.x;
+  const int m = order ? order[blockIdx.x] : blockIdx.x;
   if (!mol_live(active, status, m)) return;
   const MolDesc M = mols[m];
   const int n = M.n_ao, tid = threadIdx.x, nt = blockDim.x;
@@ -111,6 +136,8 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_
   double* tv = x + n;                // 2 * nb <= 16
   double* red = tv + 16;             // nwave
   double* qb = red + 16;             // nwave x n partial transposed products
+  double* colbuf = qb + nwave * n;   // n: column c+1 of A, read during the product for c
+  double* pan = colbuf + n;          // 3 x kNbMax: d, e, tau of the panel's columns
   double* A = work + 5 * M.mat0;
   double* d = dA + M.vec0;
   double* e = eA + M.vec0;
@@ -130,7 +157,7 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_
       const int c = k0 + j;
       // column c of the partially reduced matrix (rows c..n-1, lower part)
       for (int i = c + tid; i < n; i += nt) {
-        double s = A[(int64_t)i * n + c];
+        double s = j > 0 ? colbuf[i] : A[(int64_t)i * n + c];
         for (int q = 0; q < j; ++q)
           s -= V[q * n + i] * Wp[q * n + c] + Wp[q * n + i] * V[q * n + c];
         x[i] = s;
@@ -140,36 +167,39 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_
       for (int i = c + 2 + tid; i < n; i += nt) s2 += x[i] * x[i];
       s2 = block_sum(s2, red);
       const double alpha = x[c + 1];
-      if (tid == 0) d[c] = x[c];
+      // d, e, tau of the panel stay in LDS and the reflectors in V until the
+      // panel ends, so the barriers of the column steps wait on no global store
+      if (tid == 0) pan[j] = x[c];
       if (s2 == 0.0) {
         if (tid == 0) {
-          tau[c] = 0.0;
-          e[c] = alpha;
+          pan[kNbMax + j] = alpha;
+          pan[2 * kNbMax + j] = 0.0;
         }
-        for (int i = c + 1 + tid; i < n; i += nt) A[(int64_t)c * n + i] = 0.0;
+        if (j + 1 < b)
+          for (int i = c + 1 + tid; i < n; i += nt) colbuf[i] = A[(int64_t)i * n + c + 1];
         __syncthreads();
         continue;
       }
       const double beta = -copysign(sqrt(alpha * alpha + s2), alpha);
       const double tk = (beta - alpha) / beta;
       const double sc = 1.0 / (alpha - beta);
-      for (int i = c + 1 + tid; i < n; i += nt) {
-        const double v = i == c + 1 ? 1.0 : x[i] * sc;
-        V[j * n + i] = v;
-        A[(int64_t)c * n + i] = v;
-      }
+      for (int i = c + 1 + tid; i < n; i += nt) V[j * n + i] = i == c + 1 ? 1.0 : x[i] * sc;
       if (tid == 0) {
-        tau[c] = tk;
-        e[c] = beta;
+        pan[kNbMax + j] = beta;
+        pan[2 * kNbMax + j] = tk;
       }
       __syncthreads();
       // p = A22 v from the lower triangle of the trailing block.  A wave takes
       // 8 consecutive rows: row parts are dotted with v, and each row also adds
       // A[i][l] v_i to lane-owned accumulators for column l < i (the upper half).
+      // Tiles entirely left of the chunk's first row need no guards.  The 8 row
+      // sums are reduced together (each butterfly step halves the values a lane
+      // carries), and lane 0 keeps column c+1 for the next reflector.
       {
         const int m1 = c + 1;
         const int T = (n - m1 + 63) >> 6;
         const double* vj = V + j * n;
+        const bool grab = j + 1 < b;
         double q[TMAX];
 #pragma unroll
         for (int t = 0; t < TMAX; ++t) q[t] = 0.0;
@@ -181,16 +211,29 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_
             vi[r] = i0 + r < n ? vj[i0 + r] : 0.0;
           }
           const int tmax = (min(i0 + 7, n - 1) - m1) >> 6;
+          const int tfull = i0 + 7 < n ? (i0 - m1) >> 6 : 0;
+          const double* arow = A + (int64_t)i0 * n;
 #pragma unroll
           for (int t = 0; t < TMAX; ++t) {
-            if (t <= tmax) {
-              const int l = m1 + lane + 64 * t;
+            const int l = m1 + lane + 64 * t;
+            double a[8];
+            if (t < tfull) {
+              const double vl = vj[l];
+#pragma unroll
+              for (int r = 0; r < 8; ++r) a[r] = arow[(int64_t)r * n + l];
+              double qs = 0.0;
+#pragma unroll
+              for (int r = 0; r < 8; ++r) {
+                acc[r] += a[r] * vl;
+                qs += a[r] * vi[r];
+              }
+              q[t] += qs;
+            } else if (t <= tmax) {
               const double vl = l < n ? vj[l] : 0.0;
-              double a[8];
 #pragma unroll
               for (int r = 0; r < 8; ++r) {
                 const int i = i0 + r;
-                a[r] = (i < n && l <= i) ? A[(int64_t)i * n + l] : 0.0;
+                a[r] = (i < n && l <= i) ? arow[(int64_t)r * n + l] : 0.0;
               }
               double qs = 0.0;
 #pragma unroll
@@ -200,12 +243,40 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_
               }
               q[t] += qs;
             }
-          }
+            if (t == 0 && grab && lane == 0) {
 #pragma unroll
-          for (int r = 0; r < 8; ++r) {
-            const double sr = wsum(acc[r]);
-            if (lane == r && i0 + r < n) x[i0 + r] = sr;
+              for (int r = 0; r < 8; ++r)
+                if (i0 + r < n) colbuf[i0 + r] = a[r];
+            }
           }
+          // lane l ends with the sum of row 4*b5 + 2*b4 + b3 (b = bits of l)
+          double v4[4], v2[2], v1;
+          {
+            const bool hi = lane & 32;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const double send = hi ? acc[k] : acc[k + 4];
+              v4[k] = (hi ? acc[k + 4] : acc[k]) + __shfl_xor(send, 32, 64);
+            }
+          }
+          {
+            const bool hi = lane & 16;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+              const double send = hi ? v4[k] : v4[k + 2];
+              v2[k] = (hi ? v4[k + 2] : v4[k]) + __shfl_xor(send, 16, 64);
+            }
+          }
+          {
+            const bool hi = lane & 8;
+            const double send = hi ? v2[0] : v2[1];
+            v1 = (hi ? v2[1] : v2[0]) + __shfl_xor(send, 8, 64);
+          }
+          v1 += __shfl_xor(v1, 4, 64);
+          v1 += __shfl_xor(v1, 2, 64);
+          v1 += __shfl_xor(v1, 1, 64);
+          const int rho = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+          if ((lane & 7) == 0 && i0 + rho < n) x[i0 + rho] = v1;
         }
 #pragma unroll
         for (int t = 0; t < TMAX; ++t) {
@@ -236,35 +307,67 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_
       for (int i = c + 1 + tid; i < n; i += nt) Wp[j * n + i] = x[i] + al * V[j * n + i];
       __syncthreads();
     }
-    // trailing update of the lower triangle (rows >= k0 + b, columns r..i):
-    // one wave per row, lanes over columns, 4 column chunks per pass
+    // the panel's reflectors into the upper part of their rows; d, e, tau
+    for (int t = tid; t < b * n; t += nt) {
+      const int q = t / n, i = t - q * n, c = k0 + q;
+      if (i > c) A[(int64_t)c * n + i] = V[t];
+    }
+    if (tid < b) {
+      d[k0 + tid] = pan[tid];
+      e[k0 + tid] = pan[kNbMax + tid];
+      tau[k0 + tid] = pan[2 * kNbMax + tid];
+    }
+    // Trailing update A22 -= V W^T + W V^T on the matrix cores: one wave per
+    // 16 x 16 tile of the lower triangle (diagonal tiles in full: their upper
+    // part is unused until it receives a later reflector), K = 2 nb in steps of
+    // 4 with v_mfma_f64_16x16x4f64.  Operands at rows >= n only reach tile
+    // entries that are not stored.
     const int r = k0 + b;
-    for (int i = r + wave; i < n; i += nwave) {
-      double vi[kNbMax], wi[kNbMax];
+    if (r < n) {
+      const int TT = (n - r + 15) >> 4;
+      const int ntile = TT * (TT + 1) / 2;
+      const int lr = lane & 15, lk = lane >> 4;
+      const int nsteps = (2 * nb + 3) >> 2;
+      // this wave's tiles: linear indices wave, wave + nwave, ... of the lower
+      // triangle in row order; the next tile loads before this one is updated
+      auto load = [&](int ti, int tj, Mfma16<double>::v4& v) {
+        const int col = r + 16 * tj + lr;
 #pragma unroll
-      for (int qq = 0; qq < kNbMax; ++qq) {
-        vi[qq] = qq < b ? V[qq * n + i] : 0.0;
-        wi[qq] = qq < b ? Wp[qq * n + i] : 0.0;
-      }
-      double* ai = A + (int64_t)i * n;
-      for (int l0 = r + lane; l0 <= i; l0 += 256) {
-        double a[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int l = l0 + 64 * u;
-          a[u] = l <= i ? ai[l] : 0.0;
+        for (int rr = 0; rr < 4; ++rr) {
+          const int row = r + 16 * ti + 4 * rr + lk;
+          v[rr] = (row < n && col < n) ? A[(int64_t)row * n + col] : 0.0;
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int l = l0 + 64 * u;
-          if (l <= i) {
-            double sacc = a[u];
-#pragma unroll
-            for (int qq = 0; qq < kNbMax; ++qq)
-              if (qq < b) sacc -= vi[qq] * Wp[qq * n + l] + wi[qq] * V[qq * n + l];
-            ai[l] = sacc;
+      };
+      int ti = 0, tj = wave;
+      while (tj > ti) tj -= ++ti;
+      Mfma16<double>::v4 cur = {0, 0, 0, 0}, nxt = {0, 0, 0, 0};
+      if (wave < ntile) load(ti, tj, cur);
+      for (int idx = wave; idx < ntile; idx += nwave) {
+        int ni = ti, nj = tj + nwave;
+        while (nj > ni) nj -= ++ni;
+        if (idx + nwave < ntile) load(ni, nj, nxt);
+        const int I0 = r + 16 * ti, J0 = r + 16 * tj;
+        for (int st = 0; st < nsteps; ++st) {
+          const int k = 4 * st + lk;
+          double av = 0.0, bv = 0.0;
+          if (k < nb) {
+            av = -V[k * n + I0 + lr];
+            bv = Wp[k * n + J0 + lr];
+          } else if (k < 2 * nb) {
+            av = -Wp[(k - nb) * n + I0 + lr];
+            bv = V[(k - nb) * n + J0 + lr];
           }
+          cur = Mfma16<double>::op(av, bv, cur);
         }
+        const int col = J0 + lr;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int row = I0 + 4 * rr + lk;
+          if (row < n && col < n) A[(int64_t)row * n + col] = cur[rr];
+        }
+        cur = nxt;
+        ti = ni;
+        tj = nj;
       }
     }
     __syncthreads();
@@ -301,9 +404,10 @@ __global__ void __launch_bounds__(256) k_bisect(const MolDesc* __restrict__ mols
                                                  double* __restrict__ lamA,
                                                  double* __restrict__ tnormA, int want_all,
                                                  const int* __restrict__ active,
-                                                 const int* __restrict__ status) {
+                                                 const int* __restrict__ status,
+                                                 const int* __restrict__ order) {
   extern __shared__ double lds[];
-  const int m = blockIdx.y;
+  const int m = order ? order[blockIdx.y] : blockIdx.y;
   if (!mol_live(active, status, m)) return;
   const MolDesc M = mols[m];
   const int n = M.n_ao;
@@ -466,9 +570,10 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_
                                                const double* __restrict__ tnormA,
                                                double* __restrict__ work, int want_all,
                                                const int* __restrict__ active,
-                                               const int* __restrict__ status) {
+                                               const int* __restrict__ status,
+                                               const int* __restrict__ order) {
   extern __shared__ double lds[];
-  const int m = blockIdx.x;
+  const int m = order ? order[blockIdx.x] : blockIdx.x;
   if (!mol_live(active, status, m)) return;
   const MolDesc M = mols[m];
   const int n = M.n_ao, tid = threadIdx.x, nt = blockDim.x;
@@ -573,9 +678,10 @@ __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict_
                                                        const double* __restrict__ tauA,
                                                        int want_all,
                                                        const int* __restrict__ active,
-                                                       const int* __restrict__ status) {
+                                                       const int* __restrict__ status,
+                                                       const int* __restrict__ order) {
   extern __shared__ double lds[];
-  const int m = blockIdx.x;
+  const int m = order ? order[blockIdx.x] : blockIdx.x;
   if (!mol_live(active, status, m)) return;
   const MolDesc M = mols[m];
   const int n = M.n_ao, tid = threadIdx.x, nt = blockDim.x;
@@ -681,29 +787,6 @@ __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict_
 // or 2 (I - Y Y^T) (virtual side), on the matrix cores.  One wavefront per
 // 16 x 16 tile of P, k in steps of 4 with v_mfma_f64_16x16x4f64 (FP64 steps)
 // or v_mfma_f32_16x16x4f32 (FP32 steps of the mixed-precision policy).
-// Operand layout of the 16x16x4 instructions: lane l supplies A[l%16][l/16]
-// and B[l/16][l%16].  Result register r of lane l holds D[i][l%16] with
-// i = 4*(l/16) + r for the f32 instruction and i = 4*r + l/16 for the f64 one
-// (measured on gfx950).
-template <typename T>
-struct Mfma16;
-template <>
-struct Mfma16<double> {
-  typedef double v4 __attribute__((ext_vector_type(4)));
-  static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lane, int r) { return 4 * r + (lane >> 4); }
-};
-template <>
-struct Mfma16<float> {
-  typedef float v4 __attribute__((ext_vector_type(4)));
-  static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
-};
-
 template <typename T>
 __device__ __forceinline__ void density_tile(const double* __restrict__ Y, int n, int k, int lo,
                                              int i0, int j0, int lane, double* __restrict__ p) {
@@ -744,8 +827,9 @@ __global__ void __launch_bounds__(256) k_density(const MolDesc* __restrict__ mol
                                                   double* __restrict__ P,
                                                   const int* __restrict__ active,
                                                   const int* __restrict__ status,
-                                                  const int* __restrict__ prec) {
-  const int m = blockIdx.y;
+                                                  const int* __restrict__ prec,
+                                                  const int* __restrict__ order) {
+  const int m = order ? order[blockIdx.y] : blockIdx.y;
   if (!mol_live(active, status, m)) return;
   const MolDesc M = mols[m];
   const int n = M.n_ao;
@@ -797,16 +881,19 @@ __global__ void k_write_eigvec(const MolDesc* __restrict__ mols, const double* _
 }
 
 static size_t tridiag_lds(int max_n) {
-  return sizeof(double) * ((size_t)kPanelDoubles + max_n + 32 + (size_t)(kT / 64) * max_n);
+  return sizeof(double) * ((size_t)kPanelDoubles + 2 * (size_t)max_n + 32 + 3 * kNbMax +
+                           (size_t)(kT / 64) * max_n);
 }
 
 void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all) {
   const int* act = all ? nullptr : b.active;
   const int N = b.max_n;
   if (N <= 384)
-    k_tridiag<6><<<b.n_mol, kT, tridiag_lds(N), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status);
+    k_tridiag<6><<<b.n_mol, kT, tridiag_lds(N), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act,
+                                                     b.status, b.order);
   else
-    k_tridiag<20><<<b.n_mol, kT, tridiag_lds(N), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status);
+    k_tridiag<20><<<b.n_mol, kT, tridiag_lds(N), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act,
+                                                      b.status, b.order);
 }
 
 void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all) {
@@ -814,20 +901,21 @@ void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all) {
   dim3 gb((N + 255) / 256, b.n_mol);
   k_bisect<<<gb, 256, sizeof(double) * (2 * N + 32), s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm,
                                                           all ? 1 : 0, all ? nullptr : b.active,
-                                                          b.status);
+                                                          b.status, b.order);
 }
 
 void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all) {
   const int N = b.max_n;
   const size_t inv_lds = sizeof(double) * (3 * (size_t)N) + sizeof(int) * (2 * (size_t)N + 2);
   k_invit<<<b.n_mol, kT, inv_lds, s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm, b.W, all ? 1 : 0,
-                                       all ? nullptr : b.active, b.status);
+                                       all ? nullptr : b.active, b.status, b.order);
 }
 
 void launch_eigh_back(const BatchDev& b, hipStream_t s, bool all) {
   const int* act = all ? nullptr : b.active;
   const size_t bt_lds = sizeof(double) * ((size_t)kPanelDoubles / 2 + 2 * kNbMax * kNbMax);
-  k_backtransform<<<b.n_mol, kT, bt_lds, s>>>(b.mols, b.W, b.tau, all ? 1 : 0, act, b.status);
+  k_backtransform<<<b.n_mol, kT, bt_lds, s>>>(b.mols, b.W, b.tau, all ? 1 : 0, act, b.status,
+                                              b.order);
   if (all) {
     dim3 gw(64, b.n_mol);
     k_write_eigvec<<<gw, 256, 0, s>>>(b.mols, b.W, b.C, act, b.status);
@@ -846,7 +934,7 @@ void launch_density(const BatchDev& b, hipStream_t s, bool all) {
   if (all)
     k_density_from_c<<<grid, 256, 0, s>>>(b.mols, b.C, b.P, 64);
   else
-    k_density<<<grid, 256, 0, s>>>(b.mols, b.W, b.P, b.active, b.status, b.prec);
+    k_density<<<grid, 256, 0, s>>>(b.mols, b.W, b.P, b.active, b.status, b.prec, b.order);
 }
 
 }  // namespace sg

@@ -115,6 +115,9 @@ struct sg_server {
   int* mask_new = nullptr;  // device, per slot (+ dead slot)
   int2* pair_scratch = nullptr;
   std::vector<int> mask;    // host copy of mask_new
+  int* order_dev = nullptr;  // slots by decreasing N (device)
+  std::vector<int> order;    // host copy
+  bool order_dirty = true;
   std::set<int> free_slots;
   std::vector<MolDesc> mols;
   std::vector<AtomDesc> atoms;
@@ -178,7 +181,7 @@ int sg_server_create(sg_context* ctx, double hbm_bytes, int max_slots, int max_a
   const size_t o_tn = take(8 * ns), o_ecc = take(8 * ns), o_eel = take(8 * ns), o_enew = take(8 * ns);
   const size_t o_steps = take(4 * ns), o_act = take(4 * ns), o_conv = take(4 * ns),
                o_stat = take(4 * ns), o_nact = take(256), o_prec = take(4 * ns),
-               o_pprec = take(4 * ns), o_mask = take(4 * ns);
+               o_pprec = take(4 * ns), o_mask = take(4 * ns), o_order = take(4 * ns);
   (void)hipSetDevice(ctx->device);
   hipError_t e = hipMalloc(&S->dev, off);
   if (e != hipSuccess) {
@@ -214,6 +217,7 @@ int sg_server_create(sg_context* ctx, double hbm_bytes, int max_slots, int max_a
   d.prec = (int*)(base + o_prec);
   d.prev_prec = (int*)(base + o_pprec);
   S->mask_new = (int*)(base + o_mask);
+  S->order_dev = (int*)(base + o_order);
   hipStream_t s = ctx->stream;
   for (int* p : {d.steps, d.active, d.converged, d.status, d.prec, d.prev_prec, S->mask_new})
     if (e == hipSuccess) e = hipMemsetAsync(p, 0, 4 * ns, s);
@@ -375,6 +379,20 @@ int sg_server_step(sg_server* S, int* n_inflight, int* n_queued) {
   it.n_mol = hw;
   it.n_atoms = hw_atoms;
   it.max_n = max_n;
+  if (!fresh.empty()) S->order_dirty = true;
+  if (S->order_dirty && hw > 0) {
+    // slots in flight by decreasing N, then the free slots below the high-water mark
+    S->order.resize(hw);
+    for (int i = 0; i < hw; ++i) S->order[i] = i;
+    std::stable_sort(S->order.begin(), S->order.end(), [&](int x, int y) {
+      const int nx = S->slots[x].used ? S->mols[x].n_ao : -1;
+      const int ny = S->slots[y].used ? S->mols[y].n_ao : -1;
+      return nx > ny;
+    });
+    HIP_TRY(hipMemcpyAsync(S->order_dev, S->order.data(), sizeof(int) * hw, hipMemcpyHostToDevice, s));
+    S->order_dirty = false;
+  }
+  it.order = S->order_dev;
   if (!fresh.empty()) {
     for (int sl : fresh) S->mask[sl] = 1;
     HIP_TRY(hipMemcpyAsync(S->d.mols, S->mols.data(), sizeof(MolDesc) * hw, hipMemcpyHostToDevice, s));
@@ -479,6 +497,7 @@ int sg_server_step(sg_server* S, int* n_inflight, int* n_queued) {
       S->free_slots.insert(i);
       sl = Slot{};
       retired = true;
+      S->order_dirty = true;
     }
     // the host descriptors must outlive the copies above
     if (retired) HIP_TRY(hipStreamSynchronize(s));
