@@ -505,7 +505,78 @@ __device__ double sto_overlap(const AtomDesc& A, int la, const AtomDesc& B, int 
   return tot;
 }
 
-// Off-diagonal Hcore blocks for every atom pair (all types): lane per pair.
+// Off-diagonal Hcore blocks beta * S / 2 for one atom pair per lane, launched
+// per pair type so the lanes of a wave do the same work.  The AO pairs of
+// one shell pair (s-s, s-p, p-s, p-p) share their primitive exponents, so
+// exp(-mu R^2), the Gaussian centre and the Gauss-Hermite sums of each axis
+// are formed once per primitive pair and shell pair; each factor is computed
+// with exactly the operations sto_overlap uses, so the values are the same.
+template <int NA, int NB>
+__device__ __forceinline__ void hcore_shell(const AtomDesc& A, int sa, const AtomDesc& B, int sb,
+                                            double ab, double* tot) {
+  const double ac[3] = {A.x, A.y, A.z}, bc[3] = {B.x, B.y, B.z};
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      const double al = sa == 0 ? c_sto.expS[A.zn][i] : c_sto.expP[A.zn][i];
+      const double be = sb == 0 ? c_sto.expS[B.zn][j] : c_sto.expP[B.zn][j];
+      const double p = al + be;
+      const double mu = al * be / p;
+      double o = exp(-mu * ab);
+      o *= sa == 0 ? c_sto.cS[A.zn][i] : c_sto.cP[A.zn][i];
+      o *= sb == 0 ? c_sto.cS[B.zn][j] : c_sto.cP[B.zn][j];
+      const double sp = sqrt(p);
+      const double i00 = sqrt(M_PI) / sp;
+      double f10[3], f01[3], f11[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double P = al * ac[k];
+        P += be * bc[k];
+        P /= p;
+        const double pa = P - ac[k], pb = P - bc[k];
+        double s10 = 0.0, s01 = 0.0, s11 = 0.0;
+#pragma unroll
+        for (int g = 0; g < 12; ++g) {
+          const double x = c_ghx[g] / sp;
+          const double w = c_ghw[g];
+          const double t10 = w * (x + pa);
+          s10 += t10;
+          s01 += w * (x + pb);
+          s11 += t10 * (x + pb);
+        }
+        f10[k] = s10 / sp;
+        f01[k] = s01 / sp;
+        f11[k] = s11 / sp;
+      }
+      // AO u of shell sa (0 = s; 1..3 = px, py, pz when sa = 1), likewise v
+#pragma unroll
+      for (int u = 0; u < NA; ++u)
+#pragma unroll
+        for (int v = 0; v < NB; ++v) {
+          const int la = sa ? u + 1 : 0, lb = sb ? v + 1 : 0;
+          double t = o;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const bool pa = la == k + 1, pb = lb == k + 1;
+            t *= pa ? (pb ? f11[k] : f10[k]) : (pb ? f01[k] : i00);
+          }
+          tot[u * NB + v] += t;
+        }
+    }
+}
+
+__device__ __forceinline__ void hcore_store(const AtomDesc& A, int u, const AtomDesc& B, int v,
+                                            double s, int N, double* h) {
+  const double bu = u == 0 ? c_par.betaS[A.zn] : c_par.betaP[A.zn];
+  const double bv = v == 0 ? c_par.betaS[B.zn] : c_par.betaP[B.zn];
+  const double hv = bu + bv;
+  const double val = hv * s * 0.5;
+  const int r = A.ao0 + u, c = B.ao0 + v;
+  h[(int64_t)r * N + c] = val;
+  h[(int64_t)c * N + r] = val;
+}
+
+// XX: both heavy; HX: one of the two is hydrogen (either order); HH.
+template <int KIND>
 __global__ void __launch_bounds__(128) k_hcore_offdiag(const int2* __restrict__ pairs, int n,
                                                         const AtomDesc* __restrict__ atoms,
                                                         const MolDesc* __restrict__ mols,
@@ -513,21 +584,57 @@ __global__ void __launch_bounds__(128) k_hcore_offdiag(const int2* __restrict__ 
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int2 pr = pairs[i];
-  const AtomDesc A = atoms[pr.x], B = atoms[pr.y];
+  AtomDesc A = atoms[pr.x], B = atoms[pr.y];
   const MolDesc M = mols[A.mol];
-  const int N = M.n_ao, na = n_ao_of(A.zn), nb = n_ao_of(B.zn);
+  const int N = M.n_ao;
   double* h = H + M.mat0;
-  for (int u = 0; u < na; ++u)
-    for (int v = 0; v < nb; ++v) {
-      const double s = sto_overlap(A, u, B, v);
-      const double bu = u == 0 ? c_par.betaS[A.zn] : c_par.betaP[A.zn];
-      const double bv = v == 0 ? c_par.betaS[B.zn] : c_par.betaP[B.zn];
-      const double hv = bu + bv;
-      const double val = hv * s * 0.5;
-      const int r = A.ao0 + u, c = B.ao0 + v;
-      h[(int64_t)r * N + c] = val;
-      h[(int64_t)c * N + r] = val;
+  double ab = 0.0;
+  {
+    const double d0 = A.x - B.x, d1 = A.y - B.y, d2 = A.z - B.z;
+    ab += d0 * d0;
+    ab += d1 * d1;
+    ab += d2 * d2;
+  }
+  if (KIND == 2) {  // H-H
+    double t[1] = {0.0};
+    hcore_shell<1, 1>(A, 0, B, 0, ab, t);
+    hcore_store(A, 0, B, 0, t[0], N, h);
+  } else if (KIND == 1) {  // H-X: A is the heavy atom after the swap
+    const bool swap = A.zn == 1;
+    if (swap) {
+      const AtomDesc T = A;
+      A = B;
+      B = T;
     }
+    // overlaps are taken as S(AO of the pair's first atom, AO of its second)
+    double ss[1] = {0.0}, ps[3] = {0.0, 0.0, 0.0};
+    if (!swap) {
+      hcore_shell<1, 1>(A, 0, B, 0, ab, ss);
+      hcore_shell<3, 1>(A, 1, B, 0, ab, ps);
+    } else {
+      // first atom is the hydrogen: S(H s, X ao) as sto_overlap(H, 0, X, v)
+      double sp[3] = {0.0, 0.0, 0.0};
+      hcore_shell<1, 1>(B, 0, A, 0, ab, ss);
+      hcore_shell<1, 3>(B, 0, A, 1, ab, sp);
+      ps[0] = sp[0];
+      ps[1] = sp[1];
+      ps[2] = sp[2];
+    }
+    hcore_store(A, 0, B, 0, ss[0], N, h);
+    for (int u = 0; u < 3; ++u) hcore_store(A, u + 1, B, 0, ps[u], N, h);
+  } else {  // X-X
+    double ss[1] = {0.0}, sp[3] = {0.0, 0.0, 0.0}, ps[3] = {0.0, 0.0, 0.0};
+    double pp[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    hcore_shell<1, 1>(A, 0, B, 0, ab, ss);
+    hcore_shell<1, 3>(A, 0, B, 1, ab, sp);
+    hcore_shell<3, 1>(A, 1, B, 0, ab, ps);
+    hcore_shell<3, 3>(A, 1, B, 1, ab, pp);
+    hcore_store(A, 0, B, 0, ss[0], N, h);
+    for (int v = 0; v < 3; ++v) hcore_store(A, 0, B, v + 1, sp[v], N, h);
+    for (int u = 0; u < 3; ++u) hcore_store(A, u + 1, B, 0, ps[u], N, h);
+    for (int u = 0; u < 3; ++u)
+      for (int v = 0; v < 3; ++v) hcore_store(A, u + 1, B, v + 1, pp[u * 3 + v], N, h);
+  }
 }
 
 // (mu nu | s_B s_B) with mu, nu on A (GetValueFromArray, :285-386).
@@ -625,9 +732,15 @@ void launch_pair_integrals(const BatchDev& b, hipStream_t s, const int* mask) {
 }
 
 void launch_hcore(const BatchDev& b, hipStream_t s) {
-  if (b.n_pairs_all)
-    k_hcore_offdiag<<<blocks_for(b.n_pairs_all, 128), 128, 0, s>>>(b.pairs_all, b.n_pairs_all, b.atoms,
-                                                                   b.mols, b.H);
+  if (b.n_xx)
+    k_hcore_offdiag<0><<<blocks_for(b.n_xx, 128), 128, 0, s>>>(b.pairs_xx, b.n_xx, b.atoms, b.mols,
+                                                               b.H);
+  if (b.n_hx)
+    k_hcore_offdiag<1><<<blocks_for(b.n_hx, 128), 128, 0, s>>>(b.pairs_hx, b.n_hx, b.atoms, b.mols,
+                                                               b.H);
+  if (b.n_hh)
+    k_hcore_offdiag<2><<<blocks_for(b.n_hh, 128), 128, 0, s>>>(b.pairs_hh, b.n_hh, b.atoms, b.mols,
+                                                               b.H);
   k_hcore_diag<<<blocks_for(b.n_diag_tasks, 128), 128, 0, s>>>(b.diag_tasks, b.n_diag_tasks, b.atoms,
                                                                b.mols, b.blocks, b.H);
 }

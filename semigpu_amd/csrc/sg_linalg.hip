@@ -398,6 +398,26 @@ __device__ __forceinline__ int sturm(const double* d, const double* e2, int n, d
   return cnt;
 }
 
+// Two Sturm counts at once (independent recurrences, interleaved).
+__device__ __forceinline__ void sturm2(const double* d, const double* e2, int n, double x1,
+                                       double x2, double pivmin, int& c1, int& c2) {
+  double q1 = d[0] - x1, q2 = d[0] - x2;
+  if (fabs(q1) < pivmin) q1 = -pivmin;
+  if (fabs(q2) < pivmin) q2 = -pivmin;
+  int n1 = q1 < 0.0, n2 = q2 < 0.0;
+  for (int k = 1; k < n; ++k) {
+    const double dk = d[k], ek = e2[k - 1];
+    q1 = (dk - x1) - ek / q1;
+    q2 = (dk - x2) - ek / q2;
+    if (fabs(q1) < pivmin) q1 = -pivmin;
+    if (fabs(q2) < pivmin) q2 = -pivmin;
+    n1 += q1 < 0.0;
+    n2 += q2 < 0.0;
+  }
+  c1 = n1;
+  c2 = n2;
+}
+
 __global__ void __launch_bounds__(256) k_bisect(const MolDesc* __restrict__ mols,
                                                  const double* __restrict__ dA,
                                                  const double* __restrict__ eA,
@@ -405,7 +425,8 @@ __global__ void __launch_bounds__(256) k_bisect(const MolDesc* __restrict__ mols
                                                  double* __restrict__ tnormA, int want_all,
                                                  const int* __restrict__ active,
                                                  const int* __restrict__ status,
-                                                 const int* __restrict__ order) {
+                                                 const int* __restrict__ order,
+                                                 const int* __restrict__ steps) {
   extern __shared__ double lds[];
   const int m = order ? order[blockIdx.y] : blockIdx.y;
   if (!mol_live(active, status, m)) return;
@@ -459,7 +480,27 @@ __global__ void __launch_bounds__(256) k_bisect(const MolDesc* __restrict__ mols
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= k) return;
   const int idx = lo + j;
+  // Bisection keeping count(a) <= idx < count(b).  From the second SCF step
+  // on, the previous step's eigenvalue (the same index) brackets the new one
+  // closely: [prev - delta, prev + delta] is used when two Sturm counts
+  // confirm it, with delta = 1e-7 ||T||, else 1e-4 ||T||, else Gershgorin.
   double a = glo, b = ghi;
+  if (steps && steps[m] > 0) {
+    const double prev = lamA[M.vec0 + j];
+#pragma unroll 1
+    for (int lvl = 0; lvl < 2; ++lvl) {
+      const double del = (lvl == 0 ? 1e-7 : 1e-4) * tnorm;
+      const double x1 = prev - del, x2 = prev + del;
+      if (!(x1 > glo && x2 < ghi)) continue;
+      int c1, c2;
+      sturm2(d, e2, n, x1, x2, pivmin, c1, c2);
+      if (c1 <= idx && c2 > idx) {
+        a = x1;
+        b = x2;
+        break;
+      }
+    }
+  }
   for (int it = 0; it < 128; ++it) {
     if (b - a <= 2.0 * kEps * fmax(fabs(a), fabs(b)) + pivmin) break;
     const double mid = 0.5 * (a + b);
@@ -673,109 +714,146 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_
 }
 
 // ---------------------------------------------------------------------------
+// Y <- Q Y, Q = H_0 H_1 ... H_(nref-1), applied one group of G <= 16
+// reflectors at a time from the last group to the first, each group in
+// compact WY form I - V T V^T (LAPACK dlarft forward / dlarfb).  Every product
+// runs on the matrix cores (v_mfma_f64_16x16x4f64) in 16 x 16 tiles:
+//   Gram  V V^T (K split over the waves, partial tiles summed in order),
+//   W1 = V Y, W2 = T W1 (the f64 result registers of lane l are exactly the
+//   B operand of the next product for k-step r, so W1 never leaves the
+//   wave), Y -= V^T W2 (next tile loaded before the current one is stored).
+// V is row q = reflector k0+q (stored in the upper part of row k0+q of the
+// work matrix, zero at columns <= k0+q).  LDS: V (G x n), W2 (G x k) sharing
+// room with the partial Gram tiles, T and the Gram matrix (16 x 16 each).
+__device__ __forceinline__ int bt_group(int n, int k, int lds_doubles) {
+  int g = 16;
+  while (g > 4 && g * n + max(g * k, 2048) + 512 > lds_doubles) g -= 4;
+  return g;
+}
+
 __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict__ mols,
                                                        double* __restrict__ work,
                                                        const double* __restrict__ tauA,
                                                        int want_all,
                                                        const int* __restrict__ active,
                                                        const int* __restrict__ status,
-                                                       const int* __restrict__ order) {
+                                                       const int* __restrict__ order,
+                                                       int lds_doubles) {
   extern __shared__ double lds[];
   const int m = order ? order[blockIdx.x] : blockIdx.x;
   if (!mol_live(active, status, m)) return;
   const MolDesc M = mols[m];
   const int n = M.n_ao, tid = threadIdx.x, nt = blockDim.x;
   const int wave = tid >> 6, lane = tid & 63, nwave = nt >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
   int lo, k;
   selection(M, want_all, lo, k);
   const int nref = n > 2 ? n - 2 : 0;
   if (k == 0 || nref == 0) return;
-  const int nb = panel_width(n);
+  const int G = bt_group(n, k, lds_doubles);
+  const int rg = max(G * k, 2048);
+  double* Vs = lds;          // G x n
+  double* R = Vs + G * n;    // W2 (G x k), before that nwave partial 16x16 Gram tiles
+  double* Tm = R + rg;       // 16 x 16, T[r][q] at r*16+q
+  double* Gm = Tm + 256;     // 16 x 16 Gram
   const double* A = work + 5 * M.mat0;
   double* Y = work + 5 * M.mat0 + (int64_t)n * n;
   const double* tau = tauA + M.vec0;
-  double* Vp = lds;                     // nb x n panel (column q = reflector k0+q)
-  double* T = lds + kPanelDoubles / 2;  // nb x nb upper triangular (stride nb)
-  double* G = T + kNbMax * kNbMax;      // nb x nb Gram matrix V^T V
-  const int npanel = (nref + nb - 1) / nb;
-  for (int pnl = npanel - 1; pnl >= 0; --pnl) {
-    const int k0 = pnl * nb, b = min(nb, nref - k0);
-    for (int t = tid; t < nb * n; t += nt) {
-      const int q = t / n, i = t % n;
-      double v = 0.0;
-      if (q < b) {
-        const int c = k0 + q;
-        v = i < c + 1 ? 0.0 : A[(int64_t)c * n + i];
+  const int KT = (k + 15) >> 4;
+  typedef Mfma16<double>::v4 v4;
+  for (int k0 = ((nref - 1) / G) * G; k0 >= 0; k0 -= G) {
+    const int b = min(G, nref - k0);
+    for (int t = tid; t < G * n; t += nt) {
+      const int q = t / n, i = t - q * n, c = k0 + q;
+      Vs[t] = (q < b && i > c) ? A[(int64_t)c * n + i] : 0.0;
+    }
+    for (int t = tid; t < 256; t += nt) Tm[t] = 0.0;
+    __syncthreads();
+    // Gram V V^T: wave w sums rows i = k0+1+4w+..., step 4 nwave
+    {
+      v4 acc = {0, 0, 0, 0};
+      for (int i0 = k0 + 1 + 4 * wave; i0 < n; i0 += 4 * nwave) {
+        const int i = i0 + lk;
+        const double a = (lr < G && i < n) ? Vs[lr * n + i] : 0.0;
+        acc = Mfma16<double>::op(a, a, acc);
       }
-      Vp[t] = v;
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) R[wave * 256 + (4 * rr + lk) * 16 + lr] = acc[rr];
     }
     __syncthreads();
-    for (int pq = wave; pq < b * b; pq += nwave) {
-      const int p = pq / b, q = pq % b;
-      if (p < q) {
-        double s = 0.0;
-        for (int i = k0 + 1 + lane; i < n; i += 64) s += Vp[p * n + i] * Vp[q * n + i];
-        s = wsum(s);
-        if (lane == 0) G[p * nb + q] = s;
-      }
+    for (int t = tid; t < 256; t += nt) {
+      double g = 0.0;
+      for (int w = 0; w < nwave; ++w) g += R[w * 256 + t];
+      Gm[t] = g;
     }
     __syncthreads();
-    for (int t = tid; t < kNbMax * kNbMax; t += nt) T[t] = 0.0;
-    __syncthreads();
-    if (tid == 0) {
-      // forward compact-WY factor: H_k0 ... H_(k0+b-1) = I - V T V^T
+    // T: column i from columns < i (one wave, lanes r < i)
+    if (wave == 0) {
       for (int i = 0; i < b; ++i) {
         const double ti = tau[k0 + i];
-        T[i * nb + i] = ti;
-        for (int r = 0; r < i; ++r) {
-          double s = 0.0;
-          for (int q = r; q < i; ++q) s += T[r * nb + q] * G[q * nb + i];
-          T[r * nb + i] = -ti * s;
+        if (lane < i) {
+          double sacc = 0.0;
+          for (int q = lane; q < i; ++q) sacc += Tm[lane * 16 + q] * Gm[q * 16 + i];
+          Tm[lane * 16 + i] = -ti * sacc;
         }
+        if (lane == 0) Tm[i * 16 + i] = ti;
+        __threadfence_block();
+        __builtin_amdgcn_wave_barrier();
       }
     }
     __syncthreads();
-    for (int col = tid; col < k; col += nt) {
-      double w1[kNbMax];
-#pragma unroll
-      for (int q = 0; q < kNbMax; ++q) w1[q] = 0.0;
+    // W2 = T (V Y): one 16-column tile of Y per wave
+    for (int jt = wave; jt < KT; jt += nwave) {
+      const int j = 16 * jt + lr;
+      v4 w1 = {0, 0, 0, 0};
       for (int i0 = k0 + 1; i0 < n; i0 += 4) {
-        double yv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) yv[u] = i0 + u < n ? Y[(int64_t)(i0 + u) * k + col] : 0.0;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int i = i0 + u < n ? i0 + u : k0 + 1;
-          const double y = yv[u];
-#pragma unroll
-          for (int q = 0; q < kNbMax; ++q)
-            if (q < b) w1[q] += Vp[q * n + i] * y;
-        }
+        const int i = i0 + lk;
+        const double a = (lr < G && i < n) ? Vs[lr * n + i] : 0.0;
+        const double y = (i < n && j < k) ? Y[(int64_t)i * k + j] : 0.0;
+        w1 = Mfma16<double>::op(a, y, w1);
       }
-      double w2[kNbMax];
+      v4 w2 = {0, 0, 0, 0};
 #pragma unroll
-      for (int r = 0; r < kNbMax; ++r) {
-        double s = 0.0;
+      for (int st = 0; st < 4; ++st) w2 = Mfma16<double>::op(Tm[lr * 16 + 4 * st + lk], w1[st], w2);
 #pragma unroll
-        for (int q = 0; q < kNbMax; ++q)
-          if (r < b && q >= r && q < b) s += T[r * nb + q] * w1[q];
-        w2[r] = s;
+      for (int rr = 0; rr < 4; ++rr) {
+        const int g = 4 * rr + lk;
+        if (g < G && j < k) R[g * k + j] = w2[rr];
       }
-      for (int i0 = k0 + 1; i0 < n; i0 += 4) {
-        double yv[4];
+    }
+    __syncthreads();
+    // Y -= V^T W2 over rows k0+1..n-1
+    {
+      const int IT = (n - (k0 + 1) + 15) >> 4;
+      const int ntile = IT * KT;
+      auto load = [&](int t, v4& v) {
+        const int I0 = k0 + 1 + 16 * (t / KT), col = 16 * (t % KT) + lr;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) yv[u] = i0 + u < n ? Y[(int64_t)(i0 + u) * k + col] : 0.0;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int i = i0 + u;
-          if (i < n) {
-            double s = 0.0;
-#pragma unroll
-            for (int q = 0; q < kNbMax; ++q)
-              if (q < b) s += Vp[q * n + i] * w2[q];
-            Y[(int64_t)i * k + col] = yv[u] - s;
-          }
+        for (int rr = 0; rr < 4; ++rr) {
+          const int row = I0 + 4 * rr + lk;
+          v[rr] = (row < n && col < k) ? Y[(int64_t)row * k + col] : 0.0;
         }
+      };
+      v4 cur = {0, 0, 0, 0}, nxt = {0, 0, 0, 0};
+      if (wave < ntile) load(wave, cur);
+      for (int t = wave; t < ntile; t += nwave) {
+        if (t + nwave < ntile) load(t + nwave, nxt);
+        const int I0 = k0 + 1 + 16 * (t / KT), J0 = 16 * (t % KT);
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+          const int g = 4 * st + lk;
+          const int i = I0 + lr, j = J0 + lr;
+          const double a = (g < G && i < n) ? -Vs[g * n + i] : 0.0;
+          const double w = (g < G && j < k) ? R[g * k + j] : 0.0;
+          cur = Mfma16<double>::op(a, w, cur);
+        }
+        const int col = J0 + lr;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int row = I0 + 4 * rr + lk;
+          if (row < n && col < k) Y[(int64_t)row * k + col] = cur[rr];
+        }
+        cur = nxt;
       }
     }
     __syncthreads();
@@ -901,7 +979,8 @@ void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all) {
   dim3 gb((N + 255) / 256, b.n_mol);
   k_bisect<<<gb, 256, sizeof(double) * (2 * N + 32), s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm,
                                                           all ? 1 : 0, all ? nullptr : b.active,
-                                                          b.status, b.order);
+                                                          b.status, b.order,
+                                                          all ? nullptr : b.steps);
 }
 
 void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all) {
@@ -913,9 +992,12 @@ void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all) {
 
 void launch_eigh_back(const BatchDev& b, hipStream_t s, bool all) {
   const int* act = all ? nullptr : b.active;
-  const size_t bt_lds = sizeof(double) * ((size_t)kPanelDoubles / 2 + 2 * kNbMax * kNbMax);
-  k_backtransform<<<b.n_mol, kT, bt_lds, s>>>(b.mols, b.W, b.tau, all ? 1 : 0, act, b.status,
-                                              b.order);
+  // 80 KB (two workgroups per CU) unless even groups of 4 reflectors need more
+  const int N = b.max_n;
+  const int kmax = all ? N : N / 2 + 1;
+  const int bt_doubles = 4 * (N + kmax) + 2560 > 10240 ? 20480 : 10240;
+  k_backtransform<<<b.n_mol, kT, sizeof(double) * bt_doubles, s>>>(
+      b.mols, b.W, b.tau, all ? 1 : 0, act, b.status, b.order, bt_doubles);
   if (all) {
     dim3 gw(64, b.n_mol);
     k_write_eigvec<<<gw, 256, 0, s>>>(b.mols, b.W, b.C, act, b.status);
