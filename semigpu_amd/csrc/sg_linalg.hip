@@ -44,10 +44,50 @@ constexpr int kNbMax = 8;               // widest reflector panel
 constexpr int kPanelDoubles = kPanelLds / 8;
 constexpr double kEps = 2.220446049250313e-16;
 
+// Cross-lane moves without the LDS crossbar (gfx950): DPP within rows of 16
+// lanes, v_permlane16/32_swap across them.  Doubles move as two 32-bit halves.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+constexpr int kDppXor1 = 0xB1;       // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;       // quad_perm [2,3,0,1]
+constexpr int kDppMirror8 = 0x141;   // lane i <-> 7-i within 8
+constexpr int kDppMirror16 = 0x140;  // lane i <-> 15-i within 16
+
+// permlane32_swap: x gets y's lower half in its upper 32 lanes, y gets x's
+// upper half in its lower lanes (so x + y pairs lane l with l ^ 32);
+// permlane16_swap does the same for the 16-lane halves of each 32.
+__device__ __forceinline__ void swap32(double& x, double& y) {
+  const long long ux = __double_as_longlong(x), uy = __double_as_longlong(y);
+  const auto lo = __builtin_amdgcn_permlane32_swap((int)ux, (int)uy, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((int)(ux >> 32), (int)(uy >> 32), false, false);
+  x = __longlong_as_double(((long long)hi[0] << 32) | (unsigned int)lo[0]);
+  y = __longlong_as_double(((long long)hi[1] << 32) | (unsigned int)lo[1]);
+}
+__device__ __forceinline__ void swap16(double& x, double& y) {
+  const long long ux = __double_as_longlong(x), uy = __double_as_longlong(y);
+  const auto lo = __builtin_amdgcn_permlane16_swap((int)ux, (int)uy, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((int)(ux >> 32), (int)(uy >> 32), false, false);
+  x = __longlong_as_double(((long long)hi[0] << 32) | (unsigned int)lo[0]);
+  y = __longlong_as_double(((long long)hi[1] << 32) | (unsigned int)lo[1]);
+}
+
+// Sum over the 64 lanes (every lane gets the same value).
 __device__ __forceinline__ double wsum(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_f64<kDppXor1>(v);
+  v += dpp_f64<kDppXor2>(v);
+  v += dpp_f64<kDppMirror8>(v);
+  v += dpp_f64<kDppMirror16>(v);
+  double w = v;
+  swap16(v, w);
+  v += w;
+  w = v;
+  swap32(v, w);
+  return v + w;
 }
 
 // Block-wide sum in a fixed order; every thread gets the result.
@@ -85,6 +125,17 @@ __device__ __forceinline__ int panel_width(int n) {
   int nb = kPanelLds / (16 * (n > 0 ? n : 1));
   return nb < 1 ? 1 : (nb > kNbMax ? kNbMax : nb);
 }
+
+// Raw buffer access to one molecule's matrix: offsets are 32-bit byte offsets
+// from the matrix start, and loads past its last element return 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const double* base, int64_t count) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)(count * 8), 0x00020000);
+}
+__device__ __forceinline__ double load_f64(__amdgpu_buffer_rsrc_t r, int byte_off) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0);
+  return __longlong_as_double(((long long)v[1] << 32) | (unsigned int)v[0]);
+}
+constexpr int kSymvGroup = 2;  // 64-column tiles whose loads are issued together
 
 // Operand layout of the 16x16x4 matrix-core instructions: lane l supplies
 // A[l%16][l/16] and B[l/16][l%16].  Result register r of lane l holds
@@ -143,6 +194,7 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_
   double* e = eA + M.vec0;
   double* tau = tauA + M.vec0;
   const double* f = F + M.mat0;
+  const __amdgpu_buffer_rsrc_t arsrc = make_rsrc(A, (int64_t)n * n);
   for (int64_t t = tid; t < (int64_t)n * n; t += nt) A[t] = f[t];
   __syncthreads();
   const int nref = n > 2 ? n - 2 : 0;
@@ -210,71 +262,63 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_
             acc[r] = 0.0;
             vi[r] = i0 + r < n ? vj[i0 + r] : 0.0;
           }
-          const int tmax = (min(i0 + 7, n - 1) - m1) >> 6;
-          const int tfull = i0 + 7 < n ? (i0 - m1) >> 6 : 0;
-          const double* arow = A + (int64_t)i0 * n;
+          const int tcount = ((min(i0 + 7, n - 1) - m1) >> 6) + 1;
+          // byte offset of (row i0+r, column m1+lane); rows >= n fall outside
+          // the buffer and read 0, entries right of the diagonal are masked
+          int voff[8];
 #pragma unroll
-          for (int t = 0; t < TMAX; ++t) {
-            const int l = m1 + lane + 64 * t;
-            double a[8];
-            if (t < tfull) {
-              const double vl = vj[l];
+          for (int r = 0; r < 8; ++r) voff[r] = ((i0 + r) * n + m1 + lane) * 8;
 #pragma unroll
-              for (int r = 0; r < 8; ++r) a[r] = arow[(int64_t)r * n + l];
-              double qs = 0.0;
+          for (int tg = 0; tg < TMAX; tg += kSymvGroup) {
+            if (tg < tcount) {
+              double a[kSymvGroup][8];
 #pragma unroll
-              for (int r = 0; r < 8; ++r) {
-                acc[r] += a[r] * vl;
-                qs += a[r] * vi[r];
+              for (int g = 0; g < kSymvGroup; ++g)
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+                  a[g][r] = tg + g < TMAX ? load_f64(arsrc, voff[r] + 512 * (tg + g)) : 0.0;
+#pragma unroll
+              for (int g = 0; g < kSymvGroup; ++g) {
+                const int t = tg + g;
+                if (t < TMAX && t < tcount) {
+                  const int l = m1 + lane + 64 * t;
+                  const double vl = l < n ? vj[l] : 0.0;
+                  double qs = 0.0;
+#pragma unroll
+                  for (int r = 0; r < 8; ++r) {
+                    const int i = i0 + r;
+                    const double av = l <= i ? a[g][r] : 0.0;
+                    acc[r] += av * vl;
+                    if (l < i) qs += av * vi[r];
+                    if (t == 0 && grab && lane == 0 && i < n) colbuf[i] = av;
+                  }
+                  q[t] += qs;
+                }
               }
-              q[t] += qs;
-            } else if (t <= tmax) {
-              const double vl = l < n ? vj[l] : 0.0;
-#pragma unroll
-              for (int r = 0; r < 8; ++r) {
-                const int i = i0 + r;
-                a[r] = (i < n && l <= i) ? arow[(int64_t)r * n + l] : 0.0;
-              }
-              double qs = 0.0;
-#pragma unroll
-              for (int r = 0; r < 8; ++r) {
-                acc[r] += a[r] * vl;
-                if (l < i0 + r) qs += a[r] * vi[r];
-              }
-              q[t] += qs;
-            }
-            if (t == 0 && grab && lane == 0) {
-#pragma unroll
-              for (int r = 0; r < 8; ++r)
-                if (i0 + r < n) colbuf[i0 + r] = a[r];
             }
           }
           // lane l ends with the sum of row 4*b5 + 2*b4 + b3 (b = bits of l)
           double v4[4], v2[2], v1;
-          {
-            const bool hi = lane & 32;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-              const double send = hi ? acc[k] : acc[k + 4];
-              v4[k] = (hi ? acc[k + 4] : acc[k]) + __shfl_xor(send, 32, 64);
-            }
+          for (int k = 0; k < 4; ++k) {
+            double x0 = acc[k], x1 = acc[k + 4];
+            swap32(x0, x1);
+            v4[k] = x0 + x1;
           }
-          {
-            const bool hi = lane & 16;
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-              const double send = hi ? v4[k] : v4[k + 2];
-              v2[k] = (hi ? v4[k + 2] : v4[k]) + __shfl_xor(send, 16, 64);
-            }
+          for (int k = 0; k < 2; ++k) {
+            double y0 = v4[k], y1 = v4[k + 2];
+            swap16(y0, y1);
+            v2[k] = y0 + y1;
           }
           {
             const bool hi = lane & 8;
             const double send = hi ? v2[0] : v2[1];
-            v1 = (hi ? v2[1] : v2[0]) + __shfl_xor(send, 8, 64);
+            v1 = (hi ? v2[1] : v2[0]) + dpp_f64<kDppMirror16>(send);
           }
-          v1 += __shfl_xor(v1, 4, 64);
-          v1 += __shfl_xor(v1, 2, 64);
-          v1 += __shfl_xor(v1, 1, 64);
+          v1 += dpp_f64<kDppMirror8>(v1);
+          v1 += dpp_f64<kDppXor2>(v1);
+          v1 += dpp_f64<kDppXor1>(v1);
           const int rho = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
           if ((lane & 7) == 0 && i0 + rho < n) x[i0 + rho] = v1;
         }
