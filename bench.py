@@ -153,16 +153,11 @@ def serve(eng, rate: float, seconds: float, threshold: float, policy: str, seed:
                      "clusters n~U{2..200} 50/50, seed %d" % seed}
 
 
-def calib_copy(nbytes: int = 1 << 30):
-    """One plain device copy of known size, the scale for the PMC byte counters."""
-    import torch
-    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
-    torch.cuda.synchronize()
-    torch.mul(x, 2.0, out=y)  # one elementwise kernel: reads and writes nbytes each
-    torch.cuda.synchronize()
-    print("copy kernel: %d bytes read, %d bytes written" % (nbytes, nbytes), file=sys.stderr,
-          flush=True)
+def calib_copy(eng, nbytes: int = 1 << 30):
+    """One plain device copy kernel of known size, the scale for the PMC byte counters."""
+    eng.calibration_copy(nbytes)
+    print("k_calibration_copy: %d bytes read, %d bytes written" % (nbytes, nbytes),
+          file=sys.stderr, flush=True)
 
 
 def main():
@@ -193,7 +188,7 @@ def main():
     occ = np.array([n_occ(m) for m in mols])
     eng = api.Engine(0)
     if args.calib_copy:
-        calib_copy()
+        calib_copy(eng)
     t0 = time.time()
     b = eng.batch(mols)
     upload_s = time.time() - t0

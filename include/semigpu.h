@@ -163,6 +163,11 @@ double sg_server_clock(const sg_server* srv);
  * bisect, invit, backtransform (the eigensolver's four kernels), density,
  * energy, control.  counts[] gets the number of launches timed. */
 #define SG_N_STAGES 11
+/* One plain device-to-device copy kernel (k_calibration_copy) over freshly
+ * allocated buffers: reads and writes exactly `bytes` (a multiple of 16).  No
+ * counterpart in the reference: it is the known-size kernel that hardware byte
+ * counters (FETCH_SIZE, WRITE_SIZE) are scaled by. */
+int sg_calibration_copy(sg_context* ctx, long long bytes);
 int sg_timing_enable(sg_context* ctx, int on);
 int sg_timing_read(sg_context* ctx, double* ms, int* counts);
 int sg_timing_reset(sg_context* ctx);

@@ -34,7 +34,7 @@ EXPORTS = (
     "sg_scf_run_policy",
     "sg_pair_block_count", "sg_get_pair_blocks", "sg_get_packed", "sg_set_packed",
     "sg_get_eigen", "sg_set_eigen", "sg_batch_create_bohr", "sg_set_pair_blocks",
-    "sg_get_status", "sg_timing_enable", "sg_timing_read", "sg_timing_reset",
+    "sg_get_status", "sg_calibration_copy", "sg_timing_enable", "sg_timing_read", "sg_timing_reset",
     "sg_server_create", "sg_server_destroy", "sg_server_submit", "sg_server_step",
     "sg_server_poll", "sg_server_clock",
 )
@@ -97,6 +97,7 @@ def load_library(path: str = LIB_PATH):
     L.sg_get_status.argtypes = [_vp, _ip]
     L.sg_batch_create_bohr.argtypes = [_vp, ctypes.c_int, _ip, _ip, _dp, ctypes.POINTER(_vp)]
     L.sg_timing_enable.argtypes = [_vp, ctypes.c_int]
+    L.sg_calibration_copy.argtypes = [_vp, ctypes.c_longlong]
     L.sg_timing_read.argtypes = [_vp, _dp, _ip]
     L.sg_timing_reset.argtypes = [_vp]
     L.sg_server_create.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -170,6 +171,10 @@ class Engine:
 
     def timing_reset(self):
         _check(self.L.sg_timing_reset(self.h))
+
+    def calibration_copy(self, nbytes: int):
+        """One plain copy kernel reading and writing nbytes (byte-counter scale)."""
+        _check(self.L.sg_calibration_copy(self.h, int(nbytes)))
 
 
 class Batch:

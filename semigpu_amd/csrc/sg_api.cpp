@@ -50,6 +50,14 @@ struct sg_batch {
   int g_max_steps = -1;
 };
 
+// Plain copy, 16 bytes per lane per step: reads and writes exactly n16 * 16 bytes.
+__global__ void k_calibration_copy(const double2* __restrict__ src, double2* __restrict__ dst,
+                                   int64_t n16) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 extern "C" {
 
 const char* sg_last_error(void) { return g_err.c_str(); }
@@ -81,6 +89,25 @@ int sg_ctx_destroy(sg_context* c) {
   }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+  return SG_OK;
+}
+
+int sg_calibration_copy(sg_context* c, long long bytes) {
+  if (!c || bytes <= 0 || bytes % 16) return fail(SG_ERR_INVALID, "sg_calibration_copy: bytes must be a positive multiple of 16");
+  HIP_TRY(hipSetDevice(c->device));
+  void *src = nullptr, *dst = nullptr;
+  HIP_TRY(hipMalloc(&src, (size_t)bytes));
+  hipError_t e = hipMalloc(&dst, (size_t)bytes);
+  if (e == hipSuccess) e = hipMemsetAsync(src, 0, (size_t)bytes, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) {
+    k_calibration_copy<<<4096, 256, 0, c->stream>>>((const double2*)src, (double2*)dst, bytes / 16);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(src);
+  if (dst) (void)hipFree(dst);
+  if (e != hipSuccess) return fail(SG_ERR_HIP, std::string("sg_calibration_copy: ") + hipGetErrorString(e));
   return SG_OK;
 }
 

@@ -136,6 +136,7 @@ __device__ __forceinline__ double load_f64(__amdgpu_buffer_rsrc_t r, int byte_of
   return __longlong_as_double(((long long)v[1] << 32) | (unsigned int)v[0]);
 }
 constexpr int kSymvGroup = 2;  // 64-column tiles whose loads are issued together
+constexpr int kOutside = 0x7ffffff0;  // byte offset past any matrix buffer
 
 // Operand layout of the 16x16x4 matrix-core instructions: lane l supplies
 // A[l%16][l/16] and B[l/16][l%16].  Result register r of lane l holds
@@ -271,12 +272,17 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_
 #pragma unroll
           for (int tg = 0; tg < TMAX; tg += kSymvGroup) {
             if (tg < tcount) {
+              // entries right of the diagonal get an offset past the buffer:
+              // they read 0 and cost no memory traffic
               double a[kSymvGroup][8];
 #pragma unroll
               for (int g = 0; g < kSymvGroup; ++g)
 #pragma unroll
-                for (int r = 0; r < 8; ++r)
-                  a[g][r] = tg + g < TMAX ? load_f64(arsrc, voff[r] + 512 * (tg + g)) : 0.0;
+                for (int r = 0; r < 8; ++r) {
+                  const int t = tg + g;
+                  const bool in = lane + 64 * t <= i0 + r - m1;
+                  a[g][r] = t < TMAX ? load_f64(arsrc, in ? voff[r] + 512 * t : kOutside) : 0.0;
+                }
 #pragma unroll
               for (int g = 0; g < kSymvGroup; ++g) {
                 const int t = tg + g;
@@ -427,6 +433,16 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_
 }
 
 // ---------------------------------------------------------------------------
+// 1/q from the hardware reciprocal estimate and two Newton-Raphson steps
+// (r <- r + r (1 - q r)), shorter than the full IEEE division sequence; the
+// pivots are bounded away from 0 by pivmin.
+__device__ __forceinline__ double rcp_nr(double q) {
+  double r = __builtin_amdgcn_rcp(q);
+  r = fma(r, fma(-q, r, 1.0), r);
+  r = fma(r, fma(-q, r, 1.0), r);
+  return r;
+}
+
 // Sturm count: number of eigenvalues of T below x (LDL^T pivots < 0).
 __device__ __forceinline__ int sturm(const double* d, const double* e2, int n, double x,
                                      double pivmin) {
@@ -435,7 +451,7 @@ __device__ __forceinline__ int sturm(const double* d, const double* e2, int n, d
   if (fabs(q) < pivmin) q = -pivmin;
   cnt += q < 0.0;
   for (int k = 1; k < n; ++k) {
-    q = (d[k] - x) - e2[k - 1] / q;
+    q = (d[k] - x) - e2[k - 1] * rcp_nr(q);
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += q < 0.0;
   }
@@ -451,8 +467,8 @@ __device__ __forceinline__ void sturm2(const double* d, const double* e2, int n,
   int n1 = q1 < 0.0, n2 = q2 < 0.0;
   for (int k = 1; k < n; ++k) {
     const double dk = d[k], ek = e2[k - 1];
-    q1 = (dk - x1) - ek / q1;
-    q2 = (dk - x2) - ek / q2;
+    q1 = (dk - x1) - ek * rcp_nr(q1);
+    q2 = (dk - x2) - ek * rcp_nr(q2);
     if (fabs(q1) < pivmin) q1 = -pivmin;
     if (fabs(q2) < pivmin) q2 = -pivmin;
     n1 += q1 < 0.0;
@@ -521,37 +537,63 @@ __global__ void __launch_bounds__(256) k_bisect(const MolDesc* __restrict__ mols
   glo -= 2.0 * kEps * tnorm + 2.0 * pivmin;
   ghi += 2.0 * kEps * tnorm + 2.0 * pivmin;
   if (blockIdx.x == 0 && threadIdx.x == 0) tnormA[m] = tnorm;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= k) return;
-  const int idx = lo + j;
-  // Bisection keeping count(a) <= idx < count(b).  From the second SCF step
-  // on, the previous step's eigenvalue (the same index) brackets the new one
-  // closely: [prev - delta, prev + delta] is used when two Sturm counts
-  // confirm it, with delta = 1e-7 ||T||, else 1e-4 ||T||, else Gershgorin.
-  double a = glo, b = ghi;
-  if (steps && steps[m] > 0) {
-    const double prev = lamA[M.vec0 + j];
+  // two eigenvalues per lane (indices j and j + half), bisected in lockstep
+  // so that their Sturm recurrences interleave
+  const int half = (k + 1) >> 1;
+  const int j0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j0 >= half) return;
+  const int jj[2] = {j0, j0 + half};
+  double lo_[2], hi_[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int j = jj[u];
+    const int idx = lo + j;
+    // From the second SCF step on, the previous step's eigenvalue (same
+    // index) brackets the new one closely: [prev - delta, prev + delta] is used
+    // when two Sturm counts confirm it, with delta = 1e-7 ||T||, else
+    // 1e-4 ||T||, else the Gershgorin interval.
+    double a = glo, b = ghi;
+    if (j < k && steps && steps[m] > 0) {
+      const double prev = lamA[M.vec0 + j];
 #pragma unroll 1
-    for (int lvl = 0; lvl < 2; ++lvl) {
-      const double del = (lvl == 0 ? 1e-7 : 1e-4) * tnorm;
-      const double x1 = prev - del, x2 = prev + del;
-      if (!(x1 > glo && x2 < ghi)) continue;
-      int c1, c2;
-      sturm2(d, e2, n, x1, x2, pivmin, c1, c2);
-      if (c1 <= idx && c2 > idx) {
-        a = x1;
-        b = x2;
-        break;
+      for (int lvl = 0; lvl < 2; ++lvl) {
+        const double del = (lvl == 0 ? 1e-7 : 1e-4) * tnorm;
+        const double x1 = prev - del, x2 = prev + del;
+        if (!(x1 > glo && x2 < ghi)) continue;
+        int c1, c2;
+        sturm2(d, e2, n, x1, x2, pivmin, c1, c2);
+        if (c1 <= idx && c2 > idx) {
+          a = x1;
+          b = x2;
+          break;
+        }
       }
     }
+    lo_[u] = a;
+    hi_[u] = b;
   }
-  for (int it = 0; it < 128; ++it) {
-    if (b - a <= 2.0 * kEps * fmax(fabs(a), fabs(b)) + pivmin) break;
-    const double mid = 0.5 * (a + b);
-    if (sturm(d, e2, n, mid, pivmin) > idx) b = mid;
-    else a = mid;
+  // bisection keeping count(a) <= idx < count(b)
+  bool done[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+    done[u] = jj[u] >= k ||
+              hi_[u] - lo_[u] <= 2.0 * kEps * fmax(fabs(lo_[u]), fabs(hi_[u])) + pivmin;
+  for (int it = 0; it < 128 && !(done[0] && done[1]); ++it) {
+    const double m0 = 0.5 * (lo_[0] + hi_[0]), m1 = 0.5 * (lo_[1] + hi_[1]);
+    int c[2];
+    sturm2(d, e2, n, m0, m1, pivmin, c[0], c[1]);
+    const double mid[2] = {m0, m1};
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (done[u]) continue;
+      if (c[u] > lo + jj[u]) hi_[u] = mid[u];
+      else lo_[u] = mid[u];
+      done[u] = hi_[u] - lo_[u] <= 2.0 * kEps * fmax(fabs(lo_[u]), fabs(hi_[u])) + pivmin;
+    }
   }
-  lamA[M.vec0 + j] = 0.5 * (a + b);
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+    if (jj[u] < k) lamA[M.vec0 + jj[u]] = 0.5 * (lo_[u] + hi_[u]);
 }
 
 // ---------------------------------------------------------------------------
