@@ -948,41 +948,74 @@ __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict_
 
 // ---------------------------------------------------------------------------
 // P from the selected vectors Y (N x k, row-major): 2 Y Y^T (occupied side)
-// or 2 (I - Y Y^T) (virtual side), on the matrix cores.  One wavefront per
-// 16 x 16 tile of P, k in steps of 4 with v_mfma_f64_16x16x4f64 (FP64 steps)
-// or v_mfma_f32_16x16x4f32 (FP32 steps of the mixed-precision policy).
+// or 2 (I - Y Y^T) (virtual side), on the matrix cores with
+// v_mfma_f64_16x16x4f64 (FP64 steps) or v_mfma_f32_16x16x4f32 (FP32 steps of
+// the mixed-precision policy).  One wave computes a 32 x 32 block of the lower
+// triangle as 2 x 2 tiles of 16 x 16 (each operand fragment feeds two
+// instructions), K = k in steps of 4, and stores it with its mirror image.
 template <typename T>
-__device__ __forceinline__ void density_tile(const double* __restrict__ Y, int n, int k, int lo,
-                                             int i0, int j0, int lane, double* __restrict__ p) {
-  typename Mfma16<T>::v4 acc = {0, 0, 0, 0};
-  const int ra = i0 + (lane & 15), rb = j0 + (lane & 15), kk = lane >> 4;
-  const bool oka = ra < n, okb = rb < n;
-  const double* ya = Y + (int64_t)(oka ? ra : 0) * k;
-  const double* yb = Y + (int64_t)(okb ? rb : 0) * k;
-  int c = 0;
-  for (; c + 16 <= k; c += 16) {
-    T a[4], b[4];
+__device__ __forceinline__ void density_block(const double* __restrict__ Y, int n, int k, int lo,
+                                              int i0, int j0, int lane, double* __restrict__ p,
+                                              double* __restrict__ tr) {
+  typedef typename Mfma16<T>::v4 v4;
+  v4 acc[2][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
+  const int lr = lane & 15, kk = lane >> 4;
+  const double* ya[2];
+  const double* yb[2];
+  bool oka[2], okb[2];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      a[u] = oka ? (T)ya[c + 4 * u + kk] : (T)0;
-      b[u] = okb ? (T)yb[c + 4 * u + kk] : (T)0;
+  for (int h = 0; h < 2; ++h) {
+    const int ra = i0 + 16 * h + lr, rb = j0 + 16 * h + lr;
+    oka[h] = ra < n;
+    okb[h] = rb < n;
+    ya[h] = Y + (int64_t)(oka[h] ? ra : 0) * k;
+    yb[h] = Y + (int64_t)(okb[h] ? rb : 0) * k;
+  }
+  // four k-steps per round: 16 fragments in flight, then 16 instructions
+  for (int c0 = 0; c0 < k; c0 += 16) {
+    T a[4][2], b[4][2];
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const int cc = c0 + 4 * st + kk;
+      const bool okc = cc < k;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        a[st][h] = (oka[h] && okc) ? (T)ya[h][cc] : (T)0;
+        b[st][h] = (okb[h] && okc) ? (T)yb[h][cc] : (T)0;
+      }
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) acc = Mfma16<T>::op(a[u], b[u], acc);
-  }
-  for (; c < k; c += 4) {
-    const int cc = c + kk;
-    const T a = (oka && cc < k) ? (T)ya[cc] : (T)0;
-    const T b = (okb && cc < k) ? (T)yb[cc] : (T)0;
-    acc = Mfma16<T>::op(a, b, acc);
-  }
+    for (int st = 0; st < 4; ++st)
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int i = i0 + Mfma16<T>::row(lane, r), j = j0 + (lane & 15);
-    if (i < n && j < n) {
-      const double s = (double)acc[r];
-      p[(int64_t)i * n + j] = lo == 0 ? 2.0 * s : 2.0 * ((i == j ? 1.0 : 0.0) - s);
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) acc[u][v] = Mfma16<T>::op(a[st][u], b[st][v], acc[u][v]);
+  }
+  // rows of the block straight out; for a block below the diagonal also its
+  // transpose, through LDS so that both stores run along rows
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ii = 16 * u + Mfma16<T>::row(lane, r), jj = 16 * v + lr;
+        const int i = i0 + ii, j = j0 + jj;
+        const double s = (double)acc[u][v][r];
+        const double val = lo == 0 ? 2.0 * s : 2.0 * ((i == j ? 1.0 : 0.0) - s);
+        if (i < n && j < n) p[(int64_t)i * n + j] = val;
+        if (i0 != j0) tr[ii * 33 + jj] = val;
+      }
+  if (i0 != j0) {
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    for (int t = lane; t < 1024; t += 64) {
+      const int jj = t >> 5, ii = t & 31;
+      const int i = i0 + ii, j = j0 + jj;
+      if (i < n && j < n) p[(int64_t)j * n + i] = tr[ii * 33 + jj];
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -1002,14 +1035,18 @@ __global__ void __launch_bounds__(256) k_density(const MolDesc* __restrict__ mol
   const double* Y = work + 5 * M.mat0 + (int64_t)n * n;
   double* p = P + M.mat0;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int t16 = (n + 15) >> 4;
+  const int t32 = (n + 31) >> 5;
+  const int nblk = t32 * (t32 + 1) / 2;  // lower-triangle 32 x 32 blocks, row order
   const bool fp32 = prec && prec[m];
-  for (int tile = blockIdx.x * 4 + wave; tile < t16 * t16; tile += gridDim.x * 4) {
-    const int i0 = (tile / t16) * 16, j0 = (tile % t16) * 16;
-    if (fp32) density_tile<float>(Y, n, k, lo, i0, j0, lane, p);
-    else density_tile<double>(Y, n, k, lo, i0, j0, lane, p);
+  __shared__ double trans[4][32 * 33];  // per-wave transpose buffer
+  for (int blk = blockIdx.x * 4 + wave; blk < nblk; blk += gridDim.x * 4) {
+    int bi = 0, bj = blk;
+    while (bj > bi) bj -= ++bi;
+    if (fp32) density_block<float>(Y, n, k, lo, 32 * bi, 32 * bj, lane, p, trans[wave]);
+    else density_block<double>(Y, n, k, lo, 32 * bi, 32 * bj, lane, p, trans[wave]);
   }
 }
+
 
 // P = 2 C_occ C_occ^T from the column-major eigenvectors of a full
 // decomposition (sg_eigh followed by sg_density).
@@ -1061,9 +1098,12 @@ void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all) {
 }
 
 void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all) {
+  // two eigenvalues per lane: (k+1)/2 lanes, k <= N/2 on the SCF path
   const int N = b.max_n;
-  dim3 gb((N + 255) / 256, b.n_mol);
-  k_bisect<<<gb, 256, sizeof(double) * (2 * N + 32), s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm,
+  const int half = ((all ? N : N / 2) + 1) / 2;
+  const int threads = min(256, max(64, (half + 63) / 64 * 64));
+  dim3 gb((half + threads - 1) / threads, b.n_mol);
+  k_bisect<<<gb, threads, sizeof(double) * (2 * N + 32), s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm,
                                                           all ? 1 : 0, all ? nullptr : b.active,
                                                           b.status, b.order,
                                                           all ? nullptr : b.steps);
@@ -1072,8 +1112,11 @@ void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all) {
 void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all) {
   const int N = b.max_n;
   const size_t inv_lds = sizeof(double) * (3 * (size_t)N) + sizeof(int) * (2 * (size_t)N + 2);
-  k_invit<<<b.n_mol, kT, inv_lds, s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm, b.W, all ? 1 : 0,
-                                       all ? nullptr : b.active, b.status, b.order);
+  // one lane per vector: only as many waves as the widest selection needs
+  const int kmax = all ? N : N / 2;
+  const int threads = min(kT, max(64, (kmax + 63) / 64 * 64));
+  k_invit<<<b.n_mol, threads, inv_lds, s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm, b.W, all ? 1 : 0,
+                                            all ? nullptr : b.active, b.status, b.order);
 }
 
 void launch_eigh_back(const BatchDev& b, hipStream_t s, bool all) {
@@ -1098,11 +1141,15 @@ void launch_eigh(const BatchDev& b, hipStream_t s, bool all) {
 }
 
 void launch_density(const BatchDev& b, hipStream_t s, bool all) {
-  dim3 grid(64, b.n_mol);
-  if (all)
+  if (all) {
+    dim3 grid(64, b.n_mol);
     k_density_from_c<<<grid, 256, 0, s>>>(b.mols, b.C, b.P, 64);
-  else
+  } else {
+    // as many workgroups per molecule as the largest one has 4-block groups
+    const int t32 = (b.max_n + 31) >> 5;
+    dim3 grid((t32 * (t32 + 1) / 2 + 3) / 4, b.n_mol);
     k_density<<<grid, 256, 0, s>>>(b.mols, b.W, b.P, b.active, b.status, b.prec, b.order);
+  }
 }
 
 }  // namespace sg
