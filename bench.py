@@ -224,12 +224,15 @@ def main():
     tri_avg_ms = tri_ms / max(1, tri_cnt)
     tri_flops_launch = tri_flops_step * K / max(1, tri_cnt)
     tri_tflops = tri_flops_launch / (tri_avg_ms * 1e-3) / 1e12
+    traffic = pmc_traffic()
     roofline = {"kernel": "k_tridiag", "bound": "mfma", "achieved": round(tri_tflops, 4),
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(tri_tflops / FP64_PEAK_TFLOPS, 6), "traffic": pmc_traffic(),
+                "frac": round(tri_tflops / FP64_PEAK_TFLOPS, 6), "traffic": traffic,
                 "avg_launch_ms": round(tri_avg_ms, 4),
                 "flops_per_launch": tri_flops_launch,
-                "work": "(4/3) N^3 FP64 flops per active molecule per launch"}
+                "work": "(4/3) N^3 FP64 flops per active molecule per launch",
+                "traffic_rate_GBs": (round(traffic / (tri_avg_ms * 1e-3) / 1e9, 1)
+                                     if traffic else None)}
     T = sizes * (sizes + 1) / 2
     I = np.array([pair_block_doubles(m) for m in mols], dtype=np.float64)
     fock_ms, fock_cnt = stage["fock"]
