@@ -48,6 +48,14 @@ int sg_version(void);
 /* Context = one GPU + one stream. */
 int sg_ctx_create(int device, sg_context** out);
 int sg_ctx_destroy(sg_context* ctx);
+/* Basis size from which the molecules of batches created afterwards take the
+ * multi-workgroup eigensolver (one molecule at a time over the whole GPU)
+ * instead of one workgroup per molecule.  Default 1281: exactly the molecules
+ * above the one-workgroup limit of 1280.  Values from 128 up are accepted, so
+ * that the two solvers can be compared on the same molecules.  The largest
+ * basis a batch accepts is 8192.  No counterpart in the reference, whose
+ * dsyev_ call (scfcalculation.cpp:169-203) takes any size on the host. */
+int sg_ctx_set_large_basis(sg_context* ctx, int n_min);
 
 /* Molecules: atoms of molecule m are atom_offsets[m] .. atom_offsets[m+1]-1 of
  * Z[] / xyz_ang[3*i..3*i+2].  Replaces ReadXYZFile + ListAtomicOrbitals::SetOrbitals

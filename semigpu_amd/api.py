@@ -36,7 +36,7 @@ EXPORTS = (
     "sg_get_eigen", "sg_set_eigen", "sg_batch_create_bohr", "sg_set_pair_blocks",
     "sg_get_status", "sg_calibration_copy", "sg_timing_enable", "sg_timing_read", "sg_timing_reset",
     "sg_server_create", "sg_server_destroy", "sg_server_submit", "sg_server_step",
-    "sg_server_poll", "sg_server_clock",
+    "sg_server_poll", "sg_server_clock", "sg_ctx_set_large_basis",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -76,6 +76,7 @@ def load_library(path: str = LIB_PATH):
     L.sg_last_error.restype = ctypes.c_char_p
     L.sg_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
     L.sg_ctx_destroy.argtypes = [_vp]
+    L.sg_ctx_set_large_basis.argtypes = [_vp, ctypes.c_int]
     L.sg_batch_create.argtypes = [_vp, ctypes.c_int, _ip, _ip, _dp, ctypes.POINTER(_vp)]
     L.sg_batch_destroy.argtypes = [_vp]
     L.sg_batch_molecule_info.argtypes = [_vp, ctypes.c_int, _ip, _ip, _ip]
@@ -159,6 +160,11 @@ class Engine:
 
     def server(self, **kw) -> "Server":
         return Server(self, **kw)
+
+    def set_large_basis(self, n_min: int):
+        """Molecules of later batches with N >= n_min take the multi-workgroup
+        eigensolver (default: only those above the one-workgroup limit, 1280)."""
+        _check(self.L.sg_ctx_set_large_basis(self.h, int(n_min)))
 
     def timing(self, on: bool = True):
         _check(self.L.sg_timing_enable(self.h, 1 if on else 0))

@@ -35,6 +35,7 @@ UNITS = {
     "sg_integrals.hip": ["-ffp-contract=off"],
     "sg_fock.hip": [],
     "sg_linalg.hip": [],
+    "sg_large.hip": [],
     "sg_api.cpp": ["-x", "hip"],
     "sg_server.cpp": ["-x", "hip"],
 }

@@ -45,6 +45,8 @@ struct sg_batch {
   std::vector<int64_t> blk_count;  // doubles of pair blocks per molecule
   bool have_ints = false, have_hcore = false, have_ecc = false, have_fp32 = false;
   std::vector<double> e_cc;
+  std::vector<int> large_idx;       // molecules on the multi-workgroup eigensolver
+  std::vector<MolDesc> large_mols;  // their descriptors
   hipGraphExec_t iter_exec = nullptr;  // captured SCF iteration
   double g_threshold = -1.0, g_switch = -1.0;
   int g_max_steps = -1;
@@ -89,6 +91,14 @@ int sg_ctx_destroy(sg_context* c) {
   }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+  return SG_OK;
+}
+
+int sg_ctx_set_large_basis(sg_context* c, int n_min) {
+  if (!c) return fail(SG_ERR_INVALID, "null context");
+  if (n_min < kMinLarge)
+    return fail(SG_ERR_INVALID, "large-basis threshold below " + std::to_string(kMinLarge));
+  c->large_min = n_min;
   return SG_OK;
 }
 
@@ -229,9 +239,9 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
     blk0 += row;
     max_n = std::max(max_n, ao);
   }
-  if (max_n > kMaxBasis) {
+  if (max_n > kMaxLarge) {
     delete b;
-    return fail(SG_ERR_INVALID, "molecules above " + std::to_string(kMaxBasis) +
+    return fail(SG_ERR_INVALID, "molecules above " + std::to_string(kMaxLarge) +
                                     " basis functions are not supported");
   }
   BatchDev& d = b->d;
@@ -246,6 +256,7 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   d.n_hh = (int)phh.size();
   d.n_pairs_all = (int)pall.size();
   d.n_diag_tasks = (int)diag.size();
+  d.large_min = std::min(ctx->large_min, kMaxBasis + 1);
 
   // one allocation, 256-byte aligned sub-buffers
   size_t off = 0;
@@ -327,6 +338,19 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   std::stable_sort(ord.begin(), ord.end(),
                    [&](int x, int y) { return b->mols[x].n_ao > b->mols[y].n_ao; });
   if (e == hipSuccess) e = up(order, ord.data(), sizeof(int) * n_mol);
+  // the molecules at or above large_min lead the order
+  for (int i = 0; i < n_mol; ++i) {
+    const MolDesc& M = b->mols[ord[i]];
+    if (M.n_ao >= d.large_min) {
+      b->large_idx.push_back(ord[i]);
+      b->large_mols.push_back(M);
+    } else {
+      d.max_n_small = std::max(d.max_n_small, M.n_ao);
+    }
+  }
+  d.n_large = (int)b->large_idx.size();
+  d.large_idx = b->large_idx.data();
+  d.large_mols = b->large_mols.data();
   for (double* z : {d.H, d.F, d.P, d.C})
     if (e == hipSuccess) e = hipMemsetAsync(z, 0, sizeof(double) * mat0, s);
   if (e == hipSuccess) e = hipMemsetAsync(d.status, 0, sizeof(int) * n_mol, s);
@@ -512,7 +536,9 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
   const int fp32_cap = 40;
   // Without stage timing, one iteration is captured once as a HIP graph and
   // replayed; its kernel arguments depend on the batch and on these settings.
-  const bool use_graph = !c->timing;
+  // Batches with molecules on the multi-workgroup eigensolver (thousands of
+  // launches per iteration) launch directly.
+  const bool use_graph = !c->timing && b->d.n_large == 0;
   if (use_graph && (!b->iter_exec || b->g_threshold != threshold || b->g_max_steps != max_steps ||
                     b->g_switch != switch_ev)) {
     if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);

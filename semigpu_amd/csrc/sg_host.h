@@ -35,6 +35,7 @@ struct sg_context {
   int device = 0;
   hipStream_t stream = nullptr;
   bool timing = false;
+  int large_min = sg::kMaxBasis + 1;  // batches created from now on
   double ms[SG_N_STAGES] = {0};
   int counts[SG_N_STAGES] = {0};
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;

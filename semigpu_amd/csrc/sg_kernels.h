@@ -10,6 +10,13 @@ namespace sg {
 // Largest basis the eigensolver handles in one workgroup (LDS-resident
 // tridiagonal, 20 x 64-column row chunks in the reduction).
 constexpr int kMaxBasis = 1280;
+// Largest basis of the multi-workgroup eigensolver (bisection keeps d and
+// e^2 of one tridiagonal in LDS: 2 x 8192 doubles).
+constexpr int kMaxLarge = 8192;
+// Smallest basis the multi-workgroup eigensolver accepts (its panels and
+// tables share the last N^2 doubles of the molecule's workspace).
+constexpr int kMinLarge = 128;
+constexpr int kLargeNb = 32;  // reflectors per panel of the multi-workgroup solver
 
 // Device view of a batch of molecules.  All matrices are dense N x N per
 // molecule, row-major (symmetric ones are stored in full); eigenvectors are
@@ -55,6 +62,67 @@ struct BatchDev {
   // eigensolver and density launches take their workgroups in this order, so
   // the longest jobs start first and the launch tail is short.
   const int* order = nullptr;
+  // Molecules with N >= large_min take the multi-workgroup eigensolver (one
+  // molecule at a time over the whole GPU).  They are the first n_large
+  // entries of `order`; large_idx / large_mols are host copies of their
+  // indices and descriptors.  max_n_small is the largest N of the others.
+  int large_min = kMaxBasis + 1;
+  int n_large = 0;
+  const int* large_idx = nullptr;
+  const MolDesc* large_mols = nullptr;
+  int max_n_small = 0;
+};
+
+// Molecule m takes part in a launch: every molecule (active null), or one
+// still iterating without an eigensolver failure.
+__device__ __forceinline__ bool mol_live(const int* active, const int* status, int m) {
+  return !active || (active[m] && !status[m]);
+}
+
+// Selected eigenvalue range: all, or the smaller of occupied / virtual.
+__host__ __device__ inline void selection(const MolDesc& M, int want_all, int& lo, int& k) {
+  const int n = M.n_ao, no = M.n_occ;
+  if (want_all) {
+    lo = 0;
+    k = n;
+  } else if (no <= n - no) {
+    lo = 0;
+    k = no;
+  } else {
+    lo = no;
+    k = n - no;
+  }
+}
+
+// Workspace of a molecule on the multi-workgroup eigensolver (offsets in
+// doubles from the molecule's 5 N^2 block):
+//   0       A: the reduced matrix, full symmetric; reflector c in the upper part of row c
+//   N^2     Y: selected eigenvectors, N x k row-major (as the one-workgroup solver)
+//   2 N^2   inverse-iteration factors, 3 N x L
+//   4 N^2   V and W panels (kLargeNb x N each), the product vector (N), the
+//           compact-WY T factors (16 x 16 per group of 16 reflectors), the
+//           inverse-iteration shifts and cluster tables
+struct LargeWork {
+  double *A, *Y, *fac, *V, *W, *py, *T, *sh;
+  int *cstart, *clist;  // clist[k + 1] holds the number of clusters
+  __host__ __device__ LargeWork(double* work, const MolDesc& M) {
+    const int64_t n = M.n_ao, nn = n * n;
+    A = work + 5 * M.mat0;
+    Y = A + nn;
+    fac = A + 2 * nn;
+    V = A + 4 * nn;
+    W = V + kLargeNb * n;
+    py = W + kLargeNb * n;
+    T = py + n;
+    sh = T + 16 * n + 256;
+    cstart = (int*)(sh + n);
+    clist = cstart + n;
+  }
+  // vectors whose factors are in flight at once (3 N x L doubles of factors)
+  static __host__ __device__ int lanes(int n, int k) {
+    int l = (2 * n / 3) & ~63;
+    return k < l ? k : l;
+  }
 };
 
 // `mask` (optional, per molecule) restricts a launch to the molecules whose
@@ -84,6 +152,12 @@ void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all);
 void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all);
 void launch_eigh_back(const BatchDev& b, hipStream_t s, bool all);
 void launch_eigh(const BatchDev& b, hipStream_t s, bool all);
+// Multi-workgroup stages for molecule m (descriptor M, host copy); `active`
+// null for every molecule, else only while active[m] and !status[m].
+void launch_tridiag_large(const BatchDev& b, hipStream_t s, int m, const MolDesc& M,
+                          const int* active);
+void launch_back_large(const BatchDev& b, hipStream_t s, int m, const MolDesc& M,
+                       const int* active, bool all);
 void launch_density(const BatchDev& b, hipStream_t s, bool all);
 
 }  // namespace sg

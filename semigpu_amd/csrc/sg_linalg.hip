@@ -102,25 +102,6 @@ __device__ double block_sum(double v, double* red) {
   return t;
 }
 
-__device__ __forceinline__ bool mol_live(const int* active, const int* status, int m) {
-  return !active || (active[m] && !status[m]);
-}
-
-// Selected eigenvalue range: all, or the smaller of occupied / virtual.
-__device__ __forceinline__ void selection(const MolDesc& M, int want_all, int& lo, int& k) {
-  const int n = M.n_ao, no = M.n_occ;
-  if (want_all) {
-    lo = 0;
-    k = n;
-  } else if (no <= n - no) {
-    lo = 0;
-    k = no;
-  } else {
-    lo = no;
-    k = n - no;
-  }
-}
-
 __device__ __forceinline__ int panel_width(int n) {
   int nb = kPanelLds / (16 * (n > 0 ? n : 1));
   return nb < 1 ? 1 : (nb > kNbMax ? kNbMax : nb);
@@ -690,6 +671,70 @@ __device__ double tri_solve(const double* __restrict__ d, const double* __restri
   return sqrt(nrm);
 }
 
+
+// Clusters of the selected eigenvalues lam[0..k) and the shifts of inverse
+// iteration: consecutive eigenvalues closer than ortol share a cluster.  The
+// full decomposition uses dstein's 1e-3 ||T||; the SCF path, which only needs
+// the subspace, uses 1e-5 ||T|| (vectors of eigenvalues further apart than
+// that are orthogonal to ~eps ||T|| / gap <= 1e-11 without reorthogonalising).
+// Equal shifts inside a cluster are separated by a few ulps.  Returns the
+// number of clusters; clist[0..nc] are their starts and k.
+__device__ int make_clusters(const double* __restrict__ lam, int k, double tnorm, int want_all,
+                             double* __restrict__ sh, int* __restrict__ cstart,
+                             int* __restrict__ clist) {
+  const double ortol = (want_all ? 1e-3 : 1e-5) * tnorm;
+  int nc = 0;
+  for (int j = 0; j < k; ++j) {
+    double s = lam[j];
+    if (j == 0 || lam[j] - lam[j - 1] >= ortol) {
+      clist[nc++] = j;
+    } else {
+      const double pert = 10.0 * kEps * fmax(fabs(s), tnorm);
+      if (s - sh[j - 1] < pert) s = sh[j - 1] + pert;
+    }
+    sh[j] = s;
+    cstart[j] = clist[nc - 1];
+  }
+  clist[nc] = k;
+  return nc;
+}
+
+// Column j of Y (N x k row-major) times inv, 8 rows per batch of loads.
+__device__ void scale_column(double* __restrict__ Y, int n, int k, int j, double inv) {
+  for (int i0 = 0; i0 < n; i0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = i0 + r < n ? Y[(int64_t)(i0 + r) * k + j] : 0.0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+      if (i0 + r < n) Y[(int64_t)(i0 + r) * k + j] = v[r] * inv;
+  }
+}
+
+// Modified Gram-Schmidt over columns c0..c1-1 of Y in order, by one wave.
+__device__ void mgs_cluster(double* __restrict__ Y, int n, int k, int c0, int c1, int lane) {
+  if (c1 - c0 < 2) return;
+  for (int j = c0; j < c1; ++j) {
+    for (int q = c0; q < j; ++q) {
+      double s = 0.0;
+      for (int i = lane; i < n; i += 64) s += Y[(int64_t)i * k + q] * Y[(int64_t)i * k + j];
+      s = wsum(s);
+      for (int i = lane; i < n; i += 64) Y[(int64_t)i * k + j] -= s * Y[(int64_t)i * k + q];
+      __threadfence_block();
+      __builtin_amdgcn_wave_barrier();
+    }
+    double s = 0.0;
+    for (int i = lane; i < n; i += 64) {
+      const double v = Y[(int64_t)i * k + j];
+      s += v * v;
+    }
+    const double inv = 1.0 / sqrt(wsum(s));
+    for (int i = lane; i < n; i += 64) Y[(int64_t)i * k + j] *= inv;
+    __threadfence_block();
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_invit(const MolDesc* __restrict__ mols,
                                                const double* __restrict__ dA,
                                                const double* __restrict__ eA,
@@ -724,28 +769,7 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_
     d[i] = dA[M.vec0 + i];
     e[i] = eA[M.vec0 + i];
   }
-  if (tid == 0) {
-    // clusters: consecutive eigenvalues closer than ortol.  The full
-    // decomposition uses dstein's 1e-3 ||T||; the SCF path, which only needs
-    // the subspace, uses 1e-5 ||T|| (vectors of eigenvalues further apart than
-    // that are orthogonal to ~eps ||T|| / gap <= 1e-11 without reorthogonalising).
-    // Equal shifts inside a cluster are separated by a few ulps.
-    const double ortol = (want_all ? 1e-3 : 1e-5) * tnorm;
-    int nc = 0;
-    for (int j = 0; j < k; ++j) {
-      double s = lam[j];
-      if (j == 0 || lam[j] - lam[j - 1] >= ortol) {
-        clist[nc++] = j;
-      } else {
-        const double pert = 10.0 * kEps * fmax(fabs(s), tnorm);
-        if (s - sh[j - 1] < pert) s = sh[j - 1] + pert;
-      }
-      sh[j] = s;
-      cstart[j] = clist[nc - 1];
-    }
-    clist[nc] = k;
-    n_clusters = nc;
-  }
+  if (tid == 0) n_clusters = make_clusters(lam, k, tnorm, want_all, sh, cstart, clist);
   __syncthreads();
   for (int64_t t = tid; t < (int64_t)n * k; t += nt) {
     const int row = (int)(t / k), col = (int)(t % k);
@@ -756,47 +780,80 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_
     for (int j = tid; j < k; j += nt) {
       const double nrm = tri_solve(d, e, n, sh[j], tiny, Y, k, j, fac + (j % L), L);
       const bool single = cstart[j] == j && (j + 1 == k || cstart[j + 1] != j);
-      if (single && sweep == 2) {
-        const double inv = 1.0 / nrm;
-        for (int i0 = 0; i0 < n; i0 += 8) {
-          double v[8];
-#pragma unroll
-          for (int r = 0; r < 8; ++r) v[r] = i0 + r < n ? Y[(int64_t)(i0 + r) * k + j] : 0.0;
-#pragma unroll
-          for (int r = 0; r < 8; ++r)
-            if (i0 + r < n) Y[(int64_t)(i0 + r) * k + j] = v[r] * inv;
-        }
-      }
+      if (single && sweep == 2) scale_column(Y, n, k, j, 1.0 / nrm);
     }
     __threadfence_block();
     __syncthreads();
     // clusters of two or more: modified Gram-Schmidt in order, one wave each
-    for (int c = wave; c < n_clusters; c += nwave) {
-      const int c0 = clist[c], c1 = clist[c + 1];
-      if (c1 - c0 < 2) continue;
-      for (int j = c0; j < c1; ++j) {
-        for (int q = c0; q < j; ++q) {
-          double s = 0.0;
-          for (int i = lane; i < n; i += 64) s += Y[(int64_t)i * k + q] * Y[(int64_t)i * k + j];
-          s = wsum(s);
-          for (int i = lane; i < n; i += 64) Y[(int64_t)i * k + j] -= s * Y[(int64_t)i * k + q];
-          __threadfence_block();
-          __builtin_amdgcn_wave_barrier();
-        }
-        double s = 0.0;
-        for (int i = lane; i < n; i += 64) {
-          const double v = Y[(int64_t)i * k + j];
-          s += v * v;
-        }
-        const double inv = 1.0 / sqrt(wsum(s));
-        for (int i = lane; i < n; i += 64) Y[(int64_t)i * k + j] *= inv;
-        __threadfence_block();
-        __builtin_amdgcn_wave_barrier();
-      }
-    }
+    for (int c = wave; c < n_clusters; c += nwave) mgs_cluster(Y, n, k, clist[c], clist[c + 1], lane);
     __threadfence_block();
     __syncthreads();
   }
+}
+
+// Inverse iteration of one molecule over the whole GPU (N too large for one
+// workgroup).  Same shifts, start vectors, solves and cluster
+// orthogonalisation as k_invit, so the vectors agree bit for bit with the
+// one-workgroup solver's for the same tridiagonal:
+//   k_li_setup  clusters and shifts (one lane, tables in the workspace),
+//   k_li_solve  one vector per lane, L lanes over many workgroups, factors
+//               interleaved by lane (3 N x L),
+//   k_li_mgs    one wave per cluster of two or more.
+__global__ void k_li_setup(const MolDesc* __restrict__ mols, const double* __restrict__ lamA,
+                           const double* __restrict__ tnormA, double* __restrict__ work,
+                           int want_all, const int* __restrict__ active,
+                           const int* __restrict__ status, int m) {
+  if (threadIdx.x != 0 || !mol_live(active, status, m)) return;
+  const MolDesc M = mols[m];
+  int lo, k;
+  selection(M, want_all, lo, k);
+  LargeWork L(work, M);
+  L.clist[k + 1] = make_clusters(lamA + M.vec0, k, tnormA[m], want_all, L.sh, L.cstart, L.clist);
+}
+
+__global__ void __launch_bounds__(64) k_li_solve(const MolDesc* __restrict__ mols,
+                                                  const double* __restrict__ dA,
+                                                  const double* __restrict__ eA,
+                                                  const double* __restrict__ tnormA,
+                                                  double* __restrict__ work, int want_all,
+                                                  const int* __restrict__ active,
+                                                  const int* __restrict__ status, int m,
+                                                  int sweep) {
+  if (!mol_live(active, status, m)) return;
+  const MolDesc M = mols[m];
+  const int n = M.n_ao;
+  int lo, k;
+  selection(M, want_all, lo, k);
+  const int L = LargeWork::lanes(n, k);
+  const int j0 = blockIdx.x * 64 + threadIdx.x;
+  if (j0 >= L) return;
+  LargeWork W(work, M);
+  const double* d = dA + M.vec0;
+  const double* e = eA + M.vec0;
+  const double tiny = kEps * fmax(tnormA[m], 1e-300);
+  for (int j = j0; j < k; j += L) {
+    if (sweep == 0)
+      for (int i = 0; i < n; ++i) W.Y[(int64_t)i * k + j] = start_entry(lo + j, i);
+    const double nrm = tri_solve(d, e, n, W.sh[j], tiny, W.Y, k, j, W.fac + j0, L);
+    const bool single = W.cstart[j] == j && (j + 1 == k || W.cstart[j + 1] != j);
+    if (single && sweep == 2) scale_column(W.Y, n, k, j, 1.0 / nrm);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_li_mgs(const MolDesc* __restrict__ mols,
+                                                 double* __restrict__ work, int want_all,
+                                                 const int* __restrict__ active,
+                                                 const int* __restrict__ status, int m) {
+  if (!mol_live(active, status, m)) return;
+  const MolDesc M = mols[m];
+  const int n = M.n_ao;
+  int lo, k;
+  selection(M, want_all, lo, k);
+  LargeWork W(work, M);
+  const int nc = W.clist[k + 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int c = blockIdx.x * 4 + wave; c < nc; c += gridDim.x * 4)
+    mgs_cluster(W.Y, n, k, W.clist[c], W.clist[c + 1], lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -1086,47 +1143,91 @@ static size_t tridiag_lds(int max_n) {
                            (size_t)(kT / 64) * max_n);
 }
 
+// The one-workgroup kernels take the molecules below large_min: the tail of
+// `order` after the n_large large ones, with launch sizes from their largest N.
+namespace {
+struct Part {
+  const int* order;
+  int count, max_n;
+};
+Part small_part(const BatchDev& b) {
+  if (b.n_large == 0) return {b.order, b.n_mol, b.max_n};
+  return {b.order + b.n_large, b.n_mol - b.n_large, b.max_n_small};
+}
+}  // namespace
+
 void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all) {
   const int* act = all ? nullptr : b.active;
-  const int N = b.max_n;
-  if (N <= 384)
-    k_tridiag<6><<<b.n_mol, kT, tridiag_lds(N), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act,
-                                                     b.status, b.order);
-  else
-    k_tridiag<20><<<b.n_mol, kT, tridiag_lds(N), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act,
-                                                      b.status, b.order);
+  const Part p = small_part(b);
+  if (p.count > 0) {
+    if (p.max_n <= 384)
+      k_tridiag<6><<<p.count, kT, tridiag_lds(p.max_n), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau,
+                                                             act, b.status, p.order);
+    else
+      k_tridiag<20><<<p.count, kT, tridiag_lds(p.max_n), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau,
+                                                              act, b.status, p.order);
+  }
+  for (int i = 0; i < b.n_large; ++i) launch_tridiag_large(b, s, b.large_idx[i], b.large_mols[i], act);
 }
 
-void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all) {
+static void bisect_part(const BatchDev& b, hipStream_t s, bool all, const int* order, int count,
+                        int N) {
   // two eigenvalues per lane: (k+1)/2 lanes, k <= N/2 on the SCF path
-  const int N = b.max_n;
   const int half = ((all ? N : N / 2) + 1) / 2;
   const int threads = min(256, max(64, (half + 63) / 64 * 64));
-  dim3 gb((half + threads - 1) / threads, b.n_mol);
+  dim3 gb((half + threads - 1) / threads, count);
   k_bisect<<<gb, threads, sizeof(double) * (2 * N + 32), s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm,
                                                           all ? 1 : 0, all ? nullptr : b.active,
-                                                          b.status, b.order,
+                                                          b.status, order,
                                                           all ? nullptr : b.steps);
 }
 
+void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all) {
+  const Part p = small_part(b);
+  if (p.count > 0) bisect_part(b, s, all, p.order, p.count, p.max_n);
+  if (b.n_large > 0) bisect_part(b, s, all, b.order, b.n_large, b.max_n);
+}
+
 void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all) {
-  const int N = b.max_n;
-  const size_t inv_lds = sizeof(double) * (3 * (size_t)N) + sizeof(int) * (2 * (size_t)N + 2);
-  // one lane per vector: only as many waves as the widest selection needs
-  const int kmax = all ? N : N / 2;
-  const int threads = min(kT, max(64, (kmax + 63) / 64 * 64));
-  k_invit<<<b.n_mol, threads, inv_lds, s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm, b.W, all ? 1 : 0,
-                                            all ? nullptr : b.active, b.status, b.order);
+  const int* act = all ? nullptr : b.active;
+  const Part p = small_part(b);
+  if (p.count > 0) {
+    const int N = p.max_n;
+    const size_t inv_lds = sizeof(double) * (3 * (size_t)N) + sizeof(int) * (2 * (size_t)N + 2);
+    // one lane per vector: only as many waves as the widest selection needs
+    const int kmax = all ? N : N / 2;
+    const int threads = min(kT, max(64, (kmax + 63) / 64 * 64));
+    k_invit<<<p.count, threads, inv_lds, s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm, b.W, all ? 1 : 0,
+                                              act, b.status, p.order);
+  }
+  for (int i = 0; i < b.n_large; ++i) {
+    const int m = b.large_idx[i];
+    const MolDesc& M = b.large_mols[i];
+    int lo, k;
+    selection(M, all ? 1 : 0, lo, k);
+    if (k == 0) continue;
+    const int L = LargeWork::lanes(M.n_ao, k);
+    k_li_setup<<<1, 64, 0, s>>>(b.mols, b.eig, b.tnorm, b.W, all ? 1 : 0, act, b.status, m);
+    for (int sweep = 0; sweep < 3; ++sweep) {
+      k_li_solve<<<(L + 63) / 64, 64, 0, s>>>(b.mols, b.dT, b.eT, b.tnorm, b.W, all ? 1 : 0, act,
+                                              b.status, m, sweep);
+      k_li_mgs<<<min(1024, (k + 3) / 4), 256, 0, s>>>(b.mols, b.W, all ? 1 : 0, act, b.status, m);
+    }
+  }
 }
 
 void launch_eigh_back(const BatchDev& b, hipStream_t s, bool all) {
   const int* act = all ? nullptr : b.active;
-  // 80 KB (two workgroups per CU) unless even groups of 4 reflectors need more
-  const int N = b.max_n;
-  const int kmax = all ? N : N / 2 + 1;
-  const int bt_doubles = 4 * (N + kmax) + 2560 > 10240 ? 20480 : 10240;
-  k_backtransform<<<b.n_mol, kT, sizeof(double) * bt_doubles, s>>>(
-      b.mols, b.W, b.tau, all ? 1 : 0, act, b.status, b.order, bt_doubles);
+  const Part p = small_part(b);
+  if (p.count > 0) {
+    // 80 KB (two workgroups per CU) unless even groups of 4 reflectors need more
+    const int N = p.max_n;
+    const int kmax = all ? N : N / 2 + 1;
+    const int bt_doubles = 4 * (N + kmax) + 2560 > 10240 ? 20480 : 10240;
+    k_backtransform<<<p.count, kT, sizeof(double) * bt_doubles, s>>>(
+        b.mols, b.W, b.tau, all ? 1 : 0, act, b.status, p.order, bt_doubles);
+  }
+  for (int i = 0; i < b.n_large; ++i) launch_back_large(b, s, b.large_idx[i], b.large_mols[i], act, all);
   if (all) {
     dim3 gw(64, b.n_mol);
     k_write_eigvec<<<gw, 256, 0, s>>>(b.mols, b.W, b.C, act, b.status);
@@ -1140,15 +1241,21 @@ void launch_eigh(const BatchDev& b, hipStream_t s, bool all) {
   launch_eigh_back(b, s, all);
 }
 
+static void density_part(const BatchDev& b, hipStream_t s, const int* order, int count, int N) {
+  // as many workgroups per molecule as the largest one has 4-block groups
+  const int t32 = (N + 31) >> 5;
+  dim3 grid((t32 * (t32 + 1) / 2 + 3) / 4, count);
+  k_density<<<grid, 256, 0, s>>>(b.mols, b.W, b.P, b.active, b.status, b.prec, order);
+}
+
 void launch_density(const BatchDev& b, hipStream_t s, bool all) {
   if (all) {
     dim3 grid(64, b.n_mol);
     k_density_from_c<<<grid, 256, 0, s>>>(b.mols, b.C, b.P, 64);
   } else {
-    // as many workgroups per molecule as the largest one has 4-block groups
-    const int t32 = (b.max_n + 31) >> 5;
-    dim3 grid((t32 * (t32 + 1) / 2 + 3) / 4, b.n_mol);
-    k_density<<<grid, 256, 0, s>>>(b.mols, b.W, b.P, b.active, b.status, b.prec, b.order);
+    const Part p = small_part(b);
+    if (p.count > 0) density_part(b, s, p.order, p.count, p.max_n);
+    if (b.n_large > 0) density_part(b, s, b.order, b.n_large, b.max_n);
   }
 }
 

@@ -1,0 +1,148 @@
+"""Multi-workgroup eigensolver: molecules too large for one workgroup.
+
+The same molecules go through both eigensolvers (an engine whose large-basis
+threshold is lowered to 128 basis functions routes them to the
+multi-workgroup one), so the new path is held to the one-workgroup path,
+which is itself held to the reference.  Molecules above the one-workgroup
+limit (N > 1280) are checked through what fixes the result: eigenpairs
+against numpy (residual, orthonormality), the density's idempotency and
+trace, and the SCF energy against the oracle's for the same cluster, written
+once to tests/golden/water_250_oracle.json by tests/golden/make_oracle_large.py.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from semigpu_amd import api, geometry
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def large_engine():
+    e = api.Engine()
+    e.set_large_basis(128)
+    yield e
+    e.close()
+
+
+def packed_to_full(p, n):
+    a = np.zeros((n, n))
+    r, c = np.tril_indices(n)
+    a[r, c] = p
+    a[c, r] = p
+    return a
+
+
+def fock_of_guess(eng, mol):
+    b = eng.batch([mol])
+    b.two_center()
+    b.hcore()
+    b.guess_density()
+    b.fock()
+    return b
+
+
+def check_eigh(b, tag):
+    n = b.n_ao(0)
+    F = packed_to_full(b.packed(0, api.SG_FOCK), n)
+    b.eigh()
+    w, C = b.eigen(0)
+    wr = np.linalg.eigh(F)[0]
+    scale = np.abs(wr).max()
+    ew = np.abs(w - wr).max() / scale
+    eo = np.abs(C.T @ C - np.eye(n)).max()
+    er = np.abs(F @ C - C * w).max() / scale
+    say(tag, "N", n, "eig", ew, "orth", eo, "resid", er)
+    assert ew <= 1e-12
+    assert eo <= 1e-12
+    assert er <= 1e-11
+
+
+def test_threshold_bounds(engine):
+    with pytest.raises(api.SemiGpuError, match="below 128"):
+        engine.set_large_basis(64)
+
+
+@pytest.mark.parametrize("n_water", [40, 70])
+def test_eigh_large_path_against_numpy(large_engine, n_water):
+    """N = 240 and 420 on the multi-workgroup solver: every eigenpair."""
+    say("case eigh large-path waters", n_water)
+    mol = geometry.water_cluster(n_water, geometry.SplitMix64(5 + n_water))
+    b = fock_of_guess(large_engine, mol)
+    check_eigh(b, "large-path")
+    b.close()
+
+
+def test_scf_large_path_equals_one_workgroup(engine, large_engine):
+    """SCF energies of the two solvers agree to rounding; molecules below the
+    threshold (pyridine, N = 29) keep the one-workgroup path bit for bit."""
+    mols = [geometry.water_cluster(40, geometry.SplitMix64(1)), geometry.alkane(40),
+            geometry.pyridine(), geometry.water_cluster(70, geometry.SplitMix64(2))]
+    say("case scf both solvers", [geometry.n_basis(m) for m in mols])
+    ref = engine.batch(mols).scf(threshold=1e-6)
+    big = large_engine.batch(mols).scf(threshold=1e-6)
+    for m, a, c in zip(mols, ref, big):
+        say("N", geometry.n_basis(m), a["steps"], c["steps"], c["e_tot"] - a["e_tot"])
+        assert c["converged"] and c["status"] == 0
+        assert c["steps"] == a["steps"]
+        assert abs(c["e_tot"] - a["e_tot"]) <= 1e-7
+    assert big[2] == ref[2]
+
+
+def test_large_path_batch_equals_alone(large_engine):
+    """Large and small molecules in one batch give their results alone, bit for bit."""
+    mols = geometry.water_batch(6, seed=3, n_min=2, n_max=10)
+    mols.insert(2, geometry.water_cluster(30, geometry.SplitMix64(9)))   # N = 180
+    mols.append(geometry.water_cluster(25, geometry.SplitMix64(10)))     # N = 150
+    say("case mixed sizes", [geometry.n_basis(m) for m in mols])
+    res = large_engine.batch(mols).scf(threshold=1e-6)
+    for i in (0, 2, len(mols) - 1):
+        alone = large_engine.batch([mols[i]]).scf(threshold=1e-6)[0]
+        assert alone == res[i], i
+
+
+def test_mixed_policy_large_path(large_engine):
+    mol = geometry.water_cluster(50, geometry.SplitMix64(12))
+    say("case mixed policy large path N", geometry.n_basis(mol))
+    b = large_engine.batch([mol])
+    a = b.scf(threshold=1e-6, policy="fp64")[0]
+    c = b.scf(threshold=1e-6, policy="mixed")[0]
+    say("fp64", a["e_tot"], "mixed", c["e_tot"])
+    assert c["converged"] and abs(c["e_tot"] - a["e_tot"]) <= 1e-4
+    b.close()
+
+
+def test_above_one_workgroup_limit(engine):
+    """N = 1500 (250 waters): eigenpairs, density and SCF energy."""
+    mol = geometry.water_cluster(250, geometry.SplitMix64(31))
+    say("case N=1500 eigh")
+    b = fock_of_guess(engine, mol)
+    n = b.n_ao(0)
+    assert n == 1500
+    check_eigh(b, "N=1500")
+    b.close()
+    say("case N=1500 scf")
+    b = engine.batch([mol])
+    r = b.scf(threshold=1e-6)[0]
+    P = packed_to_full(b.packed(0, api.SG_DENSITY), n)
+    n_occ = b.info[0][2]
+    idem = np.abs(P @ P - 2.0 * P).max()
+    say("N=1500 steps", r["steps"], "e_tot", r["e_tot"], "idempotency", idem)
+    assert r["converged"] and r["status"] == 0
+    assert abs(np.trace(P) - 2 * n_occ) <= 1e-9
+    assert idem <= 1e-9
+    with open(os.path.join(HERE, "golden", "water_250_oracle.json")) as fh:
+        g = json.load(fh)
+    assert r["steps"] == g["steps"]
+    assert abs(r["e_tot"] - g["e_tot"]) <= 1e-6
+    b.close()
