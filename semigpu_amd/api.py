@@ -10,6 +10,7 @@ otherwise.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 import weakref
@@ -62,6 +63,15 @@ class SemiGpuError(RuntimeError):
 
 
 _lib = None
+_engines = weakref.WeakSet()
+
+
+@atexit.register
+def _close_engines():
+    # Engines (and their batches and servers) still open at interpreter exit
+    # are destroyed before module teardown, while the HIP runtime is intact.
+    for e in list(_engines):
+        e.close()
 
 
 def load_library(path: str = LIB_PATH):
@@ -138,6 +148,7 @@ class Engine:
         self.L = L
         self._batches = weakref.WeakSet()
         self._servers = weakref.WeakSet()
+        _engines.add(self)
 
     def close(self):
         """Destroys the context; batches still alive are destroyed first."""

@@ -21,6 +21,8 @@
 
 namespace sg {
 
+constexpr int kEnergyChunk = 16384;  // entries of a large molecule's E_el per workgroup
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -219,8 +221,9 @@ __global__ void __launch_bounds__(256) k_energy(const MolDesc* __restrict__ mols
                                                  const double* __restrict__ P,
                                                  double* __restrict__ out,
                                                  const int* __restrict__ active,
-                                                 const int* __restrict__ status) {
-  const int m = blockIdx.x;
+                                                 const int* __restrict__ status,
+                                                 const int* __restrict__ order) {
+  const int m = order ? order[blockIdx.x] : blockIdx.x;
   if (active && (!active[m] || status[m])) return;
   const MolDesc M = mols[m];
   const int64_t nn = (int64_t)M.n_ao * M.n_ao;
@@ -229,6 +232,40 @@ __global__ void __launch_bounds__(256) k_energy(const MolDesc* __restrict__ mols
   const double* p = P + M.mat0;
   double s = 0.0;
   for (int64_t t = threadIdx.x; t < nn; t += blockDim.x) s += p[t] * (h[t] + f[t]);
+  __shared__ double red[4];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[m] = 0.5 * ((red[0] + red[1]) + (red[2] + red[3]));
+}
+
+// E_el of one large molecule: workgroup g sums entries [g C, (g+1) C) into
+// part[g]; k_energy_total adds the parts in order.
+__global__ void __launch_bounds__(256) k_energy_chunks(const double* __restrict__ h,
+                                                        const double* __restrict__ f,
+                                                        const double* __restrict__ p, int64_t nn,
+                                                        double* __restrict__ part,
+                                                        const int* __restrict__ active,
+                                                        const int* __restrict__ status, int m) {
+  if (active && (!active[m] || status[m])) return;
+  const int64_t t0 = (int64_t)blockIdx.x * kEnergyChunk;
+  const int64_t t1 = t0 + kEnergyChunk < nn ? t0 + kEnergyChunk : nn;
+  double s = 0.0;
+  for (int64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) s += p[t] * (h[t] + f[t]);
+  __shared__ double red[4];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void __launch_bounds__(256) k_energy_total(const double* __restrict__ part, int chunks,
+                                                       double* __restrict__ out,
+                                                       const int* __restrict__ active,
+                                                       const int* __restrict__ status, int m) {
+  if (active && (!active[m] || status[m])) return;
+  double s = 0.0;
+  for (int g = threadIdx.x; g < chunks; g += blockDim.x) s += part[g];
   __shared__ double red[4];
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
@@ -333,8 +370,24 @@ void launch_blocks_to_fp32(const BatchDev& b, hipStream_t s) {
 }
 
 void launch_energy(const BatchDev& b, hipStream_t s, double* out, bool all, const int* mask) {
-  k_energy<<<b.n_mol, 256, 0, s>>>(b.mols, b.H, b.F, b.P, out,
-                                   mask ? mask : (all ? nullptr : b.active), b.status);
+  const int* act = mask ? mask : (all ? nullptr : b.active);
+  const int n_small = b.n_mol - b.n_large;
+  if (n_small > 0)
+    k_energy<<<n_small, 256, 0, s>>>(b.mols, b.H, b.F, b.P, out, act, b.status,
+                                     b.n_large ? b.order + b.n_large : nullptr);
+  // molecules on the multi-workgroup eigensolver: chunks of kEnergyChunk
+  // entries per workgroup, partial sums in the molecule's workspace (its T
+  // factors are dead by now), then one ordered sum
+  for (int i = 0; i < b.n_large; ++i) {
+    const MolDesc& M = b.large_mols[i];
+    const int m = b.large_idx[i];
+    const int64_t nn = (int64_t)M.n_ao * M.n_ao;
+    const int chunks = (int)((nn + kEnergyChunk - 1) / kEnergyChunk);
+    LargeWork L(b.W, M);
+    k_energy_chunks<<<chunks, 256, 0, s>>>(b.H + M.mat0, b.F + M.mat0, b.P + M.mat0, nn, L.T, act,
+                                           b.status, m);
+    k_energy_total<<<1, 256, 0, s>>>(L.T, chunks, out, act, b.status, m);
+  }
 }
 
 void launch_scf_init(const BatchDev& b, hipStream_t s, int max_steps, int policy,

@@ -22,11 +22,14 @@
 //   k_hcore_diag                        one lane per (atom, AO pair): U + core
 //        attraction summed over the other atoms in atom order (hcore.cpp:71-99).
 //   k_core_core                         one lane per molecule, pairs in the
-//        reference order (corecorerepulsion.cpp:12-65).
+//        reference order (corecorerepulsion.cpp:12-65); molecules of 512 atoms
+//        or more one workgroup each (k_core_core_wide).
 #include "sg_common.h"
 #include "sg_kernels.h"
 
 namespace sg {
+
+constexpr int kWideAtoms = 512;  // molecules from this size take k_core_core_wide
 
 // ---------------------------------------------------------------------------
 // Multipole interactions in Hartree; (R, element of first pair, element of
@@ -681,6 +684,25 @@ __global__ void k_hcore_diag(const int2* __restrict__ tasks, int n,
 }
 
 // Core-core repulsion per molecule (CoreCoreRepulsion::ComputeRepulsion).
+// Core-core term of atoms i < j (corecorerepulsion.cpp:12-65).
+__device__ __forceinline__ double core_pair(const AtomDesc& A, const AtomDesc& B,
+                                            const double* __restrict__ blk) {
+  double t = (double)(core_charge(A.zn) * core_charge(B.zn));
+  t *= blk[block_offset(B, A)];
+  double r = dist3(A.x, A.y, A.z, B.x, B.y, B.z);
+  r = r * kBohrToAng;
+  double f;
+  if (B.zn == 1 && (A.zn == 7 || A.zn == 8))
+    f = 1.0 + r * exp(-c_par.alpha[A.zn] * r) + exp(-c_par.alpha[B.zn] * r);
+  else if (A.zn == 1 && (B.zn == 7 || B.zn == 8))
+    f = 1.0 + r * exp(-c_par.alpha[B.zn] * r) + exp(-c_par.alpha[A.zn] * r);
+  else
+    f = 1.0 + exp(-c_par.alpha[A.zn] * r) + exp(-c_par.alpha[B.zn] * r);
+  return t * f;
+}
+
+// Molecules below kWideAtoms atoms: one lane each, pairs summed in the
+// reference order.
 __global__ void k_core_core(int n_mol, const AtomDesc* __restrict__ atoms,
                             const MolDesc* __restrict__ mols, const double* __restrict__ blocks,
                             double* __restrict__ e_cc, const int* __restrict__ mask) {
@@ -688,28 +710,41 @@ __global__ void k_core_core(int n_mol, const AtomDesc* __restrict__ atoms,
   if (m >= n_mol) return;
   if (mask && !mask[m]) return;
   const MolDesc M = mols[m];
+  if (M.n_atoms >= kWideAtoms) return;
   const double* blk = blocks + M.blk0;
   double e = 0.0;
   for (int i = 0; i < M.n_atoms; ++i) {
     const AtomDesc A = atoms[M.atom0 + i];
-    for (int j = i + 1; j < M.n_atoms; ++j) {
-      const AtomDesc B = atoms[M.atom0 + j];
-      double t = (double)(core_charge(A.zn) * core_charge(B.zn));
-      t *= blk[block_offset(B, A)];
-      double r = dist3(A.x, A.y, A.z, B.x, B.y, B.z);
-      r = r * kBohrToAng;
-      double f;
-      if (B.zn == 1 && (A.zn == 7 || A.zn == 8))
-        f = 1.0 + r * exp(-c_par.alpha[A.zn] * r) + exp(-c_par.alpha[B.zn] * r);
-      else if (A.zn == 1 && (B.zn == 7 || B.zn == 8))
-        f = 1.0 + r * exp(-c_par.alpha[B.zn] * r) + exp(-c_par.alpha[A.zn] * r);
-      else
-        f = 1.0 + exp(-c_par.alpha[A.zn] * r) + exp(-c_par.alpha[B.zn] * r);
-      t *= f;
-      e += t;
-    }
+    for (int j = i + 1; j < M.n_atoms; ++j) e += core_pair(A, atoms[M.atom0 + j], blk);
   }
   e_cc[m] = e;
+}
+
+// Molecules of kWideAtoms atoms or more: one workgroup each, lane t sums the
+// rows i = t, t + 256, ... in order, and the lane sums are added in a fixed order.
+__global__ void __launch_bounds__(256) k_core_core_wide(const AtomDesc* __restrict__ atoms,
+                                                         const MolDesc* __restrict__ mols,
+                                                         const double* __restrict__ blocks,
+                                                         double* __restrict__ e_cc,
+                                                         const int* __restrict__ mask) {
+  const int m = blockIdx.x;
+  if (mask && !mask[m]) return;
+  const MolDesc M = mols[m];
+  if (M.n_atoms < kWideAtoms) return;
+  const double* blk = blocks + M.blk0;
+  double e = 0.0;
+  for (int i = threadIdx.x; i < M.n_atoms; i += blockDim.x) {
+    const AtomDesc A = atoms[M.atom0 + i];
+    for (int j = i + 1; j < M.n_atoms; ++j) e += core_pair(A, atoms[M.atom0 + j], blk);
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = e;
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) e_cc[m] = red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -748,6 +783,8 @@ void launch_hcore(const BatchDev& b, hipStream_t s) {
 void launch_core_core(const BatchDev& b, hipStream_t s, const int* mask) {
   k_core_core<<<blocks_for(b.n_mol, 64), 64, 0, s>>>(b.n_mol, b.atoms, b.mols, b.blocks, b.e_cc,
                                                      mask);
+  if (b.max_n >= kWideAtoms)  // N >= n_atoms: no molecule is wide otherwise
+    k_core_core_wide<<<b.n_mol, 256, 0, s>>>(b.atoms, b.mols, b.blocks, b.e_cc, mask);
 }
 
 }  // namespace sg

@@ -6,8 +6,10 @@ multi-workgroup one), so the new path is held to the one-workgroup path,
 which is itself held to the reference.  Molecules above the one-workgroup
 limit (N > 1280) are checked through what fixes the result: eigenpairs
 against numpy (residual, orthonormality), the density's idempotency and
-trace, and the SCF energy against the oracle's for the same cluster, written
-once to tests/golden/water_250_oracle.json by tests/golden/make_oracle_large.py.
+trace, the integrals, Hcore, Fock and energy of the converged density against
+the oracle's, and the SCF energy against the oracle's SCF of the same cluster,
+written once to tests/golden/water_250_oracle.json by
+tests/golden/make_oracle_large.py.
 """
 import json
 import os
@@ -123,7 +125,13 @@ def test_mixed_policy_large_path(large_engine):
 
 
 def test_above_one_workgroup_limit(engine):
-    """N = 1500 (250 waters): eigenpairs, density and SCF energy."""
+    """N = 1500 (250 waters): eigenpairs, density, each SCF ingredient against
+    the oracle, and the SCF energy against the oracle's SCF.
+
+    E_el (-2.9e6 eV) and E_cc (+2.9e6 eV) cancel to E_tot = -8.8e4 eV, so two
+    FP64 evaluations that sum in different orders differ by ~1e-12 |E_el|;
+    the energies are held to that (3e-6 eV here), the matrices to 1e-10."""
+    import pyoracle as po
     mol = geometry.water_cluster(250, geometry.SplitMix64(31))
     say("case N=1500 eigh")
     b = fock_of_guess(engine, mol)
@@ -134,15 +142,31 @@ def test_above_one_workgroup_limit(engine):
     say("case N=1500 scf")
     b = engine.batch([mol])
     r = b.scf(threshold=1e-6)[0]
-    P = packed_to_full(b.packed(0, api.SG_DENSITY), n)
+    P = b.packed(0, api.SG_DENSITY)
+    H = b.packed(0, api.SG_HCORE)
+    F = b.packed(0, api.SG_FOCK)
+    Pf = packed_to_full(P, n)
     n_occ = b.info[0][2]
-    idem = np.abs(P @ P - 2.0 * P).max()
-    say("N=1500 steps", r["steps"], "e_tot", r["e_tot"], "idempotency", idem)
+    idem = np.abs(Pf @ Pf - 2.0 * Pf).max()
+    say("N=1500 steps", r["steps"], "e_el", r["e_el"], "e_cc", r["e_cc"], "idempotency", idem)
     assert r["converged"] and r["status"] == 0
-    assert abs(np.trace(P) - 2 * n_occ) <= 1e-9
+    assert abs(np.trace(Pf) - 2 * n_occ) <= 1e-9
     assert idem <= 1e-9
+    om = po.Molecule(mol)
+    blocks = om.pair_blocks()
+    assert np.array_equal(b.pair_blocks(0), blocks)
+    h_or = om.hcore(blocks)
+    assert np.abs(H - h_or).max() <= 1e-10 * np.abs(h_or).max()
+    f_or = om.fock(blocks, h_or, P)
+    assert np.abs(F - f_or).max() <= 1e-10 * np.abs(f_or).max()
+    e_el_or = po.energy(n, h_or, f_or, P)
+    tol = 1e-12 * abs(r["e_el"])
+    say("E_el vs oracle energy of the same P", r["e_el"] - e_el_or, "tol", tol)
+    assert abs(r["e_el"] - e_el_or) <= tol
+    assert abs(r["e_cc"] - om.core_core(blocks)) <= 1e-12 * abs(r["e_cc"])
     with open(os.path.join(HERE, "golden", "water_250_oracle.json")) as fh:
         g = json.load(fh)
+    say("E_tot vs oracle SCF", r["e_tot"] - g["e_tot"])
     assert r["steps"] == g["steps"]
-    assert abs(r["e_tot"] - g["e_tot"]) <= 1e-6
+    assert abs(r["e_tot"] - g["e_tot"]) <= tol
     b.close()
