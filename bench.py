@@ -116,12 +116,18 @@ def cpu_baseline(mols, steps, budget_s: float = 25.0):
     }
 
 
+# basis-size bounds of the server's size classes (each its own stream and pace).
+# Two classes plus the context stream stay within the process's 4 hardware
+# queues, so no class shares a queue with another (which would serialise them).
+SERVE_CLASSES = (400,)
+
+
 def serve(eng, rate: float, seconds: float, threshold: float, policy: str, seed: int = 7,
           limit_s: float = 90.0):
     """Poisson arrivals of alkanes C_nH_2n+2 (n <= 60) and water clusters (n <= 200)."""
     reqs = geometry.serving_stream(rate, seconds, seed)
     srv = eng.server(hbm_bytes=64e9, max_slots=16384, max_atoms=1 << 21, threshold=threshold,
-                     max_steps=100, policy=policy)
+                     max_steps=100, policy=policy, classes=SERVE_CLASSES)
     done = []
     i = 0
     t0 = srv.clock()
@@ -144,11 +150,19 @@ def serve(eng, rate: float, seconds: float, threshold: float, policy: str, seed:
     lat = np.array([r["t_done"] - t0 - reqs[r["id"]][0] for r in done])
     t_end = max(r["t_done"] for r in done) - t0 if done else 0.0
     conv = sum(1 for r in done if r["converged"])
+    edges = (0,) + tuple(SERVE_CLASSES) + (1 << 30,)
+    by_class = []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        lc = np.array([r["t_done"] - t0 - reqs[r["id"]][0] for r in done if lo <= r["n_ao"] < hi])
+        by_class.append({"n_ao": [lo, min(hi, 1280)], "completed": int(len(lc)),
+                         "p50_ms": round(float(np.percentile(lc, 50)) * 1e3, 2) if len(lc) else None,
+                         "p99_ms": round(float(np.percentile(lc, 99)) * 1e3, 2) if len(lc) else None})
     return {"rate_per_s": rate, "window_s": seconds, "requests": len(reqs), "completed": len(done),
             "converged": conv, "server_steps": steps,
             "value": round(len(done) / max(t_end, 1e-9), 3), "unit": "molecules/s",
             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2) if len(lat) else None,
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2) if len(lat) else None,
+            "size_classes": list(SERVE_CLASSES), "by_class": by_class,
             "basis": "arrival time to result on the server clock; alkanes n~U{1..60} / water "
                      "clusters n~U{2..200} 50/50, seed %d" % seed}
 

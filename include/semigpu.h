@@ -132,14 +132,20 @@ int sg_set_eigen(sg_batch* b, int mol, const double* eigval, const double* eigve
 
 /* Continuous-batching SCF server (no counterpart in the reference, whose main()
  * runs one molecule per process, semiGPU.cc:30-110).  Requests (one molecule
- * each) queue in arrival order; every sg_server_step() admits queued molecules
- * while their HBM footprint fits the server's pools (first fit) and a slot is
- * free, sets them up (integrals, Hcore, core-core, guess, E_0), runs ONE SCF
- * iteration for every molecule in flight, and retires the molecules that
- * converged, reached max_steps or failed.  State stays in HBM between steps.
- * A molecule's result does not depend on what else is in flight.
- *   hbm_bytes  device memory the server may hold (one allocation)
+ * each) go to the size class of their basis size N and queue there in arrival
+ * order.  Each class has its own pools, stream and SCF iterations, so that
+ * small molecules are not paced by large ones; their kernels overlap on the GPU.
+ * sg_server_step() visits every class whose last iteration has finished: it
+ * retires the molecules that converged, reached max_steps or failed, admits
+ * queued molecules while their HBM footprint fits the class's pools (first
+ * fit) and a slot is free, sets them up (integrals, Hcore, core-core, guess,
+ * E_0) and launches ONE SCF iteration for every molecule in flight in the
+ * class.  State stays in HBM between steps.  A molecule's result does not
+ * depend on what else is in flight.
+ *   hbm_bytes  device memory the server may hold (one pool for H, F, P, C,
+ *              workspace, pair blocks and vectors, shared by the classes)
  *   max_slots  molecules in flight at most;  max_atoms  atoms in flight at most
+ *              (both split evenly between the classes)
  *   policy     SG_POLICY_FP64 / SG_POLICY_MIXED (switch_ev as for sg_scf_run_policy)
  * Times are seconds on the server's clock (sg_server_clock, 0 at creation). */
 typedef struct sg_server sg_server;
@@ -153,13 +159,23 @@ typedef struct {
 int sg_server_create(sg_context* ctx, double hbm_bytes, int max_slots, int max_atoms,
                      double threshold_ev, int max_steps, int policy, double switch_ev,
                      sg_server** out);
+/* Same with n_classes size classes: class i takes N in [bounds[i-1], bounds[i])
+ * (bounds[-1] = 0, the last class up to the largest N the server accepts, 1280);
+ * bounds[] has n_classes - 1 increasing entries.  sg_server_create is one class.
+ * Each class has its own stream; HIP maps streams onto a few hardware queues
+ * per process (4 by default) and streams that share one run in sequence, so
+ * classes beyond the free queues lose their independence. */
+int sg_server_create_classes(sg_context* ctx, double hbm_bytes, int max_slots, int max_atoms,
+                             double threshold_ev, int max_steps, int policy, double switch_ev,
+                             int n_classes, const int* bounds, sg_server** out);
 int sg_server_destroy(sg_server* srv);
 /* Queues one molecule (Angstrom); rejects elements outside H, C, N, O and
  * molecules larger than the server's pools. */
 int sg_server_submit(sg_server* srv, long long request_id, int n_atoms, const int* Z,
                      const double* xyz_ang);
-/* One admission + SCF iteration + retirement round; reports the molecules in
- * flight and still queued afterwards. */
+/* One retirement + admission + SCF-iteration round for every class that is
+ * ready (does not wait for the others; sleeps ~100 us when no class was ready);
+ * reports the molecules in flight and still queued afterwards. */
 int sg_server_step(sg_server* srv, int* n_inflight, int* n_queued);
 /* Moves up to max_out finished results (completion order) into out[];
  * returns how many, or a negative error code. */

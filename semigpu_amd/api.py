@@ -37,7 +37,7 @@ EXPORTS = (
     "sg_get_eigen", "sg_set_eigen", "sg_batch_create_bohr", "sg_set_pair_blocks",
     "sg_get_status", "sg_calibration_copy", "sg_timing_enable", "sg_timing_read", "sg_timing_reset",
     "sg_server_create", "sg_server_destroy", "sg_server_submit", "sg_server_step",
-    "sg_server_poll", "sg_server_clock", "sg_ctx_set_large_basis",
+    "sg_server_poll", "sg_server_clock", "sg_ctx_set_large_basis", "sg_server_create_classes",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -114,6 +114,10 @@ def load_library(path: str = LIB_PATH):
     L.sg_server_create.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                    ctypes.POINTER(_vp)]
+    L.sg_server_create_classes.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_double, ctypes.c_int, _ip,
+                                           ctypes.POINTER(_vp)]
     L.sg_server_destroy.argtypes = [_vp]
     L.sg_server_submit.argtypes = [_vp, ctypes.c_longlong, ctypes.c_int, _ip, _dp]
     L.sg_server_step.argtypes = [_vp, _ip, _ip]
@@ -306,13 +310,16 @@ class Server:
 
     def __init__(self, engine: Engine, hbm_bytes: float = 8e9, max_slots: int = 4096,
                  max_atoms: int = 1 << 18, threshold: float = 5e-3, max_steps: int = 100,
-                 policy: str = "fp64", switch_ev: float = 1e-3):
+                 policy: str = "fp64", switch_ev: float = 1e-3, classes: Sequence[int] = ()):
+        """classes: increasing basis-size bounds of the size classes (empty: one class)."""
         self.engine = engine
         self.L = engine.L
         h = _vp()
         pol = {"fp64": 0, "mixed": 1}[policy]
-        _check(self.L.sg_server_create(engine.h, float(hbm_bytes), max_slots, max_atoms,
-                                       threshold, max_steps, pol, switch_ev, ctypes.byref(h)))
+        bounds = np.asarray(list(classes), dtype=np.int32)
+        _check(self.L.sg_server_create_classes(engine.h, float(hbm_bytes), max_slots, max_atoms,
+                                               threshold, max_steps, pol, switch_ev,
+                                               len(bounds) + 1, _i(bounds), ctypes.byref(h)))
         self.h = h
         engine._servers.add(self)
 
@@ -333,7 +340,8 @@ class Server:
         _check(self.L.sg_server_submit(self.h, request_id, len(Z), _i(Z), _d(xyz)))
 
     def step(self):
-        """One admission + iteration + retirement round -> (in flight, queued)."""
+        """One retirement + admission + iteration round of every ready size class
+        -> (in flight, queued)."""
         n_in, n_q = ctypes.c_int(), ctypes.c_int()
         _check(self.L.sg_server_step(self.h, ctypes.byref(n_in), ctypes.byref(n_q)))
         return n_in.value, n_q.value
@@ -351,10 +359,11 @@ class Server:
     def clock(self) -> float:
         return self.L.sg_server_clock(self.h)
 
-    def drain(self, max_iters: int = 100000):
+    def drain(self, max_seconds: float = 600.0):
         """Steps until nothing is queued or in flight; returns every result."""
         out = []
-        for _ in range(max_iters):
+        t_end = self.clock() + max_seconds
+        while self.clock() < t_end:
             n_in, n_q = self.step()
             out.extend(self.poll())
             if n_in == 0 and n_q == 0:

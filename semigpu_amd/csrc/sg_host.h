@@ -43,21 +43,24 @@ struct sg_context {
 
 namespace sg {
 
-// Records HIP events around one stage on the context stream when timing is on.
+// Records HIP events around one stage on stream s (default: the context's)
+// when timing is on.
 struct StageTimer {
   sg_context* c;
   int stage;
+  hipStream_t s;
   hipEvent_t a = nullptr, b = nullptr;
-  StageTimer(sg_context* ctx, int st) : c(ctx), stage(st) {
+  StageTimer(sg_context* ctx, int st, hipStream_t stream = nullptr)
+      : c(ctx), stage(st), s(stream ? stream : ctx->stream) {
     if (c->timing) {
       (void)hipEventCreate(&a);
       (void)hipEventCreate(&b);
-      (void)hipEventRecord(a, c->stream);
+      (void)hipEventRecord(a, s);
     }
   }
   ~StageTimer() {
     if (c->timing) {
-      (void)hipEventRecord(b, c->stream);
+      (void)hipEventRecord(b, s);
       c->pending.push_back({stage, {a, b}});
     }
   }
@@ -71,19 +74,19 @@ inline void scf_iteration(sg_context* c, const BatchDev& d, hipStream_t s, doubl
 // The eigensolver's four kernels, each timed as its own stage.
 inline void timed_eigh(sg_context* c, const BatchDev& d, hipStream_t s, bool all) {
   {
-    StageTimer t(c, ST_TRIDIAG);
+    StageTimer t(c, ST_TRIDIAG, s);
     launch_eigh_tridiag(d, s, all);
   }
   {
-    StageTimer t(c, ST_BISECT);
+    StageTimer t(c, ST_BISECT, s);
     launch_eigh_bisect(d, s, all);
   }
   {
-    StageTimer t(c, ST_INVIT);
+    StageTimer t(c, ST_INVIT, s);
     launch_eigh_invit(d, s, all);
   }
   {
-    StageTimer t(c, ST_BACK);
+    StageTimer t(c, ST_BACK, s);
     launch_eigh_back(d, s, all);
   }
 }
@@ -92,19 +95,19 @@ inline void scf_iteration(sg_context* c, const BatchDev& d, hipStream_t s, doubl
                           int max_steps, double switch_ev, int fp32_cap) {
   timed_eigh(c, d, s, false);
   {
-    StageTimer t(c, ST_DENSITY);
+    StageTimer t(c, ST_DENSITY, s);
     launch_density(d, s, false);
   }
   {
-    StageTimer t(c, ST_FOCK);
+    StageTimer t(c, ST_FOCK, s);
     launch_fock(d, s, false);
   }
   {
-    StageTimer t(c, ST_ENERGY);
+    StageTimer t(c, ST_ENERGY, s);
     launch_energy(d, s, d.e_new, false);
   }
   {
-    StageTimer t(c, ST_CONTROL);
+    StageTimer t(c, ST_CONTROL, s);
     launch_reset_count(d, s);
     launch_scf_update(d, s, threshold, max_steps, switch_ev, fp32_cap);
   }

@@ -97,3 +97,30 @@ def test_server_step_cap(engine):
     assert len(out) == 1 and out[0]["steps"] == 3 and not out[0]["converged"]
     alone = engine.batch([geometry.pyridine()]).scf(threshold=1e-12, max_steps=3)[0]
     assert {k: out[0][k] for k in KEYS} == alone
+
+
+def test_size_classes_equal_alone_and_decouple(engine):
+    """Size classes: every result is the molecule's result alone, and small
+    molecules queued behind a large one finish before it (their class
+    iterates at its own pace on its own stream)."""
+    big = geometry.water_cluster(100, geometry.SplitMix64(21))       # N = 600
+    small = geometry.water_batch(12, seed=5, n_min=2, n_max=12)      # N <= 72
+    srv = engine.server(hbm_bytes=512e6, max_slots=32, max_atoms=2048, threshold=1e-6,
+                        classes=(200,))
+    srv.submit(0, big)
+    for i, m in enumerate(small):
+        srv.submit(1 + i, m)
+    out = {r["id"]: r for r in srv.drain()}
+    srv.close()
+    assert sorted(out) == list(range(len(small) + 1))
+    say("big done", out[0]["t_done"], "small done", max(out[1 + i]["t_done"]
+                                                        for i in range(len(small))))
+    assert max(out[1 + i]["t_done"] for i in range(len(small))) < out[0]["t_done"]
+    for i, m in enumerate([big] + small):
+        alone = engine.batch([m]).scf(threshold=1e-6)[0]
+        assert {k: out[i][k] for k in KEYS} == alone, i
+
+
+def test_size_class_bounds_rejected(engine):
+    with pytest.raises(api.SemiGpuError, match="bounds"):
+        engine.server(hbm_bytes=64e6, max_slots=8, max_atoms=512, classes=(300, 200))
