@@ -21,15 +21,15 @@
 //        (hcore.cpp:38-69) with the STO-6G overlap (overlap.cpp:68-192).
 //   k_hcore_diag                        one lane per (atom, AO pair): U + core
 //        attraction summed over the other atoms in atom order (hcore.cpp:71-99).
-//   k_core_core                         one lane per molecule, pairs in the
-//        reference order (corecorerepulsion.cpp:12-65); molecules of 512 atoms
-//        or more one workgroup each (k_core_core_wide).
+//   k_core_core                         one workgroup per molecule: pair terms
+//        evaluated in parallel, summed in the reference order
+//        (corecorerepulsion.cpp:12-65).
 #include "sg_common.h"
 #include "sg_kernels.h"
 
 namespace sg {
 
-constexpr int kWideAtoms = 512;  // molecules from this size take k_core_core_wide
+constexpr int kCoreChunk = 2048;  // core-core pair terms per round of k_core_core
 
 // ---------------------------------------------------------------------------
 // Multipole interactions in Hartree; (R, element of first pair, element of
@@ -701,50 +701,51 @@ __device__ __forceinline__ double core_pair(const AtomDesc& A, const AtomDesc& B
   return t * f;
 }
 
-// Molecules below kWideAtoms atoms: one lane each, pairs summed in the
-// reference order.
-__global__ void k_core_core(int n_mol, const AtomDesc* __restrict__ atoms,
-                            const MolDesc* __restrict__ mols, const double* __restrict__ blocks,
-                            double* __restrict__ e_cc, const int* __restrict__ mask) {
-  int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n_mol) return;
-  if (mask && !mask[m]) return;
-  const MolDesc M = mols[m];
-  if (M.n_atoms >= kWideAtoms) return;
-  const double* blk = blocks + M.blk0;
-  double e = 0.0;
-  for (int i = 0; i < M.n_atoms; ++i) {
-    const AtomDesc A = atoms[M.atom0 + i];
-    for (int j = i + 1; j < M.n_atoms; ++j) e += core_pair(A, atoms[M.atom0 + j], blk);
-  }
-  e_cc[m] = e;
-}
-
-// Molecules of kWideAtoms atoms or more: one workgroup each, lane t sums the
-// rows i = t, t + 256, ... in order, and the lane sums are added in a fixed order.
-__global__ void __launch_bounds__(256) k_core_core_wide(const AtomDesc* __restrict__ atoms,
-                                                         const MolDesc* __restrict__ mols,
-                                                         const double* __restrict__ blocks,
-                                                         double* __restrict__ e_cc,
-                                                         const int* __restrict__ mask) {
+// One workgroup per molecule.  The pairs (i, j > i) are taken in the
+// reference order in rounds of kCoreChunk: the lanes evaluate the terms of a
+// round into LDS, then one lane adds them to the running sum in order, so the
+// sum is the reference's serial sum bit for bit.
+__global__ void __launch_bounds__(256) k_core_core(const AtomDesc* __restrict__ atoms,
+                                                    const MolDesc* __restrict__ mols,
+                                                    const double* __restrict__ blocks,
+                                                    double* __restrict__ e_cc,
+                                                    const int* __restrict__ mask) {
   const int m = blockIdx.x;
   if (mask && !mask[m]) return;
   const MolDesc M = mols[m];
-  if (M.n_atoms < kWideAtoms) return;
+  const int na = M.n_atoms;
+  const int64_t npair = (int64_t)na * (na - 1) / 2;
   const double* blk = blocks + M.blk0;
+  const AtomDesc* at = atoms + M.atom0;
+  __shared__ double term[kCoreChunk];
   double e = 0.0;
-  for (int i = threadIdx.x; i < M.n_atoms; i += blockDim.x) {
-    const AtomDesc A = atoms[M.atom0 + i];
-    for (int j = i + 1; j < M.n_atoms; ++j) e += core_pair(A, atoms[M.atom0 + j], blk);
-  }
-  __shared__ double red[256];
-  red[threadIdx.x] = e;
-  __syncthreads();
-  for (int h = 128; h >= 1; h >>= 1) {
-    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+  for (int64_t p0 = 0; p0 < npair; p0 += kCoreChunk) {
+    const int cnt = (int)(npair - p0 < kCoreChunk ? npair - p0 : kCoreChunk);
+    for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+      // row i of pair p: the largest i with i (2 na - i - 1) / 2 <= p
+      const int64_t p = p0 + q;
+      const double b = 2.0 * na - 1.0;
+      int i = (int)((b - sqrt(b * b - 8.0 * (double)p)) * 0.5);
+      i = i < 0 ? 0 : (i > na - 2 ? na - 2 : i);
+      while (i > 0 && (int64_t)i * (2 * na - i - 1) / 2 > p) --i;
+      while ((int64_t)(i + 1) * (2 * na - i - 2) / 2 <= p) ++i;
+      const int j = (int)(p - (int64_t)i * (2 * na - i - 1) / 2) + i + 1;
+      term[q] = core_pair(at[i], at[j], blk);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int q0 = 0; q0 < cnt; q0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = q0 + u < cnt ? term[q0 + u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (q0 + u < cnt) e += v[u];
+      }
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) e_cc[m] = red[0];
+  if (threadIdx.x == 0) e_cc[m] = e;
 }
 
 // ---------------------------------------------------------------------------
@@ -781,10 +782,7 @@ void launch_hcore(const BatchDev& b, hipStream_t s) {
 }
 
 void launch_core_core(const BatchDev& b, hipStream_t s, const int* mask) {
-  k_core_core<<<blocks_for(b.n_mol, 64), 64, 0, s>>>(b.n_mol, b.atoms, b.mols, b.blocks, b.e_cc,
-                                                     mask);
-  if (b.max_n >= kWideAtoms)  // N >= n_atoms: no molecule is wide otherwise
-    k_core_core_wide<<<b.n_mol, 256, 0, s>>>(b.atoms, b.mols, b.blocks, b.e_cc, mask);
+  k_core_core<<<b.n_mol, 256, 0, s>>>(b.atoms, b.mols, b.blocks, b.e_cc, mask);
 }
 
 }  // namespace sg

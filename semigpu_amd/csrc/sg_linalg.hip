@@ -16,7 +16,7 @@
 //                   Sturm count, one lane per eigenvalue (only the indices the
 //                   caller selected).
 //  k_invit          eigenvectors of the tridiagonal matrix by inverse iteration
-//                   (3 sweeps, Gaussian elimination with partial pivoting, one
+//                   (2 sweeps, Gaussian elimination with partial pivoting, one
 //                   lane per vector).  Eigenvalues closer than 1e-3*||T|| (full
 //                   decomposition) or 1e-5*||T|| (SCF subspace) form a
 //                   cluster; after each sweep a cluster's vectors are
@@ -43,6 +43,11 @@ constexpr int kPanelLds = 48 * 1024;    // bytes of LDS for the V/W panel
 constexpr int kNbMax = 8;               // widest reflector panel
 constexpr int kPanelDoubles = kPanelLds / 8;
 constexpr double kEps = 2.220446049250313e-16;
+// Inverse-iteration solves per vector.  With eigenvalues bisected to a few
+// ulps, one solve from a random start already leaves the wanted direction
+// dominant by ~1/eps; the second solve (after the cluster orthogonalisation)
+// settles the vector to working accuracy.
+constexpr int kInvitSweeps = 2;
 
 // Cross-lane moves without the LDS crossbar (gfx950): DPP within rows of 16
 // lanes, v_permlane16/32_swap across them.  Doubles move as two 32-bit halves.
@@ -593,7 +598,7 @@ __device__ __forceinline__ double start_entry(int vec, int row) {
 // this lane (stride fs between components).  Loads are issued ahead of
 // the recurrence (4 rows).  Returns the 2-norm of the solution.  The right-hand side
 // is not rescaled: pivots below `tiny` are raised to it, so each solve grows
-// the vector by at most ~1/tiny and three sweeps stay far from overflow.
+// the vector by at most ~1/tiny and the sweeps stay far from overflow.
 __device__ double tri_solve(const double* __restrict__ d, const double* __restrict__ e, int n,
                             double lam, double tiny, double* __restrict__ Y, int ldy, int col,
                             double* __restrict__ fac, int fs) {
@@ -776,11 +781,11 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_
     Y[t] = start_entry(lo + col, row);
   }
   __syncthreads();
-  for (int sweep = 0; sweep < 3; ++sweep) {
+  for (int sweep = 0; sweep < kInvitSweeps; ++sweep) {
     for (int j = tid; j < k; j += nt) {
       const double nrm = tri_solve(d, e, n, sh[j], tiny, Y, k, j, fac + (j % L), L);
       const bool single = cstart[j] == j && (j + 1 == k || cstart[j + 1] != j);
-      if (single && sweep == 2) scale_column(Y, n, k, j, 1.0 / nrm);
+      if (single && sweep == kInvitSweeps - 1) scale_column(Y, n, k, j, 1.0 / nrm);
     }
     __threadfence_block();
     __syncthreads();
@@ -836,7 +841,7 @@ __global__ void __launch_bounds__(64) k_li_solve(const MolDesc* __restrict__ mol
       for (int i = 0; i < n; ++i) W.Y[(int64_t)i * k + j] = start_entry(lo + j, i);
     const double nrm = tri_solve(d, e, n, W.sh[j], tiny, W.Y, k, j, W.fac + j0, L);
     const bool single = W.cstart[j] == j && (j + 1 == k || W.cstart[j + 1] != j);
-    if (single && sweep == 2) scale_column(W.Y, n, k, j, 1.0 / nrm);
+    if (single && sweep == kInvitSweeps - 1) scale_column(W.Y, n, k, j, 1.0 / nrm);
   }
 }
 
@@ -1208,7 +1213,7 @@ void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all) {
     if (k == 0) continue;
     const int L = LargeWork::lanes(M.n_ao, k);
     k_li_setup<<<1, 64, 0, s>>>(b.mols, b.eig, b.tnorm, b.W, all ? 1 : 0, act, b.status, m);
-    for (int sweep = 0; sweep < 3; ++sweep) {
+    for (int sweep = 0; sweep < kInvitSweeps; ++sweep) {
       k_li_solve<<<(L + 63) / 64, 64, 0, s>>>(b.mols, b.dT, b.eT, b.tnorm, b.W, all ? 1 : 0, act,
                                               b.status, m, sweep);
       k_li_mgs<<<min(1024, (k + 3) / 4), 256, 0, s>>>(b.mols, b.W, all ? 1 : 0, act, b.status, m);
