@@ -54,6 +54,17 @@ def fock_of_guess(eng, mol):
     return b
 
 
+def exact_energy(P, H, F, n):
+    """1/2 sum_ij P_ij (H_ij + F_ij) over the full symmetric matrices, with
+    exactly rounded summation (math.fsum), from packed lower triangles."""
+    import math
+    t = P * (H + F)
+    diag = np.array([i * (i + 1) // 2 + i for i in range(n)])
+    off = np.ones(t.size, dtype=bool)
+    off[diag] = False
+    return 0.5 * math.fsum(np.concatenate([t[diag], 2.0 * t[off]]).tolist())
+
+
 def check_eigh(b, tag):
     n = b.n_ao(0)
     F = packed_to_full(b.packed(0, api.SG_FOCK), n)
@@ -159,10 +170,12 @@ def test_above_one_workgroup_limit(engine):
     assert np.abs(H - h_or).max() <= 1e-10 * np.abs(h_or).max()
     f_or = om.fock(blocks, h_or, P)
     assert np.abs(F - f_or).max() <= 1e-10 * np.abs(f_or).max()
-    e_el_or = po.energy(n, h_or, f_or, P)
+    e_exact = exact_energy(P, H, F, n)
+    e_el_or = po.energy(n, H, F, P)
     tol = 1e-12 * abs(r["e_el"])
-    say("E_el vs oracle energy of the same P", r["e_el"] - e_el_or, "tol", tol)
-    assert abs(r["e_el"] - e_el_or) <= tol
+    say("E_el - exact sum", r["e_el"] - e_exact, "oracle sum - exact", e_el_or - e_exact,
+        "tol", tol)
+    assert abs(r["e_el"] - e_exact) <= tol
     assert abs(r["e_cc"] - om.core_core(blocks)) <= 1e-12 * abs(r["e_cc"])
     with open(os.path.join(HERE, "golden", "water_250_oracle.json")) as fh:
         g = json.load(fh)
@@ -176,8 +189,9 @@ def test_water_box_1000(engine):
     """The 1000-water box (3000 atoms, N = 6000) on the multi-workgroup path:
     FP64 SCF to 1e-6 eV, the converged density idempotent with the right
     trace (checked through products with random vectors), its Fock matrix
-    and E_el against the oracle's from the same integrals and density, and
-    the mixed policy within 1e-4 eV of FP64."""
+    against the oracle's from the same integrals and density, E_el against the
+    exactly rounded sum of its matrices, and the mixed policy within 1e-4 eV
+    of FP64."""
     import pyoracle as po
     mol = geometry.water_box()
     b = engine.batch([mol])
@@ -203,9 +217,11 @@ def test_water_box_1000(engine):
     blocks = b.pair_blocks(0)
     f_or = om.fock(blocks, H, P)
     assert np.abs(F - f_or).max() <= 1e-10 * np.abs(f_or).max()
-    e_or = po.energy(n, H, f_or, P)
-    say("box E_el - oracle", r["e_el"] - e_or)
-    assert abs(r["e_el"] - e_or) <= 1e-12 * abs(r["e_el"])
+    # E_el = 1/2 sum P (H + F) of the device's own matrices, summed exactly
+    e_exact = exact_energy(P, H, F, n)
+    e_or = po.energy(n, H, F, P)
+    say("box E_el - exact", r["e_el"] - e_exact, "oracle sum - exact", e_or - e_exact)
+    assert abs(r["e_el"] - e_exact) <= 1e-12 * abs(e_exact)
     say("case water box mixed scf")
     c = b.scf(threshold=1e-6, max_steps=60, policy="mixed")[0]
     say("box mixed", c, "dE", c["e_tot"] - r["e_tot"])
