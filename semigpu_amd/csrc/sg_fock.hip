@@ -22,6 +22,7 @@
 namespace sg {
 
 constexpr int kEnergyChunk = 16384;  // entries of a large molecule's E_el per workgroup
+constexpr double kFp32Floor = 7.62939453125e-06;  // 2^-17 = 64 ulps of FP32 at 1
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -323,7 +324,10 @@ __global__ void k_scf_update(int n_mol, double threshold, int max_steps, double 
   const int step_prec = prec[m];
   const int conv = step_prec == 0 && prev_prec[m] == 0 && de < threshold;
   prev_prec[m] = step_prec;
-  if (step_prec && (de < switch_ev || st >= fp32_cap)) prec[m] = 0;
+  // FP32 steps end at |dE| < switch_ev, or once |dE| is below what FP32
+  // contractions resolve on this energy (kFp32Floor |E_el|, 64 FP32 ulps), or
+  // after fp32_cap steps
+  if (step_prec && (de < switch_ev || de < kFp32Floor * fabs(e) || st >= fp32_cap)) prec[m] = 0;
   converged[m] = conv;
   const int still = !conv && st < max_steps;
   active[m] = still;

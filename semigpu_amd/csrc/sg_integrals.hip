@@ -701,10 +701,21 @@ __device__ __forceinline__ double core_pair(const AtomDesc& A, const AtomDesc& B
   return t * f;
 }
 
-// One workgroup per molecule.  The pairs (i, j > i) are taken in the
-// reference order in rounds of kCoreChunk: the lanes evaluate the terms of a
-// round into LDS, then one lane adds them to the running sum in order, so the
-// sum is the reference's serial sum bit for bit.
+// Pair (i, j > i) of index p in the reference order (row-major over i < j).
+__device__ __forceinline__ void pair_of(int64_t p, int na, int& i, int& j) {
+  // row i: the largest i with i (2 na - i - 1) / 2 <= p
+  const double b = 2.0 * na - 1.0;
+  i = (int)((b - sqrt(b * b - 8.0 * (double)p)) * 0.5);
+  i = i < 0 ? 0 : (i > na - 2 ? na - 2 : i);
+  while (i > 0 && (int64_t)i * (2 * na - i - 1) / 2 > p) --i;
+  while ((int64_t)(i + 1) * (2 * na - i - 2) / 2 <= p) ++i;
+  j = (int)(p - (int64_t)i * (2 * na - i - 1) / 2) + i + 1;
+}
+
+// One workgroup per molecule.  The pairs are taken in the reference order in
+// rounds of kCoreChunk terms: waves 1-3 evaluate round r + 1 into one LDS
+// buffer while lane 0 of wave 0 adds round r from the other to the running
+// sum, in order, so the sum is the reference's serial sum bit for bit.
 __global__ void __launch_bounds__(256) k_core_core(const AtomDesc* __restrict__ atoms,
                                                     const MolDesc* __restrict__ mols,
                                                     const double* __restrict__ blocks,
@@ -717,30 +728,38 @@ __global__ void __launch_bounds__(256) k_core_core(const AtomDesc* __restrict__ 
   const int64_t npair = (int64_t)na * (na - 1) / 2;
   const double* blk = blocks + M.blk0;
   const AtomDesc* at = atoms + M.atom0;
-  __shared__ double term[kCoreChunk];
-  double e = 0.0;
-  for (int64_t p0 = 0; p0 < npair; p0 += kCoreChunk) {
+  __shared__ double term[2][kCoreChunk];
+  const int nr = (int)((npair + kCoreChunk - 1) / kCoreChunk);
+  auto fill = [&](int r, int t0, int nt) {
+    const int64_t p0 = (int64_t)r * kCoreChunk;
     const int cnt = (int)(npair - p0 < kCoreChunk ? npair - p0 : kCoreChunk);
-    for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-      // row i of pair p: the largest i with i (2 na - i - 1) / 2 <= p
-      const int64_t p = p0 + q;
-      const double b = 2.0 * na - 1.0;
-      int i = (int)((b - sqrt(b * b - 8.0 * (double)p)) * 0.5);
-      i = i < 0 ? 0 : (i > na - 2 ? na - 2 : i);
-      while (i > 0 && (int64_t)i * (2 * na - i - 1) / 2 > p) --i;
-      while ((int64_t)(i + 1) * (2 * na - i - 2) / 2 <= p) ++i;
-      const int j = (int)(p - (int64_t)i * (2 * na - i - 1) / 2) + i + 1;
-      term[q] = core_pair(at[i], at[j], blk);
+    for (int q = t0; q < cnt; q += nt) {
+      int i, j;
+      pair_of(p0 + q, na, i, j);
+      term[r & 1][q] = core_pair(at[i], at[j], blk);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int q0 = 0; q0 < cnt; q0 += 8) {
-        double v[8];
+  };
+  if (nr > 0) fill(0, threadIdx.x, blockDim.x);
+  __syncthreads();
+  double e = 0.0;
+  for (int r = 0; r < nr; ++r) {
+    if (threadIdx.x >= 64) {
+      if (r + 1 < nr) fill(r + 1, threadIdx.x - 64, blockDim.x - 64);
+    } else if (threadIdx.x == 0) {
+      const int64_t p0 = (int64_t)r * kCoreChunk;
+      const int cnt = (int)(npair - p0 < kCoreChunk ? npair - p0 : kCoreChunk);
+      const double* tb = term[r & 1];
+      double v[8], w[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = q0 + u < cnt ? term[q0 + u] : 0.0;
+      for (int u = 0; u < 8; ++u) v[u] = u < cnt ? tb[u] : 0.0;
+      for (int q0 = 0; q0 < cnt; q0 += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) w[u] = q0 + 8 + u < cnt ? tb[q0 + 8 + u] : 0.0;
 #pragma unroll
         for (int u = 0; u < 8; ++u)
           if (q0 + u < cnt) e += v[u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = w[u];
       }
     }
     __syncthreads();

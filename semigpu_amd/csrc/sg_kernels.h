@@ -101,10 +101,13 @@ __host__ __device__ inline void selection(const MolDesc& M, int want_all, int& l
 //   2 N^2   inverse-iteration factors, 3 N x L
 //   4 N^2   V and W panels (kLargeNb x N each), the product vector (N), the
 //           compact-WY T factors (16 x 16 per group of 16 reflectors), the
-//           inverse-iteration shifts and cluster tables
+//           inverse-iteration shifts and cluster tables; during the
+//           tridiagonalisation the T, shift and cluster rooms hold the column
+//           step's partial sums and the current column
 struct LargeWork {
   double *A, *Y, *fac, *V, *W, *py, *T, *sh;
   int *cstart, *clist;  // clist[k + 1] holds the number of clusters
+  double *part, *part2, *xv;
   __host__ __device__ LargeWork(double* work, const MolDesc& M) {
     const int64_t n = M.n_ao, nn = n * n;
     A = work + 5 * M.mat0;
@@ -117,6 +120,9 @@ struct LargeWork {
     sh = T + 16 * n + 256;
     cstart = (int*)(sh + n);
     clist = cstart + n;
+    part = T;                              // <= (N/32 + 1) x (2 kLargeNb + 1)
+    part2 = sh;                            // <= N/256 + 1
+    xv = (double*)cstart;                  // N (cstart and clist: N + 1 doubles)
   }
   // vectors whose factors are in flight at once (3 N x L doubles of factors)
   static __host__ __device__ int lanes(int n, int k) {

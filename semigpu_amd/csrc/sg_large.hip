@@ -7,8 +7,10 @@
 // the whole GPU for one molecule:
 //  * the work matrix is kept in full symmetric storage, so the product with
 //    the trailing block is a row-wise dot that workgroups split by rows;
-//  * one column step (finish the previous column's W, build the next
-//    reflector) is a single-workgroup launch; the product is a grid launch;
+//  * a column is two grid launches: the product (which also forms the
+//    reflector from the column and leaves per-workgroup partial dots), then
+//    the completion of W's column with the next column's corrections (one row
+//    per thread), whose workgroups each add the partial dots in order;
 //  * the trailing update of each panel is a grid of 16 x 16 matrix-core tiles;
 //  * the back-transformation gives each workgroup a 16-column slice of Y and
 //    walks the reflector groups with their T factors precomputed.
@@ -21,7 +23,9 @@ namespace sg {
 
 namespace {
 
-constexpr int kLT = 1024;  // threads of the single-workgroup column step
+constexpr int kColRows = 256;                    // rows per workgroup of the column kernels
+constexpr int kProdWaves = 8;                    // waves per product workgroup (4 rows each)
+constexpr int kPartStride = 2 * kLargeNb + 1;    // partial dots per product workgroup
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -35,17 +39,6 @@ __device__ __forceinline__ double wred(double v) {
   return v;
 }
 
-__device__ double bsum(double v, double* red) {
-  v = wred(v);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
-  __syncthreads();
-  if (l == 0) red[w] = v;
-  __syncthreads();
-  double t = 0.0;
-  for (int i = 0; i < nw; ++i) t += red[i];
-  return t;
-}
-
 }  // namespace
 
 __global__ void k_lt_copy(const double* __restrict__ F, double* __restrict__ A, int64_t nn,
@@ -56,113 +49,102 @@ __global__ void k_lt_copy(const double* __restrict__ F, double* __restrict__ A, 
     A[t] = F[t];
 }
 
-// Column step of panel k0: if fin, W column j-1 from the product py (the
-// dlatrd update of column c-1); if col, reflector of column c = k0 + j.
-__global__ void __launch_bounds__(kLT) k_lt_column(int n, double* __restrict__ A,
-                                                    double* __restrict__ V,
-                                                    double* __restrict__ W,
-                                                    const double* __restrict__ py,
-                                                    double* __restrict__ d, double* __restrict__ e,
-                                                    double* __restrict__ tau, int k0, int j,
-                                                    int fin, int col,
-                                                    const int* __restrict__ active,
-                                                    const int* __restrict__ status, int m) {
-  if (!mol_live(active, status, m)) return;
-  extern __shared__ double lds[];
-  double* x = lds;           // n
-  double* red = x + n;       // 16
-  double* tv = red + 16;     // 2 kLargeNb
-  const int tid = threadIdx.x, nt = blockDim.x;
-  const int wave = tid >> 6, lane = tid & 63, nwave = nt >> 6;
-  if (fin) {
-    const int jj = j - 1, c1 = k0 + jj;
-    const double tk = tau[c1];
-    const double* vj = V + (int64_t)jj * n;
-    for (int qq = wave; qq < 2 * jj; qq += nwave) {
-      const double* cv = qq < jj ? W + (int64_t)qq * n : V + (int64_t)(qq - jj) * n;
-      double s = 0.0;
-      for (int i = c1 + 1 + lane; i < n; i += 64) s += cv[i] * vj[i];
-      s = wred(s);
-      if (lane == 0) tv[qq] = s;
-    }
-    __syncthreads();
-    double pv = 0.0;
-    for (int i = c1 + 1 + tid; i < n; i += nt) {
-      double s = py[i];
-      for (int q = 0; q < jj; ++q)
-        s -= V[(int64_t)q * n + i] * tv[q] + W[(int64_t)q * n + i] * tv[jj + q];
-      s *= tk;
-      x[i] = s;
-      pv += s * vj[i];
-    }
-    pv = bsum(pv, red);
-    const double al = -0.5 * tk * pv;
-    double* wj = W + (int64_t)jj * n;
-    for (int i = tid; i < n; i += nt) wj[i] = i > c1 ? x[i] + al * vj[i] : 0.0;
-    __threadfence_block();
-    __syncthreads();
-  }
-  if (!col) return;
-  const int c = k0 + j;
-  // column c = row c of the symmetric matrix, less the panel's corrections
-  for (int i = c + tid; i < n; i += nt) {
-    double s = A[(int64_t)c * n + i];
-    for (int q = 0; q < j; ++q)
-      s -= V[(int64_t)q * n + i] * W[(int64_t)q * n + c] + W[(int64_t)q * n + i] * V[(int64_t)q * n + c];
-    x[i] = s;
-  }
+// Sum of one value per thread over the workgroup, in a fixed order.
+__device__ double block_total(double v, double* red) {
+  v = wred(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
+  if (l == 0) red[w] = v;
   __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < nw; ++i) t += red[i];
+  __syncthreads();
+  return t;
+}
+
+// Column c with no panel corrections (the first column of a panel): x = row
+// c of A from the diagonal on, and per-workgroup partial sums of x_i^2, i >= c+2.
+__global__ void __launch_bounds__(kColRows) k_lt_x0(int n, const double* __restrict__ A,
+                                                     double* __restrict__ xv,
+                                                     double* __restrict__ part2, int c,
+                                                     const int* __restrict__ active,
+                                                     const int* __restrict__ status, int m) {
+  if (!mol_live(active, status, m)) return;
+  __shared__ double red[kColRows / 64];
+  const int i = c + blockIdx.x * kColRows + threadIdx.x;
+  double sq = 0.0;
+  if (i < n) {
+    const double v = A[(int64_t)c * n + i];
+    xv[i] = v;
+    if (i >= c + 2) sq = v * v;
+  }
+  sq = block_total(sq, red);
+  if (threadIdx.x == 0) part2[blockIdx.x] = sq;
+}
+
+// Reflector of column c (j-th of its panel) from x and the partial sums of
+// x^2, then the product py = A v over rows i > c (4 rows per wave, v formed
+// on the fly from x) and, per workgroup of 32 rows, the partial dots the next
+// step needs: v.py, W_q.v and V_q.v (q < j).  Writes v into V_j and into the
+// upper part of row c of A; d, e, tau of column c.
+__global__ void __launch_bounds__(64 * kProdWaves) k_lt_prod(int n, double* __restrict__ A,
+                                                              double* __restrict__ V,
+                                                              const double* __restrict__ W,
+                                                              const double* __restrict__ xv,
+                                                              const double* __restrict__ part2,
+                                                              double* __restrict__ part,
+                                                              double* __restrict__ py,
+                                                              double* __restrict__ d,
+                                                              double* __restrict__ e,
+                                                              double* __restrict__ tau, int c, int j,
+                                                              const int* __restrict__ active,
+                                                              const int* __restrict__ status,
+                                                              int m) {
+  if (!mol_live(active, status, m)) return;
+  constexpr int R = 4 * kProdWaves;  // rows per workgroup
+  __shared__ double red[kProdWaves];
+  __shared__ double rows_py[R], rows_v[R];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // reflector scalars (every workgroup computes the same numbers)
+  const int nG = (n - c + kColRows - 1) / kColRows;
   double s2 = 0.0;
-  for (int i = c + 2 + tid; i < n; i += nt) s2 += x[i] * x[i];
-  s2 = bsum(s2, red);
-  const double alpha = x[c + 1];
-  double* vj = V + (int64_t)j * n;
-  if (s2 == 0.0) {
-    if (tid == 0) {
-      d[c] = x[c];
-      e[c] = alpha;
-      tau[c] = 0.0;
-    }
-    for (int i = tid; i < n; i += nt) {
-      vj[i] = 0.0;
-      if (i > c) A[(int64_t)c * n + i] = 0.0;
-    }
-    return;
+  for (int g = threadIdx.x; g < nG; g += blockDim.x) s2 += part2[g];
+  s2 = block_total(s2, red);
+  const double alpha = xv[c + 1];
+  const bool zero = s2 == 0.0;
+  double beta = alpha, tk = 0.0, sc = 0.0;
+  if (!zero) {
+    beta = -copysign(sqrt(alpha * alpha + s2), alpha);
+    tk = (beta - alpha) / beta;
+    sc = 1.0 / (alpha - beta);
   }
-  const double beta = -copysign(sqrt(alpha * alpha + s2), alpha);
-  const double tk = (beta - alpha) / beta;
-  const double sc = 1.0 / (alpha - beta);
-  for (int i = tid; i < n; i += nt) {
-    const double v = i <= c ? 0.0 : (i == c + 1 ? 1.0 : x[i] * sc);
-    vj[i] = v;
-    if (i > c) A[(int64_t)c * n + i] = v;
-  }
-  if (tid == 0) {
-    d[c] = x[c];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    d[c] = xv[c];
     e[c] = beta;
     tau[c] = tk;
   }
-}
-
-// py[i] = sum_{l > c} A[i][l] v_l for i > c: a wave takes 4 rows, the 4 row
-// sums are reduced together.
-__global__ void __launch_bounds__(256) k_lt_symv(int n, const double* __restrict__ A,
-                                                  const double* __restrict__ v,
-                                                  double* __restrict__ py, int c,
-                                                  const int* __restrict__ active,
-                                                  const int* __restrict__ status, int m) {
-  if (!mol_live(active, status, m)) return;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i0 = c + 1 + (blockIdx.x * 4 + wave) * 4;
-  if (i0 >= n) return;
+  const double one = zero ? 0.0 : 1.0;
+  auto vof = [&](int l) { return l == c + 1 ? one : xv[l] * sc; };
+  const int r0 = c + 1 + blockIdx.x * R;  // first row of this workgroup
+  const int i0 = r0 + wave * 4;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const double* ar[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) ar[r] = A + (int64_t)min(i0 + r, n - 1) * n;
-  for (int l = c + 1 + lane; l < n; l += 64) {
-    const double vl = v[l];
+  if (i0 < n) {
+    // two 64-column tiles per batch of loads; the second is clamped to the
+    // row's last column and weighted 0 past it
+    for (int l = c + 1 + lane; l < n; l += 128) {
+      const int l2 = l + 64 < n ? l + 64 : n - 1;
+      const double v0 = vof(l), v1 = l + 64 < n ? vof(l2) : 0.0;
+      double a0[4], a1[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) acc[r] += ar[r][l] * vl;
+      for (int r = 0; r < 4; ++r) {
+        a0[r] = ar[r][l];
+        a1[r] = ar[r][l2];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] += a0[r] * v0 + a1[r] * v1;
+    }
   }
   // lanes < 32 keep rows 0,1 / >= 32 rows 2,3; then 16-lane halves
   double v2[2], v1;
@@ -180,7 +162,144 @@ __global__ void __launch_bounds__(256) k_lt_symv(int n, const double* __restrict
 #pragma unroll
   for (int off = 8; off >= 1; off >>= 1) v1 += __shfl_xor(v1, off, 64);
   const int rho = ((lane >> 5) & 1) * 2 + ((lane >> 4) & 1);
-  if ((lane & 15) == 0 && i0 + rho < n) py[i0 + rho] = v1;
+  if ((lane & 15) == 0) {
+    const int i = i0 + rho;
+    rows_py[wave * 4 + rho] = i < n ? v1 : 0.0;
+    if (i < n) py[i] = v1;
+  }
+  if (threadIdx.x < R) {
+    const int i = r0 + threadIdx.x;
+    const double v = i < n ? vof(i) : 0.0;
+    rows_v[threadIdx.x] = v;
+    if (i < n) {
+      V[(int64_t)j * n + i] = v;
+      A[(int64_t)c * n + i] = v;
+    }
+  }
+  __syncthreads();
+  // partial dots over this workgroup's rows: [0, j) W_q.v, [j, 2j) V_q.v, 2j v.py
+  const int nd = 2 * j + 1;
+  if (threadIdx.x < nd) {
+    const int t = threadIdx.x;
+    const double* src = t < j ? W + (int64_t)t * n : V + (int64_t)(t - j) * n;
+    double sacc = 0.0;
+    for (int b0 = 0; b0 < R; b0 += 16) {
+      double x[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = min(r0 + b0 + r, n - 1);
+        x[r] = t == 2 * j ? rows_py[b0 + r] : src[i];
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) sacc += x[r] * rows_v[b0 + r];  // rows_v is 0 past the matrix
+    }
+    part[(int64_t)blockIdx.x * kPartStride + t] = sacc;
+  }
+}
+
+// Completes W_j (the dlatrd update of column c = k0 + j):
+//   s = tau (py - V tv1 - W tv2),  W_j = s + a v,  a = -tau/2 (s.v),
+// with tv1 = W^T v, tv2 = V^T v and s.v = tau (v.py - 2 tv1.tv2) from the
+// product's partial dots (added up by every workgroup); then, when another
+// column of the panel follows, x for column c+1 (row c+1 of A less the panel's
+// corrections) and the partial sums of x^2.  One row per thread.
+__global__ void __launch_bounds__(kColRows) k_lt_cols(int n, const double* __restrict__ A,
+                                                       const double* __restrict__ V,
+                                                       double* __restrict__ W,
+                                                       const double* __restrict__ py,
+                                                       const double* __restrict__ part,
+                                                       const double* __restrict__ tau,
+                                                       double* __restrict__ xv,
+                                                       double* __restrict__ part2, int c, int j,
+                                                       int next, const int* __restrict__ active,
+                                                       const int* __restrict__ status, int m) {
+  if (!mol_live(active, status, m)) return;
+  __shared__ double tv[2 * kLargeNb + 1];
+  __shared__ double red[kColRows / 64];
+  {
+    // the product's partial dots, summed in workgroup order: thread T takes
+    // dot T % 64 over the workgroups g = T / 64, T / 64 + 4, ...; the four
+    // slices are then added in order (every workgroup forms the same sums)
+    __shared__ double sl[kColRows / 64][64];
+    const int t = threadIdx.x & 63, k = threadIdx.x >> 6;
+    const int G = (n - c - 1 + 4 * kProdWaves - 1) / (4 * kProdWaves), nd = 2 * j + 1;
+    double sacc = 0.0;
+    if (t < nd) {
+      for (int g0 = k; g0 < G; g0 += 4 * (kColRows / 64)) {
+        double x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int g = g0 + u * (kColRows / 64);
+          x[u] = g < G ? part[(int64_t)g * kPartStride + t] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sacc += x[u];
+      }
+    }
+    sl[k][t] = sacc;
+    __syncthreads();
+    if (threadIdx.x < nd) {
+      double v = 0.0;
+#pragma unroll
+      for (int q = 0; q < kColRows / 64; ++q) v += sl[q][threadIdx.x];
+      tv[threadIdx.x] = v;
+    }
+  }
+  __syncthreads();
+  const double tk = tau[c];
+  double cross = 0.0;
+  for (int q = 0; q < j; ++q) cross += tv[q] * tv[j + q];
+  const double al = -0.5 * tk * (tk * (tv[2 * j] - 2.0 * cross));
+  const double* vj = V + (int64_t)j * n;
+  // loads of 8 panel columns issued before their products
+  auto s_of = [&](int r) {
+    double s = py[r];
+    for (int q0 = 0; q0 < j; q0 += 8) {
+      double a[8], b[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int q = min(q0 + u, j - 1);
+        a[u] = V[(int64_t)q * n + r];
+        b[u] = W[(int64_t)q * n + r];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (q0 + u < j) s -= a[u] * tv[q0 + u] + b[u] * tv[j + q0 + u];
+    }
+    return tk * s;
+  };
+  const int i = c + 1 + blockIdx.x * kColRows + threadIdx.x;
+  double w = 0.0;
+  if (i < n) {
+    w = s_of(i) + al * vj[i];
+    W[(int64_t)j * n + i] = w;
+  }
+  if (!next) return;
+  const int c1 = c + 1;
+  const double wc1 = s_of(c1) + al * vj[c1];
+  double sq = 0.0;
+  if (i < n) {
+    double x = A[(int64_t)c1 * n + i];
+    for (int q0 = 0; q0 < j; q0 += 8) {
+      double a[8], b[8], ac[8], bc[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int q = min(q0 + u, j - 1);
+        a[u] = V[(int64_t)q * n + i];
+        b[u] = W[(int64_t)q * n + i];
+        ac[u] = V[(int64_t)q * n + c1];
+        bc[u] = W[(int64_t)q * n + c1];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (q0 + u < j) x -= a[u] * bc[u] + b[u] * ac[u];
+    }
+    x -= vj[i] * wc1 + w * vj[c1];
+    xv[i] = x;
+    if (i >= c1 + 2) sq = x * x;
+  }
+  sq = block_total(sq, red);
+  if (threadIdx.x == 0) part2[blockIdx.x] = sq;
 }
 
 // A[r:, r:] -= V W^T + W V^T (both triangles), one 16 x 16 tile per wave.
@@ -362,18 +481,18 @@ void launch_tridiag_large(const BatchDev& b, hipStream_t s, int m, const MolDesc
   const int* st = b.status;
   k_lt_copy<<<1024, 256, 0, s>>>(b.F + M.mat0, L.A, (int64_t)n * n, act, st, m);
   const int nref = n > 2 ? n - 2 : 0;
-  const size_t col_lds = sizeof(double) * ((size_t)n + 16 + 2 * kLargeNb);
   for (int k0 = 0; k0 < nref; k0 += kLargeNb) {
     const int nb = min(kLargeNb, nref - k0);
+    k_lt_x0<<<(n - k0 + kColRows - 1) / kColRows, kColRows, 0, s>>>(n, L.A, L.xv, L.part2, k0,
+                                                                     act, st, m);
     for (int j = 0; j < nb; ++j) {
       const int c = k0 + j;
-      k_lt_column<<<1, kLT, col_lds, s>>>(n, L.A, L.V, L.W, L.py, d, e, tau, k0, j, j > 0, 1, act,
-                                          st, m);
       const int rows = n - c - 1;
-      k_lt_symv<<<(rows + 15) / 16, 256, 0, s>>>(n, L.A, L.V + (int64_t)j * n, L.py, c, act, st, m);
+      k_lt_prod<<<(rows + 4 * kProdWaves - 1) / (4 * kProdWaves), 64 * kProdWaves, 0, s>>>(
+          n, L.A, L.V, L.W, L.xv, L.part2, L.part, L.py, d, e, tau, c, j, act, st, m);
+      k_lt_cols<<<(rows + kColRows - 1) / kColRows, kColRows, 0, s>>>(
+          n, L.A, L.V, L.W, L.py, L.part, tau, L.xv, L.part2, c, j, j + 1 < nb, act, st, m);
     }
-    k_lt_column<<<1, kLT, col_lds, s>>>(n, L.A, L.V, L.W, L.py, d, e, tau, k0, nb, 1, 0, act, st,
-                                        m);
     const int r = k0 + nb;
     const int TT = (n - r + 15) >> 4;
     const int64_t tiles = (int64_t)TT * TT;
