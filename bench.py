@@ -186,6 +186,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-extra", default="",
+                    help="more arrival rates, RATE:SECONDS[,RATE:SECONDS...], reported under "
+                         "serving_rates (e.g. 1000:1,5000:1)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 1 GiB device copy (PMC byte-counter calibration)")
@@ -347,6 +350,15 @@ def main():
         out["latency"] = {"p50_ms": out["serving"]["p50_ms"], "p99_ms": out["serving"]["p99_ms"],
                           "basis": "serving stream, arrival to result"}
         print("serving: %.1f s" % (time.time() - t), file=sys.stderr, flush=True)
+        extra = [x for x in args.serve_extra.split(",") if x]
+        if extra:
+            out["serving_rates"] = [out["serving"]]
+            for x in extra:
+                rate, secs = (float(v) for v in x.split(":"))
+                t = time.time()
+                out["serving_rates"].append(serve(eng, rate, secs, args.threshold, args.policy))
+                print("serving at %g/s for %g s: %.1f s" % (rate, secs, time.time() - t),
+                      file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mols, steps)
     eng.close()

@@ -100,9 +100,10 @@ int sg_scf_run(sg_batch* b, double threshold_ev, int max_steps, sg_scf_result* r
 /* Same SCF with a precision policy (no counterpart in the reference, which is
  * FP64 throughout).  SG_POLICY_FP64 is sg_scf_run.  SG_POLICY_MIXED runs the
  * Fock contractions and the density product in FP32 (density on the matrix
- * cores) until a molecule's |dE| < switch_ev (1e-3 eV suggested), then in FP64
- * to convergence; integrals, eigensolver and energies stay FP64, and
- * convergence is only tested between two FP64 steps. */
+ * cores) until a molecule's |dE| < switch_ev (1e-3 eV suggested), or |dE| <
+ * 2^-17 |E_el| (below what FP32 contractions resolve on that energy), or 40
+ * FP32 steps; then in FP64 to convergence.  Integrals, eigensolver and energies
+ * stay FP64, and convergence is only tested between two FP64 steps. */
 #define SG_POLICY_FP64 0
 #define SG_POLICY_MIXED 1
 int sg_scf_run_policy(sg_batch* b, double threshold_ev, int max_steps, int policy,
