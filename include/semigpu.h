@@ -113,6 +113,13 @@ int sg_scf_run_policy(sg_batch* b, double threshold_ev, int max_steps, int polic
  * (status[n_mol]; > 0 where dsyev would report info > 0). */
 int sg_get_status(const sg_batch* b, int* status);
 
+/* Heat of formation (kcal/mol) of a molecule from its SCF total energy (eV):
+ * (E_tot - sum E_isol) x 23.060529 + sum dH_f(atom), with the atomic tables
+ * the reference carries unused (heat, eisol: MNDO_parameters.h:40-41,
+ * MNDO_parameters.cpp:58-121).  Host arithmetic only (no device); no
+ * counterpart in the reference, which stops at E_tot (semiGPU.cc:95). */
+int sg_heat_of_formation(int n_atoms, const int* Z, double e_tot_ev, double* dhf_kcal);
+
 /* Host copies for the reference's host-side views. */
 long sg_pair_block_count(const sg_batch* b, int mol);
 /* double**** all2CenterIntegral contents in the reference jagged order

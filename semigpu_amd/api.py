@@ -38,6 +38,7 @@ EXPORTS = (
     "sg_get_status", "sg_calibration_copy", "sg_timing_enable", "sg_timing_read", "sg_timing_reset",
     "sg_server_create", "sg_server_destroy", "sg_server_submit", "sg_server_step",
     "sg_server_poll", "sg_server_clock", "sg_ctx_set_large_basis", "sg_server_create_classes",
+    "sg_heat_of_formation",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -87,6 +88,7 @@ def load_library(path: str = LIB_PATH):
     L.sg_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
     L.sg_ctx_destroy.argtypes = [_vp]
     L.sg_ctx_set_large_basis.argtypes = [_vp, ctypes.c_int]
+    L.sg_heat_of_formation.argtypes = [ctypes.c_int, _ip, ctypes.c_double, _dp]
     L.sg_batch_create.argtypes = [_vp, ctypes.c_int, _ip, _ip, _dp, ctypes.POINTER(_vp)]
     L.sg_batch_destroy.argtypes = [_vp]
     L.sg_batch_molecule_info.argtypes = [_vp, ctypes.c_int, _ip, _ip, _ip]
@@ -139,6 +141,16 @@ def _d(a: np.ndarray):
 
 def _i(a: np.ndarray):
     return a.ctypes.data_as(_ip)
+
+
+def heat_of_formation(molecule, e_tot: float) -> float:
+    """MNDO heat of formation (kcal/mol) of a molecule [(Z, x, y, z), ...] from
+    its SCF total energy in eV (sg_heat_of_formation; host arithmetic)."""
+    L = load_library()
+    Z = np.asarray([int(a[0]) for a in molecule], dtype=np.int32)
+    out = ctypes.c_double()
+    _check(L.sg_heat_of_formation(len(Z), _i(Z), float(e_tot), ctypes.byref(out)))
+    return out.value
 
 
 class Engine:

@@ -675,6 +675,30 @@ int sg_set_pair_blocks(sg_batch* b, int mol, const double* in) {
   return SG_OK;
 }
 
+int sg_heat_of_formation(int n_atoms, const int* Z, double e_tot_ev, double* dhf_kcal) {
+  if (n_atoms <= 0 || !Z || !dhf_kcal) return fail(SG_ERR_INVALID, "bad arguments");
+  // MNDO atomic tables (MNDO_parameters.cpp:58-59, 80-81, 100-101, 120-121):
+  // experimental heat of formation of the atom (kcal/mol) and the MNDO
+  // energy of the isolated atom (eV); 1 eV = 23.060529 kcal/mol
+  double e_isol = 0.0, h_atoms = 0.0;
+  for (int i = 0; i < n_atoms; ++i) {
+    double heat, eisol;
+    switch (Z[i]) {
+      case 1: heat = 52.102; eisol = -11.9062760; break;
+      case 6: heat = 170.89; eisol = -120.500606; break;
+      case 7: heat = 113.0; eisol = -202.566201; break;
+      case 8: heat = 59.559; eisol = -317.868506; break;
+      default:
+        return fail(SG_ERR_UNSUPPORTED, "element Z=" + std::to_string(Z[i]) +
+                                            " has no MNDO parameters here");
+    }
+    e_isol += eisol;
+    h_atoms += heat;
+  }
+  *dhf_kcal = (e_tot_ev - e_isol) * 23.060529 + h_atoms;
+  return SG_OK;
+}
+
 int sg_get_status(const sg_batch* b, int* status) {
   if (!b || !status) return fail(SG_ERR_INVALID, "bad arguments");
   HIP_TRY(hipSetDevice(b->ctx->device));
