@@ -48,6 +48,7 @@ constexpr double kEps = 2.220446049250313e-16;
 // dominant by ~1/eps; the second solve (after the cluster orthogonalisation)
 // settles the vector to working accuracy.
 constexpr int kInvitSweeps = 2;
+constexpr int kSolveBatch = 4;  // rows whose loads are in flight ahead of the solve
 
 // Cross-lane moves without the LDS crossbar (gfx950): DPP within rows of 16
 // lanes, v_permlane16/32_swap across them.  Doubles move as two 32-bit halves.
@@ -595,50 +596,58 @@ __device__ __forceinline__ double start_entry(int vec, int row) {
 
 // Solves (T - lam I) y = y in place (y = column `col` of Y, row stride ldy) by
 // Gaussian elimination with partial pivoting; fac holds 3 doubles per row for
-// this lane (stride fs between components).  Loads are issued ahead of
-// the recurrence (4 rows).  Returns the 2-norm of the solution.  The right-hand side
+// this lane (stride fs between components).  The loads of the next
+// kSolveBatch rows are issued while the current ones are processed.  Returns
+// the 2-norm of the solution.  The right-hand side
 // is not rescaled: pivots below `tiny` are raised to it, so each solve grows
 // the vector by at most ~1/tiny and the sweeps stay far from overflow.
 __device__ double tri_solve(const double* __restrict__ d, const double* __restrict__ e, int n,
                             double lam, double tiny, double* __restrict__ Y, int ldy, int col,
                             double* __restrict__ fac, int fs) {
+  constexpr int B = kSolveBatch;
   double ca = d[0] - lam, cb = n > 1 ? e[0] : 0.0;
   double y0 = Y[col];
-  for (int i0 = 0; i0 < n - 1; i0 += 4) {
-    double yb[4];
+  // forward elimination; the right-hand side of the next B rows is loaded
+  // while the current B rows are eliminated
+  double yb[B];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = i0 + 1 + r;
-      yb[r] = i < n ? Y[(int64_t)i * ldy + col] : 0.0;
+  for (int r = 0; r < B; ++r) {
+    const int i = 1 + r;
+    yb[r] = i < n ? Y[(int64_t)i * ldy + col] : 0.0;
+  }
+  for (int i0 = 0; i0 < n - 1; i0 += B) {
+    double yn[B];
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+      const int i = i0 + B + 1 + r;
+      yn[r] = i < n ? Y[(int64_t)i * ldy + col] : 0.0;
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < B; ++r) {
       const int i = i0 + r;
       if (i < n - 1) {
         const double sub = e[i], na = d[i + 1] - lam, nb = i + 2 < n ? e[i + 1] : 0.0;
         const double y1 = yb[r];
         double* fr = fac + (int64_t)i * 3 * fs;
-        if (fabs(ca) >= fabs(sub)) {
-          const double mlt = ca != 0.0 ? sub / ca : 0.0;
-          fr[0] = ca;
-          fr[fs] = cb;
-          fr[2 * fs] = 0.0;
-          Y[(int64_t)i * ldy + col] = y0;
-          y0 = y1 - mlt * y0;
-          ca = na - mlt * cb;
-          cb = nb;
-        } else {
-          const double mlt = ca / sub;
-          fr[0] = sub;
-          fr[fs] = na;
-          fr[2 * fs] = nb;
-          Y[(int64_t)i * ldy + col] = y1;
-          y0 = y0 - mlt * y1;
-          ca = cb - mlt * na;
-          cb = -mlt * nb;
-        }
+        // partial pivoting without a branch: pv keeps row i as the pivot row,
+        // else rows i and i+1 swap; one reciprocal (v_rcp_f64 + 2 Newton
+        // steps) per row
+        const bool pv = fabs(ca) >= fabs(sub);
+        const double num = pv ? sub : ca, den = pv ? ca : sub;
+        const double mlt = den != 0.0 ? num * rcp_nr(den) : 0.0;
+        fr[0] = den;
+        fr[fs] = pv ? cb : na;
+        fr[2 * fs] = pv ? 0.0 : nb;
+        Y[(int64_t)i * ldy + col] = pv ? y0 : y1;
+        const double ny0 = pv ? y1 - mlt * y0 : y0 - mlt * y1;
+        const double nca = pv ? na - mlt * cb : cb - mlt * na;
+        cb = pv ? nb : -mlt * nb;
+        y0 = ny0;
+        ca = nca;
       }
     }
+#pragma unroll
+    for (int r = 0; r < B; ++r) yb[r] = yn[r];
   }
   {
     double* fr = fac + (int64_t)(n - 1) * 3 * fs;
@@ -647,35 +656,48 @@ __device__ double tri_solve(const double* __restrict__ d, const double* __restri
     fr[2 * fs] = 0.0;
     Y[(int64_t)(n - 1) * ldy + col] = y0;
   }
+  // back substitution, the factors and right-hand side of the next B rows
+  // loaded while the current B rows are solved
   double x1 = 0.0, x2 = 0.0, nrm = 0.0;
-  for (int i0 = n - 1; i0 >= 0; i0 -= 4) {
-    double g0[4], g1[4], g2[4], yv[4];
+  double g0[B], g1[B], g2[B], yv[B];
+  auto load = [&](int i0, double* a0, double* a1, double* a2, double* av) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < B; ++r) {
       const int i = i0 - r >= 0 ? i0 - r : 0;
       const double* g = fac + (int64_t)i * 3 * fs;
-      g0[r] = g[0];
-      g1[r] = g[fs];
-      g2[r] = g[2 * fs];
-      yv[r] = Y[(int64_t)i * ldy + col];
+      a0[r] = g[0];
+      a1[r] = g[fs];
+      a2[r] = g[2 * fs];
+      av[r] = Y[(int64_t)i * ldy + col];
     }
+  };
+  load(n - 1, g0, g1, g2, yv);
+  for (int i0 = n - 1; i0 >= 0; i0 -= B) {
+    double h0[B], h1[B], h2[B], hv[B];
+    load(i0 - B, h0, h1, h2, hv);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < B; ++r) {
       const int i = i0 - r;
       if (i >= 0) {
         double u0 = g0[r];
         if (fabs(u0) < tiny) u0 = u0 >= 0.0 ? tiny : -tiny;
-        const double xi = (yv[r] - g1[r] * x1 - g2[r] * x2) / u0;
+        const double xi = (yv[r] - g1[r] * x1 - g2[r] * x2) * rcp_nr(u0);
         Y[(int64_t)i * ldy + col] = xi;
         x2 = x1;
         x1 = xi;
         nrm += xi * xi;
       }
     }
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+      g0[r] = h0[r];
+      g1[r] = h1[r];
+      g2[r] = h2[r];
+      yv[r] = hv[r];
+    }
   }
   return sqrt(nrm);
 }
-
 
 // Clusters of the selected eigenvalues lam[0..k) and the shifts of inverse
 // iteration: consecutive eigenvalues closer than ortol share a cluster.  The
@@ -740,7 +762,7 @@ __device__ void mgs_cluster(double* __restrict__ Y, int n, int k, int c0, int c1
   }
 }
 
-__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_invit(const MolDesc* __restrict__ mols,
+__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(3))) k_invit(const MolDesc* __restrict__ mols,
                                                const double* __restrict__ dA,
                                                const double* __restrict__ eA,
                                                const double* __restrict__ lamA,
