@@ -117,9 +117,9 @@ def cpu_baseline(mols, steps, budget_s: float = 25.0):
 
 
 # basis-size bounds of the server's size classes (each its own stream and pace).
-# Two classes plus the context stream stay within the process's 4 hardware
+# Three classes plus the context stream stay within the process's 4 hardware
 # queues, so no class shares a queue with another (which would serialise them).
-SERVE_CLASSES = (400,)
+SERVE_CLASSES = (300, 700)
 
 
 def serve(eng, rate: float, seconds: float, threshold: float, policy: str, seed: int = 7,
