@@ -1200,7 +1200,7 @@ void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all) {
 static void bisect_part(const BatchDev& b, hipStream_t s, bool all, const int* order, int count,
                         int N) {
   // two eigenvalues per lane: (k+1)/2 lanes, k <= N/2 on the SCF path
-  const int half = ((all ? N : N / 2) + 1) / 2;
+  const int half = std::max(1, ((all ? N : N / 2) + 1) / 2);  // N = 1: no wanted value
   const int threads = min(256, max(64, (half + 63) / 64 * 64));
   dim3 gb((half + threads - 1) / threads, count);
   k_bisect<<<gb, threads, sizeof(double) * (2 * N + 32), s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm,

@@ -218,3 +218,41 @@ def test_eigensolver_largest_basis(engine):
     assert np.abs(C.T @ C - np.eye(n)).max() <= 1e-12
     assert np.abs(F @ C - C * w).max() <= 1e-11 * scale
     b.close()
+
+
+EDGE = [
+    [(1, 0.0, 0.0, 0.0)],                       # N = 1, no occupied orbital
+    [(6, 0.0, 0.0, 0.0)], [(7, 0.0, 0.0, 0.0)], [(8, 0.0, 0.0, 0.0)],   # N = 4, one atom
+    [(1, 0.0, 0.0, 0.0), (1, 0.0, 0.0, 0.74)],  # N = 2 (no Householder reflector)
+    [(1, 0.0, 0.0, 0.0), (8, 0.0, 0.0, 0.97)],  # odd electron count (n_occ = floor)
+    [(6, 0.0, 0.0, 0.0), (8, 0.0, 0.0, 1.13)],
+]
+
+
+def test_edge_molecules_against_oracle(engine):
+    """Single atoms, H2, OH and CO, alone and in one batch with a large
+    cluster: energies and step counts as the oracle's, batch equal to alone."""
+    import pyoracle as po
+    mols = EDGE + [geometry.water_cluster(40, geometry.SplitMix64(3))]
+    say("case edge molecules", [geometry.n_basis(m) for m in mols])
+    res = engine.batch(mols).scf(threshold=1e-6)
+    for m, r in zip(mols, res):
+        o = po.Molecule(m).scf(threshold=1e-6, mode="nddo")
+        assert r["status"] == 0 and r["converged"] == o["converged"]
+        assert r["steps"] == o["steps"]
+        assert abs(r["e_tot"] - o["e_tot"]) <= 1e-6
+    for i in range(len(EDGE)):
+        assert engine.batch([mols[i]]).scf(threshold=1e-6)[0] == res[i], i
+
+
+def test_empty_molecule_rejected(engine):
+    L = engine.L
+    import ctypes
+    offs = np.array([0, 0, 1], dtype=np.int32)
+    Z = np.array([1], dtype=np.int32)
+    xyz = np.zeros(3)
+    h = ctypes.c_void_p()
+    rc = L.sg_batch_create(engine.h, 2, offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                           Z.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                           xyz.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(h))
+    assert rc == -1 and b"no atoms" in L.sg_last_error()

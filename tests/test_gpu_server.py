@@ -124,3 +124,19 @@ def test_size_classes_equal_alone_and_decouple(engine):
 def test_size_class_bounds_rejected(engine):
     with pytest.raises(api.SemiGpuError, match="bounds"):
         engine.server(hbm_bytes=64e6, max_slots=8, max_atoms=512, classes=(300, 200))
+
+
+def test_server_edge_molecules(engine):
+    """Single atoms, H2 and OH through the server (one class, then alone)."""
+    mols = [[(1, 0.0, 0.0, 0.0)], [(8, 0.0, 0.0, 0.0)], [(1, 0.0, 0.0, 0.0), (1, 0.0, 0.0, 0.74)],
+            [(1, 0.0, 0.0, 0.0), (8, 0.0, 0.0, 0.97)]]
+    srv = engine.server(hbm_bytes=16e6, max_slots=4, max_atoms=64, threshold=1e-6)
+    srv.submit(0, mols[0])
+    out = srv.drain()                      # a lone H atom in flight (N = 1)
+    for i, m in enumerate(mols[1:], 1):
+        srv.submit(i, m)
+    out += srv.drain()
+    srv.close()
+    by_id = {r["id"]: r for r in out}
+    for i, m in enumerate(mols):
+        assert {k: by_id[i][k] for k in KEYS} == engine.batch([m]).scf(threshold=1e-6)[0], i
