@@ -84,18 +84,6 @@ static __constant__ Sto6g c_sto = {
       1.094694498739664e+00, 4.078180472887439e-01, 4.191104734371579e-02}},
 };
 
-// 12-point Gauss-Hermite nodes and weights (overlap.cpp:26-37).
-static __constant__ double c_ghx[12] = {
-    -3.889724897869781919272, -3.020637025120889771711, -2.279507080501059900188,
-    -1.59768263515260479671,  -0.9477883912401637437046, -0.314240376254359111277,
-    3.889724897869781919272,  3.020637025120889771711,  2.279507080501059900188,
-    1.59768263515260479671,   0.9477883912401637437046, 0.314240376254359111277};
-static __constant__ double c_ghw[12] = {
-    2.65855168435630160602E-7, 8.5736870435878586546E-5, 0.00390539058462906185999,
-    0.05160798561588392999187, 0.2604923102641611292334, 0.5701352362624795783471,
-    2.65855168435630160602E-7, 8.5736870435878586546E-5, 0.00390539058462906185999,
-    0.05160798561588392999187, 0.2604923102641611292334, 0.5701352362624795783471};
-
 // MOPAC pair order inside a heavy atom: SS, SX, XX, SY, XY, YY, SZ, XZ, YZ, ZZ
 // (twocenterintegral.cpp:399-402); MOPIDX[a][b] is the slot of AO pair (a, b).
 __host__ __device__ constexpr int mop_a(int k) {

@@ -463,57 +463,13 @@ __global__ void k_one_center(int n_atoms, const AtomDesc* __restrict__ atoms,
 }
 
 // ---------------------------------------------------------------------------
-// STO-6G overlap of AO (atom A, type la) with AO (atom B, type lb); type
-// 0 = s, 1..3 = px, py, pz.  Overlap::ComputeOverlap with the 12-point
-// Gauss-Hermite factor (overlap.cpp:68-134, 177-192).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double gh_1d(int la, int lb, double p, double pa, double pb) {
-  const double sp = sqrt(p);
-  if (la == 0 && lb == 0) return sqrt(M_PI) / sp;
-  double s = 0.0;
-#pragma unroll
-  for (int i = 0; i < 12; ++i) {
-    const double x = c_ghx[i] / sp;
-    double term = c_ghw[i];
-    if (la) term *= (x + pa);
-    if (lb) term *= (x + pb);
-    s += term;
-  }
-  return s / sp;
-}
-
-__device__ double sto_overlap(const AtomDesc& A, int la, const AtomDesc& B, int lb) {
-  const double ac[3] = {A.x, A.y, A.z}, bc[3] = {B.x, B.y, B.z};
-  double ab = 0.0;
-  for (int i = 0; i < 3; ++i) ab += (ac[i] - bc[i]) * (ac[i] - bc[i]);
-  const int lA[3] = {la == 1, la == 2, la == 3}, lB[3] = {lb == 1, lb == 2, lb == 3};
-  double tot = 0.0;
-  for (int i = 0; i < 6; ++i)
-    for (int j = 0; j < 6; ++j) {
-      const double al = la == 0 ? c_sto.expS[A.zn][i] : c_sto.expP[A.zn][i];
-      const double be = lb == 0 ? c_sto.expS[B.zn][j] : c_sto.expP[B.zn][j];
-      const double p = al + be;
-      const double mu = al * be / p;
-      double o = exp(-mu * ab);
-      o *= la == 0 ? c_sto.cS[A.zn][i] : c_sto.cP[A.zn][i];
-      o *= lb == 0 ? c_sto.cS[B.zn][j] : c_sto.cP[B.zn][j];
-      for (int k = 0; k < 3; ++k) {
-        double P = al * ac[k];
-        P += be * bc[k];
-        P /= p;
-        o *= gh_1d(lA[k], lB[k], p, P - ac[k], P - bc[k]);
-      }
-      tot += o;
-    }
-  return tot;
-}
-
 // Off-diagonal Hcore blocks beta * S / 2 for one atom pair per lane, launched
-// per pair type so the lanes of a wave do the same work.  The AO pairs of
-// one shell pair (s-s, s-p, p-s, p-p) share their primitive exponents, so
-// exp(-mu R^2), the Gaussian centre and the Gauss-Hermite sums of each axis
-// are formed once per primitive pair and shell pair; each factor is computed
-// with exactly the operations sto_overlap uses, so the values are the same.
+// per pair type so the lanes of a wave do the same work.  The STO-6G overlap
+// is Overlap::ComputeOverlap (overlap.cpp:68-134): a sum over the 6 x 6
+// primitive pairs of exp(-mu R^2) times one Gaussian moment per axis.  The AO
+// pairs of one shell pair (s-s, s-p, p-s, p-p) share their primitive
+// exponents, so exp(-mu R^2), the Gaussian centre and the axis moments are
+// formed once per primitive pair and shell pair.
 template <int NA, int NB>
 __device__ __forceinline__ void hcore_shell(const AtomDesc& A, int sa, const AtomDesc& B, int sb,
                                             double ab, double* tot) {
@@ -527,8 +483,12 @@ __device__ __forceinline__ void hcore_shell(const AtomDesc& A, int sa, const Ato
       double o = exp(-mu * ab);
       o *= sa == 0 ? c_sto.cS[A.zn][i] : c_sto.cP[A.zn][i];
       o *= sb == 0 ? c_sto.cS[B.zn][j] : c_sto.cP[B.zn][j];
-      const double sp = sqrt(p);
-      const double i00 = sqrt(M_PI) / sp;
+      // Per axis, the reference's 12-point Gauss-Hermite sums of 1, (x+pa),
+      // (x+pb) and (x+pa)(x+pb) against exp(-p x^2) (overlap.cpp:114-124,
+      // 177-192) are exact for these degrees, so their closed forms are used:
+      // sqrt(pi/p) {1, pa, pb, pa pb + 1/(2p)}
+      const double i00 = sqrt(M_PI / p);
+      const double h2 = 0.5 / p;
       double f10[3], f01[3], f11[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -536,19 +496,9 @@ __device__ __forceinline__ void hcore_shell(const AtomDesc& A, int sa, const Ato
         P += be * bc[k];
         P /= p;
         const double pa = P - ac[k], pb = P - bc[k];
-        double s10 = 0.0, s01 = 0.0, s11 = 0.0;
-#pragma unroll
-        for (int g = 0; g < 12; ++g) {
-          const double x = c_ghx[g] / sp;
-          const double w = c_ghw[g];
-          const double t10 = w * (x + pa);
-          s10 += t10;
-          s01 += w * (x + pb);
-          s11 += t10 * (x + pb);
-        }
-        f10[k] = s10 / sp;
-        f01[k] = s01 / sp;
-        f11[k] = s11 / sp;
+        f10[k] = pa * i00;
+        f01[k] = pb * i00;
+        f11[k] = (pa * pb + h2) * i00;
       }
       // AO u of shell sa (0 = s; 1..3 = px, py, pz when sa = 1), likewise v
 #pragma unroll
