@@ -277,7 +277,7 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
                o_dT = take(sizeof(double) * vec0), o_eT = take(sizeof(double) * vec0),
                o_tn = take(sizeof(double) * n_mol);
   const size_t o_ecc = take(sizeof(double) * n_mol), o_eel = take(sizeof(double) * n_mol),
-               o_enew = take(sizeof(double) * n_mol);
+               o_enew = take(sizeof(double) * n_mol), o_eatom = take(sizeof(double) * n_atoms);
   const size_t o_steps = take(sizeof(int) * n_mol), o_act = take(sizeof(int) * n_mol),
                o_conv = take(sizeof(int) * n_mol), o_stat = take(sizeof(int) * n_mol),
                o_nact = take(sizeof(int) * 64), o_prec = take(sizeof(int) * n_mol),
@@ -312,6 +312,7 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   d.e_cc = (double*)(base + o_ecc);
   d.e_el = (double*)(base + o_eel);
   d.e_new = (double*)(base + o_enew);
+  d.e_atom = (double*)(base + o_eatom);
   d.steps = (int*)(base + o_steps);
   d.active = (int*)(base + o_act);
   d.converged = (int*)(base + o_conv);
@@ -523,7 +524,7 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
   }
   {
     StageTimer t(c, ST_ENERGY);
-    launch_energy(b->d, s, b->d.e_new, true);
+    launch_energy_fock(b->d, s, b->d.e_new, true);
   }
   {
     StageTimer t(c, ST_CONTROL);

@@ -85,25 +85,33 @@ __device__ __forceinline__ void partner(const S* __restrict__ b, const double* _
   }
 }
 
+// Writes F_AB and F_BA = H - K/2 and adds the block's share of the
+// electronic energy, 2 sum P_uv (H_uv + F_uv) (both triangles), to e.
 template <bool HA, bool HB, bool ROW, typename T>
-__device__ __forceinline__ void write_offdiag(const double* __restrict__ h, double* __restrict__ f,
-                                              int N, int ao_a, int ao_b, const T (&K)[4][4]) {
+__device__ __forceinline__ void write_offdiag(const double* __restrict__ h,
+                                              const double* __restrict__ p, double* __restrict__ f,
+                                              int N, int ao_a, int ao_b, const T (&K)[4][4],
+                                              double& e) {
   constexpr int NA = HA ? 4 : 1, NB = HB ? 4 : 1;
 #pragma unroll
   for (int x = 0; x < NA; ++x)
 #pragma unroll
     for (int y = 0; y < NB; ++y) {
       const int64_t r = ao_a + x, c = ao_b + y;
-      const double v = h[r * N + c] - 0.5 * (double)K[x][y];
+      const double hv = h[r * N + c];
+      const double v = hv - 0.5 * (double)K[x][y];
       f[r * N + c] = v;
       f[c * N + r] = v;
+      e += 2.0 * p[r * N + c] * (hv + v);
     }
 }
 
+// Returns (in every lane) atom A's share of sum P (H + F): the blocks (A, B)
+// with B before A counted twice, (A, A) once.
 template <bool HA, typename T, typename S>
-__device__ void fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc* __restrict__ atoms,
-                          const S* __restrict__ blk, const double* __restrict__ h,
-                          const double* __restrict__ p, double* __restrict__ f, int lane) {
+__device__ double fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc* __restrict__ atoms,
+                            const S* __restrict__ blk, const double* __restrict__ h,
+                            const double* __restrict__ p, double* __restrict__ f, int lane) {
   const int N = M.n_ao;
   T J[10];
 #pragma unroll
@@ -114,6 +122,7 @@ __device__ void fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc* _
 #pragma unroll
     for (int y = 0; y < 4; ++y) Kaa[x][y] = (T)0;
   bool own_diag = false;
+  double e = 0.0;
   for (int b0 = 0; b0 < M.n_atoms; b0 += 64) {
     const int bl = b0 + lane;
     if (bl >= M.n_atoms) continue;
@@ -135,9 +144,9 @@ __device__ void fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc* _
           for (int y = 0; y < 4; ++y) Kaa[x][y] = K[x][y];
         own_diag = true;
       } else if (B.zn > 1) {
-        write_offdiag<HA, true, true>(h, f, N, A.ao0, B.ao0, K);
+        write_offdiag<HA, true, true>(h, p, f, N, A.ao0, B.ao0, K, e);
       } else {
-        write_offdiag<HA, false, true>(h, f, N, A.ao0, B.ao0, K);
+        write_offdiag<HA, false, true>(h, p, f, N, A.ao0, B.ao0, K, e);
       }
     } else {
       T K[4][4];
@@ -157,9 +166,12 @@ __device__ void fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc* _
       for (int y = 0; y < NA; ++y) {
         const int64_t r = A.ao0 + x, c = A.ao0 + y;
         const double g = Jd[mop_idx(x, y)] - 0.5 * (double)Kaa[x][y];
-        f[r * N + c] = h[r * N + c] + g;
+        const double hv = h[r * N + c], fv = hv + g;
+        f[r * N + c] = fv;
+        e += p[r * N + c] * (hv + fv);
       }
   }
+  return wave_sum(e);
 }
 
 __global__ void __launch_bounds__(256) k_fock(int n_atoms, const AtomDesc* __restrict__ atoms,
@@ -168,6 +180,7 @@ __global__ void __launch_bounds__(256) k_fock(int n_atoms, const AtomDesc* __res
                                                const float* __restrict__ blocks32,
                                                const double* __restrict__ H,
                                                const double* __restrict__ P, double* __restrict__ F,
+                                               double* __restrict__ e_atom,
                                                const int* __restrict__ active,
                                                const int* __restrict__ status,
                                                const int* __restrict__ prec) {
@@ -180,15 +193,33 @@ __global__ void __launch_bounds__(256) k_fock(int n_atoms, const AtomDesc* __res
   const double* h = H + M.mat0;
   const double* p = P + M.mat0;
   double* f = F + M.mat0;
+  double e;
   if (prec && prec[A.mol]) {
     const float* blk = blocks32 + M.blk0;
-    if (A.zn > 1) fock_atom<true, float>(A, M, atoms, blk, h, p, f, lane);
-    else fock_atom<false, float>(A, M, atoms, blk, h, p, f, lane);
+    if (A.zn > 1) e = fock_atom<true, float>(A, M, atoms, blk, h, p, f, lane);
+    else e = fock_atom<false, float>(A, M, atoms, blk, h, p, f, lane);
   } else {
     const double* blk = blocks + M.blk0;
-    if (A.zn > 1) fock_atom<true, double>(A, M, atoms, blk, h, p, f, lane);
-    else fock_atom<false, double>(A, M, atoms, blk, h, p, f, lane);
+    if (A.zn > 1) e = fock_atom<true, double>(A, M, atoms, blk, h, p, f, lane);
+    else e = fock_atom<false, double>(A, M, atoms, blk, h, p, f, lane);
   }
+  if (lane == 0) e_atom[ia] = e;
+}
+
+// E_el = 1/2 sum of the atoms' shares written by k_fock, in atom order
+// (ElectronicEnergy::ComputeEnergy, electronicenergy.cpp:12-36, of the F
+// just built).
+__global__ void k_energy_atoms(int n_mol, const MolDesc* __restrict__ mols,
+                               const double* __restrict__ e_atom, double* __restrict__ out,
+                               const int* __restrict__ active, const int* __restrict__ status) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_mol) return;
+  if (active && (!active[m] || status[m])) return;
+  const MolDesc M = mols[m];
+  const double* ea = e_atom + M.atom0;
+  double s = 0.0;
+  for (int a = 0; a < M.n_atoms; ++a) s += ea[a];
+  out[m] = 0.5 * s;
 }
 
 __global__ void k_blocks_to_fp32(int64_t n, const double* __restrict__ src, float* __restrict__ dst) {
@@ -364,9 +395,15 @@ void launch_guess_density(const BatchDev& b, hipStream_t s, const int* mask) {
 
 void launch_fock(const BatchDev& b, hipStream_t s, bool all, const int* mask) {
   k_fock<<<blocks_for(b.n_atoms, 4), 256, 0, s>>>(b.n_atoms, b.atoms, b.mols, b.blocks, b.blocks32,
-                                                  b.H, b.P, b.F,
+                                                  b.H, b.P, b.F, b.e_atom,
                                                   mask ? mask : (all ? nullptr : b.active), b.status,
                                                   all ? nullptr : b.prec);
+}
+
+void launch_energy_fock(const BatchDev& b, hipStream_t s, double* out, bool all, const int* mask) {
+  k_energy_atoms<<<blocks_for(b.n_mol, 64), 64, 0, s>>>(b.n_mol, b.mols, b.e_atom, out,
+                                                        mask ? mask : (all ? nullptr : b.active),
+                                                        b.status);
 }
 
 void launch_blocks_to_fp32(const BatchDev& b, hipStream_t s) {

@@ -104,7 +104,7 @@ inline void scf_iteration(sg_context* c, const BatchDev& d, hipStream_t s, doubl
   }
   {
     StageTimer t(c, ST_ENERGY, s);
-    launch_energy(d, s, d.e_new, false);
+    launch_energy_fock(d, s, d.e_new, false);
   }
   {
     StageTimer t(c, ST_CONTROL, s);

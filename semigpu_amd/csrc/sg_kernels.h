@@ -50,6 +50,7 @@ struct BatchDev {
   double* e_cc = nullptr;    // per molecule
   double* e_el = nullptr;    // current electronic energy
   double* e_new = nullptr;   // energy of the step just computed
+  double* e_atom = nullptr;  // per atom: its share of sum P (H + F), written by the Fock build
   int* steps = nullptr;
   int* active = nullptr;     // 1 while the SCF of the molecule iterates
   int* converged = nullptr;
@@ -140,6 +141,10 @@ void launch_guess_density(const BatchDev& b, hipStream_t s, const int* mask = nu
 void launch_fock(const BatchDev& b, hipStream_t s, bool all, const int* mask = nullptr);
 void launch_energy(const BatchDev& b, hipStream_t s, double* out, bool all,
                    const int* mask = nullptr);
+// E_el of the Fock matrices just built by launch_fock (same all / mask), from
+// the per-atom shares the Fock build leaves; launch_energy reads H, F and P.
+void launch_energy_fock(const BatchDev& b, hipStream_t s, double* out, bool all,
+                        const int* mask = nullptr);
 void launch_scf_init(const BatchDev& b, hipStream_t s, int max_steps, int policy,
                      const int* mask = nullptr);
 // Zeroes the device counter of molecules still iterating (a kernel rather than

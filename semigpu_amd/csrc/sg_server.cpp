@@ -195,7 +195,7 @@ namespace {
 // Per-lane device memory: slot descriptors and scalars, atoms, pair scratch.
 size_t lane_bytes(int max_slots, int max_atoms, int64_t pair_cap) {
   const size_t ns = max_slots + 1;
-  return sizeof(MolDesc) * ns + sizeof(AtomDesc) * max_atoms + sizeof(int2) * pair_cap +
+  return sizeof(MolDesc) * ns + (sizeof(AtomDesc) + 8) * max_atoms + sizeof(int2) * pair_cap +
          ns * (5 * 8 + 9 * 4) + 256 + 16 * 256;
 }
 
@@ -211,6 +211,7 @@ int lane_create(const BatchDev& shared, Lane* L, int max_slots, int max_atoms, i
   };
   const size_t ns = max_slots + 1;  // + dead slot
   const size_t o_mols = take(sizeof(MolDesc) * ns), o_atoms = take(sizeof(AtomDesc) * max_atoms);
+  const size_t o_eatom = take(8 * (size_t)max_atoms);
   const size_t o_pairs = take(sizeof(int2) * L->pair_cap);
   const size_t o_tn = take(8 * ns), o_ecc = take(8 * ns), o_eel = take(8 * ns), o_enew = take(8 * ns);
   const size_t o_steps = take(4 * ns), o_act = take(4 * ns), o_conv = take(4 * ns),
@@ -229,6 +230,7 @@ int lane_create(const BatchDev& shared, Lane* L, int max_slots, int max_atoms, i
   d = shared;
   d.mols = (MolDesc*)(base + o_mols);
   d.atoms = (AtomDesc*)(base + o_atoms);
+  d.e_atom = (double*)(base + o_eatom);
   L->pair_scratch = (int2*)(base + o_pairs);
   d.tnorm = (double*)(base + o_tn);
   d.e_cc = (double*)(base + o_ecc);
@@ -469,7 +471,7 @@ int lane_launch(sg_server* S, Lane* L) {
     }
     {
       StageTimer t(c, ST_ENERGY, s);
-      launch_energy(nb, s, nb.e_new, true, L->mask_new);
+      launch_energy_fock(nb, s, nb.e_new, true, L->mask_new);
     }
     {
       StageTimer t(c, ST_CONTROL, s);
