@@ -10,7 +10,9 @@
 //   u on A, v on B:  G_uv = -1/2 sum_{i in A, j in B} P_ij (ui|vj)
 // with w_B(l) = P_l (x2 for the off-diagonal AO pairs of B).
 //
-// k_fock: one wavefront per atom A, lanes over the partner atoms B.  Each lane
+// k_fock: one wavefront per atom A, lanes over the partner atoms B (heavy
+// atoms first, then hydrogens, so that a pass of 64 lanes mostly runs one
+// block shape).  Each lane
 // reads block (A,B) once, adds its Coulomb term to a per-lane J_A, and -- for
 // B below A in the molecule -- forms the exchange block K_AB and writes F_AB and
 // F_BA.  J_A is reduced across the wave with a fixed butterfly, so every run
@@ -108,10 +110,13 @@ __device__ __forceinline__ void write_offdiag(const double* __restrict__ h,
 
 // Returns (in every lane) atom A's share of sum P (H + F): the blocks (A, B)
 // with B before A counted twice, (A, A) once.
+// Partners are taken in the molecule's partner order (heavy atoms, then
+// hydrogens), so the lanes of one pass mostly share a block shape.
 template <bool HA, typename T, typename S>
 __device__ double fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc* __restrict__ atoms,
-                            const S* __restrict__ blk, const double* __restrict__ h,
-                            const double* __restrict__ p, double* __restrict__ f, int lane) {
+                            const int* __restrict__ partners, const S* __restrict__ blk,
+                            const double* __restrict__ h, const double* __restrict__ p,
+                            double* __restrict__ f, int lane) {
   const int N = M.n_ao;
   T J[10];
 #pragma unroll
@@ -126,7 +131,7 @@ __device__ double fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc*
   for (int b0 = 0; b0 < M.n_atoms; b0 += 64) {
     const int bl = b0 + lane;
     if (bl >= M.n_atoms) continue;
-    const AtomDesc B = atoms[M.atom0 + bl];
+    const AtomDesc B = atoms[partners[M.atom0 + bl]];
     const bool row = B.local <= A.local;
     const S* b = blk + (row ? block_offset(A, B) : block_offset(B, A));
     if (row) {
@@ -175,6 +180,7 @@ __device__ double fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc*
 }
 
 __global__ void __launch_bounds__(256) k_fock(int n_atoms, const AtomDesc* __restrict__ atoms,
+                                               const int* __restrict__ partners,
                                                const MolDesc* __restrict__ mols,
                                                const double* __restrict__ blocks,
                                                const float* __restrict__ blocks32,
@@ -196,12 +202,12 @@ __global__ void __launch_bounds__(256) k_fock(int n_atoms, const AtomDesc* __res
   double e;
   if (prec && prec[A.mol]) {
     const float* blk = blocks32 + M.blk0;
-    if (A.zn > 1) e = fock_atom<true, float>(A, M, atoms, blk, h, p, f, lane);
-    else e = fock_atom<false, float>(A, M, atoms, blk, h, p, f, lane);
+    if (A.zn > 1) e = fock_atom<true, float>(A, M, atoms, partners, blk, h, p, f, lane);
+    else e = fock_atom<false, float>(A, M, atoms, partners, blk, h, p, f, lane);
   } else {
     const double* blk = blocks + M.blk0;
-    if (A.zn > 1) e = fock_atom<true, double>(A, M, atoms, blk, h, p, f, lane);
-    else e = fock_atom<false, double>(A, M, atoms, blk, h, p, f, lane);
+    if (A.zn > 1) e = fock_atom<true, double>(A, M, atoms, partners, blk, h, p, f, lane);
+    else e = fock_atom<false, double>(A, M, atoms, partners, blk, h, p, f, lane);
   }
   if (lane == 0) e_atom[ia] = e;
 }
@@ -394,7 +400,8 @@ void launch_guess_density(const BatchDev& b, hipStream_t s, const int* mask) {
 }
 
 void launch_fock(const BatchDev& b, hipStream_t s, bool all, const int* mask) {
-  k_fock<<<blocks_for(b.n_atoms, 4), 256, 0, s>>>(b.n_atoms, b.atoms, b.mols, b.blocks, b.blocks32,
+  k_fock<<<blocks_for(b.n_atoms, 4), 256, 0, s>>>(b.n_atoms, b.atoms, b.partners, b.mols, b.blocks,
+                                                  b.blocks32,
                                                   b.H, b.P, b.F, b.e_atom,
                                                   mask ? mask : (all ? nullptr : b.active), b.status,
                                                   all ? nullptr : b.prec);

@@ -29,6 +29,9 @@ struct BatchDev {
   int64_t n_blocks = 0;  // sum of pair-block doubles
   MolDesc* mols = nullptr;
   AtomDesc* atoms = nullptr;
+  // per molecule, over its atom range: global atom indices of its heavy atoms,
+  // then its hydrogens (the Fock build's partner order)
+  int* partners = nullptr;
   // type-sorted atom-pair lists (row atom, column atom), row local > column local
   int2* pairs_xx = nullptr; int n_xx = 0;
   int2* pairs_hx = nullptr; int n_hx = 0;
