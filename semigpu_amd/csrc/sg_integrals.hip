@@ -565,7 +565,7 @@ __global__ void __launch_bounds__(128) k_hcore_offdiag(const int2* __restrict__ 
       hcore_shell<1, 1>(A, 0, B, 0, ab, ss);
       hcore_shell<3, 1>(A, 1, B, 0, ab, ps);
     } else {
-      // first atom is the hydrogen: S(H s, X ao) as sto_overlap(H, 0, X, v)
+      // first atom is the hydrogen: S(H s, X ao), the reference's argument order
       double sp[3] = {0.0, 0.0, 0.0};
       hcore_shell<1, 1>(B, 0, A, 0, ab, ss);
       hcore_shell<1, 3>(B, 0, A, 1, ab, sp);
