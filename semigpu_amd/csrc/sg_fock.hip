@@ -112,20 +112,17 @@ __device__ __forceinline__ void write_offdiag(const double* __restrict__ h,
 // with B before A counted twice, (A, A) once.
 // Partners are taken in the molecule's partner order (heavy atoms, then
 // hydrogens), so the lanes of one pass mostly share a block shape.
+// kaa: 16 doubles of LDS for this wave (the exchange block K_AA, kept out of
+// registers while the other partners run).
 template <bool HA, typename T, typename S>
 __device__ double fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc* __restrict__ atoms,
                             const int* __restrict__ partners, const S* __restrict__ blk,
                             const double* __restrict__ h, const double* __restrict__ p,
-                            double* __restrict__ f, int lane) {
+                            double* __restrict__ f, double* __restrict__ kaa, int lane) {
   const int N = M.n_ao;
   T J[10];
 #pragma unroll
   for (int k = 0; k < 10; ++k) J[k] = (T)0;
-  T Kaa[4][4];
-#pragma unroll
-  for (int x = 0; x < 4; ++x)
-#pragma unroll
-    for (int y = 0; y < 4; ++y) Kaa[x][y] = (T)0;
   bool own_diag = false;
   double e = 0.0;
   for (int b0 = 0; b0 < M.n_atoms; b0 += 64) {
@@ -146,7 +143,7 @@ __device__ double fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc*
 #pragma unroll
         for (int x = 0; x < 4; ++x)
 #pragma unroll
-          for (int y = 0; y < 4; ++y) Kaa[x][y] = K[x][y];
+          for (int y = 0; y < 4; ++y) kaa[x * 4 + y] = (double)K[x][y];
         own_diag = true;
       } else if (B.zn > 1) {
         write_offdiag<HA, true, true>(h, p, f, N, A.ao0, B.ao0, K, e);
@@ -170,7 +167,7 @@ __device__ double fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc*
 #pragma unroll
       for (int y = 0; y < NA; ++y) {
         const int64_t r = A.ao0 + x, c = A.ao0 + y;
-        const double g = Jd[mop_idx(x, y)] - 0.5 * (double)Kaa[x][y];
+        const double g = Jd[mop_idx(x, y)] - 0.5 * kaa[x * 4 + y];
         const double hv = h[r * N + c], fv = hv + g;
         f[r * N + c] = fv;
         e += p[r * N + c] * (hv + fv);
@@ -190,6 +187,7 @@ __global__ void __launch_bounds__(256) k_fock(int n_atoms, const AtomDesc* __res
                                                const int* __restrict__ active,
                                                const int* __restrict__ status,
                                                const int* __restrict__ prec) {
+  __shared__ double kaa_lds[4][16];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ia = blockIdx.x * 4 + wave;
   if (ia >= n_atoms) return;
@@ -199,15 +197,16 @@ __global__ void __launch_bounds__(256) k_fock(int n_atoms, const AtomDesc* __res
   const double* h = H + M.mat0;
   const double* p = P + M.mat0;
   double* f = F + M.mat0;
+  double* kaa = kaa_lds[wave];
   double e;
   if (prec && prec[A.mol]) {
     const float* blk = blocks32 + M.blk0;
-    if (A.zn > 1) e = fock_atom<true, float>(A, M, atoms, partners, blk, h, p, f, lane);
-    else e = fock_atom<false, float>(A, M, atoms, partners, blk, h, p, f, lane);
+    if (A.zn > 1) e = fock_atom<true, float>(A, M, atoms, partners, blk, h, p, f, kaa, lane);
+    else e = fock_atom<false, float>(A, M, atoms, partners, blk, h, p, f, kaa, lane);
   } else {
     const double* blk = blocks + M.blk0;
-    if (A.zn > 1) e = fock_atom<true, double>(A, M, atoms, partners, blk, h, p, f, lane);
-    else e = fock_atom<false, double>(A, M, atoms, partners, blk, h, p, f, lane);
+    if (A.zn > 1) e = fock_atom<true, double>(A, M, atoms, partners, blk, h, p, f, kaa, lane);
+    else e = fock_atom<false, double>(A, M, atoms, partners, blk, h, p, f, kaa, lane);
   }
   if (lane == 0) e_atom[ia] = e;
 }
@@ -401,8 +400,7 @@ void launch_guess_density(const BatchDev& b, hipStream_t s, const int* mask) {
 
 void launch_fock(const BatchDev& b, hipStream_t s, bool all, const int* mask) {
   k_fock<<<blocks_for(b.n_atoms, 4), 256, 0, s>>>(b.n_atoms, b.atoms, b.partners, b.mols, b.blocks,
-                                                  b.blocks32,
-                                                  b.H, b.P, b.F, b.e_atom,
+                                                  b.blocks32, b.H, b.P, b.F, b.e_atom,
                                                   mask ? mask : (all ? nullptr : b.active), b.status,
                                                   all ? nullptr : b.prec);
 }
