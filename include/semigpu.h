@@ -170,9 +170,10 @@ int sg_server_create(sg_context* ctx, double hbm_bytes, int max_slots, int max_a
 /* Same with n_classes size classes: class i takes N in [bounds[i-1], bounds[i])
  * (bounds[-1] = 0, the last class up to the largest N the server accepts, 1280);
  * bounds[] has n_classes - 1 increasing entries.  sg_server_create is one class.
- * Each class has its own stream; HIP maps streams onto a few hardware queues
- * per process (4 by default) and streams that share one run in sequence, so
- * classes beyond the free queues lose their independence. */
+ * Each class has its own stream; HIP maps streams onto GPU_MAX_HW_QUEUES
+ * hardware queues per process (4 by default) and streams that share one run
+ * in sequence, so with more than one or two classes raise it (8 gives each
+ * class its own queue). */
 int sg_server_create_classes(sg_context* ctx, double hbm_bytes, int max_slots, int max_atoms,
                              double threshold_ev, int max_steps, int policy, double switch_ev,
                              int n_classes, const int* bounds, sg_server** out);
