@@ -153,8 +153,11 @@ struct Mfma16<float> {
 // Lower-triangle storage: only A[i][l], l <= i, is read and updated; reflector
 // c is kept in the (otherwise unused) upper part of row c.  TMAX bounds the
 // number of 64-column chunks of a row (n <= 64*TMAX).
+// Occupancy: two workgroups per CU for TMAX = 6 (N <= 384, 80 KB of LDS);
+// above that the LDS already holds one workgroup per CU, so the register
+// budget is left at two waves per SIMD (no spills of the 20 row accumulators).
 template <int TMAX>
-__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(4))) k_tridiag(const MolDesc* __restrict__ mols,
+__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(TMAX <= 6 ? 4 : 2))) k_tridiag(const MolDesc* __restrict__ mols,
                                                  const double* __restrict__ F,
                                                  double* __restrict__ work,
                                                  double* __restrict__ dA, double* __restrict__ eA,
