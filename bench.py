@@ -8,10 +8,11 @@ every molecule until all have converged.  value = molecules converged within
 1e-4 eV of the batched FP64 result / step time.
 
 Printed JSON (one line): the driver's contract fields plus
-  roofline          the dominant kernel (Householder tridiagonalisation) rate:
-                    algorithmic FP64 flops per launch / average launch time, the
-                    time from HIP events around each launch on the library's
-                    stream during the timed steps
+  roofline          the dominant kernel (Householder tridiagonalisation, FP64
+                    launch) rate: algorithmic flops of its FP64 molecules per
+                    launch / average launch time, the time from HIP events around
+                    each launch on the library's stream during the timed steps
+  tridiag_fp32_roofline  the same for the FP32 launch of the mixed policy
   fock_roofline     the Fock build's HBM rate (algorithmic bytes / kernel time)
   density_roofline  the density product's flop rate (algorithmic flops / time)
   plain_fp64        one molecule at a time through the same kernels (subsample)
@@ -276,22 +277,40 @@ def main():
     conv = np.array([r["converged"] for r in res])
 
     # Algorithmic work per launch = work of one timed step / launches of one step.
+    # The FP64 reduction launch takes each SCF step's FP64 molecules (FP32 steps
+    # of the mixed policy go to their own launch, timed as tridiag_fp32, unless
+    # the batch is small enough for one launch to take both).
     K = max(1, args.steps)
     N3 = sizes.astype(np.float64) ** 3
+    s32 = b.fp32_steps().astype(np.float64)
     tri_ms, tri_cnt = stage["tridiag"]
-    tri_flops_step = float(np.sum(steps * (4.0 / 3.0) * N3))
+    t32_ms, t32_cnt = stage["tridiag_fp32"]
+    steps64 = steps - s32 if t32_cnt else steps
+    tri_flops_step = float(np.sum(steps64 * (4.0 / 3.0) * N3))
     tri_avg_ms = tri_ms / max(1, tri_cnt)
     tri_flops_launch = tri_flops_step * K / max(1, tri_cnt)
     tri_tflops = tri_flops_launch / (tri_avg_ms * 1e-3) / 1e12
     traffic = pmc_traffic()
-    roofline = {"kernel": "k_tridiag", "bound": "mfma", "achieved": round(tri_tflops, 4),
+    roofline = {"kernel": "k_tridiag<6, double>", "bound": "mfma", "achieved": round(tri_tflops, 4),
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(tri_tflops / FP64_PEAK_TFLOPS, 6), "traffic": traffic,
                 "avg_launch_ms": round(tri_avg_ms, 4),
                 "flops_per_launch": tri_flops_launch,
-                "work": "(4/3) N^3 FP64 flops per active molecule per launch",
+                "work": "(4/3) N^3 flops per molecule in an FP64 step, per launch",
                 "traffic_rate_GBs": (round(traffic / (tri_avg_ms * 1e-3) / 1e9, 1)
                                      if traffic else None)}
+    tri32_roof = None
+    if t32_cnt:
+        t32_avg_ms = t32_ms / t32_cnt
+        t32_flops_launch = float(np.sum(s32 * (4.0 / 3.0) * N3)) * K / t32_cnt
+        t32_tflops = t32_flops_launch / (t32_avg_ms * 1e-3) / 1e12
+        tri32_roof = {"kernel": "k_tridiag<6, float>", "bound": "mfma",
+                      "achieved": round(t32_tflops, 4), "peak": FP32_PEAK_TFLOPS,
+                      "unit": "TFLOP/s", "frac": round(t32_tflops / FP32_PEAK_TFLOPS, 6),
+                      "avg_launch_ms": round(t32_avg_ms, 4),
+                      "flops_per_launch": t32_flops_launch,
+                      "fp32_steps_mean": round(float(s32.mean()), 3),
+                      "work": "(4/3) N^3 flops per molecule in an FP32 step, per launch"}
     T = sizes * (sizes + 1) / 2
     I = np.array([pair_block_doubles(m) for m in mols], dtype=np.float64)
     fock_ms, fock_cnt = stage["fock"]
@@ -365,10 +384,12 @@ def main():
                    "molecules": len(mols), "basis_min": int(sizes.min()),
                    "basis_max": int(sizes.max()),
                    "precision_policy": ("fp64 throughout" if args.policy == "fp64" else
-                                        "fp32 Fock/density (MFMA) until |dE|<1e-3 eV, then fp64; "
-                                        "integrals, eigensolver, energies fp64"),
+                                        "fp32 Fock, tridiagonal reduction, back-transform and "
+                                        "density until |dE|<1e-3 eV, then fp64; integrals, "
+                                        "bisection, inverse iteration, energies fp64"),
                    "scf_threshold_eV": args.threshold},
         "roofline": roofline,
+        "tridiag_fp32_roofline": tri32_roof,
         "fock_roofline": fock_roof,
         "density_roofline": den_roof,
         "stage_ms_per_step": stage_ms,

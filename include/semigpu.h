@@ -108,6 +108,10 @@ int sg_scf_run(sg_batch* b, double threshold_ev, int max_steps, sg_scf_result* r
 #define SG_POLICY_MIXED 1
 int sg_scf_run_policy(sg_batch* b, double threshold_ev, int max_steps, int policy,
                       double switch_ev, sg_scf_result* results);
+/* FP32 steps of each molecule in the last sg_scf_run_policy (0 under
+ * SG_POLICY_FP64); steps32 gets n_mol entries.  No counterpart in the
+ * reference: it splits the steps for per-precision rates. */
+int sg_scf_fp32_steps(const sg_batch* b, int* steps32);
 
 /* Per-molecule eigensolver status of the last sg_eigh / SCF step
  * (status[n_mol]; > 0 where dsyev would report info > 0). */
@@ -194,8 +198,11 @@ double sg_server_clock(const sg_server* srv);
 /* Per-stage GPU time (HIP events on the context stream), milliseconds summed
  * since the last reset; stage order: two_center, hcore, guess, fock, tridiag,
  * bisect, invit, backtransform (the eigensolver's four kernels), density,
- * energy, control.  counts[] gets the number of launches timed. */
-#define SG_N_STAGES 11
+ * energy, control, tridiag_fp32.  tridiag is the FP64 reduction launch of each
+ * eigensolve (with the FP32 molecules too when every workgroup is resident at
+ * once); tridiag_fp32 the separate FP32 launch of the mixed policy's larger
+ * batches.  counts[] gets the number of launches timed. */
+#define SG_N_STAGES 12
 /* One plain device-to-device copy kernel (k_calibration_copy) over freshly
  * allocated buffers: reads and writes exactly `bytes` (a multiple of 16).  No
  * counterpart in the reference: it is the known-size kernel that hardware byte

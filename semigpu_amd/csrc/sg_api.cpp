@@ -49,7 +49,7 @@ struct sg_batch {
   std::vector<MolDesc> large_mols;  // their descriptors
   hipGraphExec_t iter_exec = nullptr;  // captured SCF iteration
   double g_threshold = -1.0, g_switch = -1.0;
-  int g_max_steps = -1;
+  int g_max_steps = -1, g_mixed = -1;
 };
 
 // Plain copy, 16 bytes per lane per step: reads and writes exactly n16 * 16 bytes.
@@ -282,7 +282,8 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   const size_t o_steps = take(sizeof(int) * n_mol), o_act = take(sizeof(int) * n_mol),
                o_conv = take(sizeof(int) * n_mol), o_stat = take(sizeof(int) * n_mol),
                o_nact = take(sizeof(int) * 64), o_prec = take(sizeof(int) * n_mol),
-               o_pprec = take(sizeof(int) * n_mol), o_order = take(sizeof(int) * n_mol);
+               o_pprec = take(sizeof(int) * n_mol), o_order = take(sizeof(int) * n_mol),
+               o_s32 = take(sizeof(int) * n_mol);
   (void)hipSetDevice(ctx->device);
   hipError_t e = hipMalloc(&b->dev_mem, off);
   if (e != hipSuccess) {
@@ -322,6 +323,7 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   d.n_active = (int*)(base + o_nact);
   d.prec = (int*)(base + o_prec);
   d.prev_prec = (int*)(base + o_pprec);
+  d.steps32 = (int*)(base + o_s32);
   int* order = (int*)(base + o_order);
   d.order = order;
   hipStream_t s = ctx->stream;
@@ -369,6 +371,7 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   if (e == hipSuccess) e = hipMemsetAsync(d.status, 0, sizeof(int) * n_mol, s);
   if (e == hipSuccess) e = hipMemsetAsync(d.active, 0, sizeof(int) * n_mol, s);
   if (e == hipSuccess) e = hipMemsetAsync(d.prec, 0, sizeof(int) * n_mol, s);
+  if (e == hipSuccess) e = hipMemsetAsync(d.steps32, 0, sizeof(int) * n_mol, s);
   if (e == hipSuccess) e = hipMemsetAsync(d.prev_prec, 0, sizeof(int) * n_mol, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
@@ -507,6 +510,15 @@ int sg_scf_run(sg_batch* b, double threshold, int max_steps, sg_scf_result* res)
   return sg_scf_run_policy(b, threshold, max_steps, SG_POLICY_FP64, 1e-3, res);
 }
 
+int sg_scf_fp32_steps(const sg_batch* b, int* steps32) {
+  if (!b || !steps32) return fail(SG_ERR_INVALID, "null batch or output");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  HIP_TRY(hipMemcpyAsync(steps32, b->d.steps32, sizeof(int) * b->d.n_mol, hipMemcpyDeviceToHost,
+                         b->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  return SG_OK;
+}
+
 int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, double switch_ev,
                       sg_scf_result* res) {
   if (!b || max_steps < 0 || !(threshold >= 0.0) || (policy != SG_POLICY_FP64 &&
@@ -551,9 +563,10 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
   // replayed; its kernel arguments depend on the batch and on these settings.
   // Batches with molecules on the multi-workgroup eigensolver (thousands of
   // launches per iteration) launch directly.
+  b->d.mixed = policy == SG_POLICY_MIXED ? 1 : 0;
   const bool use_graph = !c->timing && b->d.n_large == 0;
   if (use_graph && (!b->iter_exec || b->g_threshold != threshold || b->g_max_steps != max_steps ||
-                    b->g_switch != switch_ev)) {
+                    b->g_switch != switch_ev || b->g_mixed != b->d.mixed)) {
     if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
     b->iter_exec = nullptr;
     hipGraph_t g = nullptr;
@@ -573,6 +586,7 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
     b->g_threshold = threshold;
     b->g_max_steps = max_steps;
     b->g_switch = switch_ev;
+    b->g_mixed = b->d.mixed;
   }
   for (int it = 0; it < max_steps && n_active > 0; ++it) {
     if (use_graph)

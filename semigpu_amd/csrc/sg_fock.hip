@@ -315,13 +315,14 @@ __global__ void k_scf_init(int n_mol, int max_steps, int policy, const double* _
                            int* __restrict__ active, int* __restrict__ converged,
                            int* __restrict__ status, int* __restrict__ n_active,
                            int* __restrict__ prec, int* __restrict__ prev_prec,
-                           const int* __restrict__ mask) {
+                           int* __restrict__ steps32, const int* __restrict__ mask) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m == 0 && !mask) *n_active = 0;
   if (m >= n_mol) return;
   if (mask && !mask[m]) return;
   e_el[m] = e0[m];
   steps[m] = 0;
+  if (steps32) steps32[m] = 0;
   active[m] = max_steps > 0 ? 1 : 0;
   converged[m] = 0;
   status[m] = 0;
@@ -345,7 +346,8 @@ __global__ void k_scf_update(int n_mol, double threshold, int max_steps, double 
                              double* __restrict__ e_el, int* __restrict__ steps,
                              int* __restrict__ active, int* __restrict__ converged,
                              const int* __restrict__ status, int* __restrict__ n_active,
-                             int* __restrict__ prec, int* __restrict__ prev_prec) {
+                             int* __restrict__ prec, int* __restrict__ prev_prec,
+                             int* __restrict__ steps32) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n_mol || !active[m]) return;
   if (status[m]) {
@@ -358,6 +360,7 @@ __global__ void k_scf_update(int n_mol, double threshold, int max_steps, double 
   steps[m] = st;
   const double de = fabs(old - e);
   const int step_prec = prec[m];
+  if (steps32 && step_prec) steps32[m] += 1;
   const int conv = step_prec == 0 && prev_prec[m] == 0 && de < threshold;
   prev_prec[m] = step_prec;
   // FP32 steps end at |dE| < switch_ev, or once |dE| is below what FP32
@@ -440,7 +443,8 @@ void launch_scf_init(const BatchDev& b, hipStream_t s, int max_steps, int policy
                      const int* mask) {
   k_scf_init<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(b.n_mol, max_steps, policy, b.e_new, b.e_el,
                                                       b.steps, b.active, b.converged, b.status,
-                                                      b.n_active, b.prec, b.prev_prec, mask);
+                                                      b.n_active, b.prec, b.prev_prec, b.steps32,
+                                                      mask);
 }
 
 void launch_set_prec(const BatchDev& b, hipStream_t s, int v, const int* mask) {
@@ -452,7 +456,7 @@ void launch_scf_update(const BatchDev& b, hipStream_t s, double threshold, int m
   k_scf_update<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(b.n_mol, threshold, max_steps, switch_ev,
                                                         fp32_cap, b.e_new, b.e_el, b.steps,
                                                         b.active, b.converged, b.status, b.n_active,
-                                                        b.prec, b.prev_prec);
+                                                        b.prec, b.prev_prec, b.steps32);
 }
 
 }  // namespace sg

@@ -22,9 +22,9 @@ int fail(int code, const std::string& msg);
 
 enum {
   ST_TWO_CENTER = 0, ST_HCORE, ST_GUESS, ST_FOCK, ST_TRIDIAG, ST_BISECT, ST_INVIT, ST_BACK,
-  ST_DENSITY, ST_ENERGY, ST_CONTROL
+  ST_DENSITY, ST_ENERGY, ST_CONTROL, ST_TRIDIAG32
 };
-static_assert(ST_CONTROL + 1 == SG_N_STAGES, "stage list and SG_N_STAGES disagree");
+static_assert(ST_TRIDIAG32 + 1 == SG_N_STAGES, "stage list and SG_N_STAGES disagree");
 
 int check_launch(const char* what);
 bool supported_element(int z);
@@ -75,7 +75,11 @@ inline void scf_iteration(sg_context* c, const BatchDev& d, hipStream_t s, doubl
 inline void timed_eigh(sg_context* c, const BatchDev& d, hipStream_t s, bool all) {
   {
     StageTimer t(c, ST_TRIDIAG, s);
-    launch_eigh_tridiag(d, s, all);
+    launch_eigh_tridiag(d, s, all, 0);
+  }
+  if (tridiag_fp32_launch(d, all)) {
+    StageTimer t(c, ST_TRIDIAG32, s);
+    launch_eigh_tridiag(d, s, all, 1);
   }
   {
     StageTimer t(c, ST_BISECT, s);

@@ -61,6 +61,10 @@ struct BatchDev {
   int* n_active = nullptr;   // device counter
   int* prec = nullptr;       // precision of the next SCF step: 1 = FP32 contractions, 0 = FP64
   int* prev_prec = nullptr;  // precision of the step just done
+  int* steps32 = nullptr;    // FP32 steps of the current SCF per molecule (null: not counted)
+  // The SCF runs the mixed-precision policy: only then can a step's molecules
+  // be in FP32, and the eigensolver launches its FP32 kernels.
+  int mixed = 0;
   int max_n = 0;             // largest N in the batch
   // Molecule indices by decreasing N (or null for 0..n_mol-1): the per-molecule
   // eigensolver and density launches take their workgroups in this order, so
@@ -160,8 +164,12 @@ void launch_set_prec(const BatchDev& b, hipStream_t s, int v, const int* mask = 
 void launch_clear_hf(const BatchDev& b, hipStream_t s, const int* mask);
 // Eigensolver of F (all = every eigenpair, else the selected subspace of the
 // active molecules): tridiagonalisation, bisection, inverse iteration,
-// back-transformation; launch_eigh runs the four in order.
-void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all);
+// back-transformation; launch_eigh runs the four in order.  The reduction's
+// launches: which = 0 the FP64 one (both types when all workgroups are
+// resident at once) and the multi-workgroup molecules, 1 the separate FP32
+// one (tridiag_fp32_launch: whether there is one), -1 all.
+bool tridiag_fp32_launch(const BatchDev& b, bool all);
+void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all, int which);
 void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all);
 void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all);
 void launch_eigh_back(const BatchDev& b, hipStream_t s, bool all);

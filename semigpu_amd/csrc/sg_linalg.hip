@@ -39,6 +39,8 @@
 namespace sg {
 
 constexpr int kT = 512;                 // threads per eigensolver workgroup
+constexpr int kDeviceCUs = 256;         // compute units of an MI355X
+static_assert(kT <= 512, "the tridiagonal reduction's block sums hold 8 wave slots");
 constexpr int kPanelLds = 48 * 1024;    // bytes of LDS for the V/W panel
 constexpr int kNbMax = 8;               // widest reflector panel
 constexpr int kPanelDoubles = kPanelLds / 8;
@@ -96,11 +98,14 @@ __device__ __forceinline__ double wsum(double v) {
   return v + w;
 }
 
-// Block-wide sum in a fixed order; every thread gets the result.
+// Block-wide sum in a fixed order with one barrier; every thread gets the
+// result.  red (one slot per wave) is rewritten by the next sum that uses it,
+// so consecutive sums must alternate between two such buffers: a buffer is
+// then written again only after a barrier that every thread reaches after
+// reading it.
 __device__ double block_sum(double v, double* red) {
   v = wsum(v);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, nw = blockDim.x >> 6;
-  __syncthreads();
   if (l == 0) red[w] = v;
   __syncthreads();
   double t = 0.0;
@@ -108,21 +113,62 @@ __device__ double block_sum(double v, double* red) {
   return t;
 }
 
+// Reflector panel: V and W (nb columns of length n each) fill at most
+// panel_bytes<R>() of LDS, the same width in either type.
+template <typename R>
+__host__ __device__ constexpr int panel_bytes() {
+  return kPanelLds / 8 * (int)sizeof(R);
+}
+template <typename R>
 __device__ __forceinline__ int panel_width(int n) {
-  int nb = kPanelLds / (16 * (n > 0 ? n : 1));
+  int nb = panel_bytes<R>() / (2 * (int)sizeof(R) * (n > 0 ? n : 1));
   return nb < 1 ? 1 : (nb > kNbMax ? kNbMax : nb);
 }
 
 // Raw buffer access to one molecule's matrix: offsets are 32-bit byte offsets
 // from the matrix start, and loads past its last element return 0.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const double* base, int64_t count) {
-  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)(count * 8), 0x00020000);
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, 0x00020000);
 }
 __device__ __forceinline__ double load_f64(__amdgpu_buffer_rsrc_t r, int byte_off) {
   const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0);
   return __longlong_as_double(((long long)v[1] << 32) | (unsigned int)v[0]);
 }
-constexpr int kSymvGroup = 2;  // 64-column tiles whose loads are issued together
+template <typename R>
+__device__ __forceinline__ R load_r(__amdgpu_buffer_rsrc_t r, int byte_off);
+template <>
+__device__ __forceinline__ double load_r<double>(__amdgpu_buffer_rsrc_t r, int byte_off) {
+  return load_f64(r, byte_off);
+}
+template <>
+__device__ __forceinline__ float load_r<float>(__amdgpu_buffer_rsrc_t r, int byte_off) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
+}
+// 32-bit forms of the cross-lane moves, and a DPP move for either type
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_r(double v) { return dpp_f64<CTRL>(v); }
+template <int CTRL>
+__device__ __forceinline__ float dpp_r(float v) { return dpp_f32<CTRL>(v); }
+__device__ __forceinline__ void swap32(float& x, float& y) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(x), __float_as_int(y), false, false);
+  x = __int_as_float(r[0]);
+  y = __int_as_float(r[1]);
+}
+__device__ __forceinline__ void swap16(float& x, float& y) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_int(x), __float_as_int(y), false, false);
+  x = __int_as_float(r[0]);
+  y = __int_as_float(r[1]);
+}
+// 64-column tiles of a row block whose loads are issued together (3 or 6
+// in either type measured slower: more registers, no fewer stalls)
+template <int TMAX, typename R>
+constexpr int symv_group() {
+  return 2;
+}
 constexpr int kOutside = 0x7ffffff0;  // byte offset past any matrix buffer
 
 // Operand layout of the 16x16x4 matrix-core instructions: lane l supplies
@@ -152,62 +198,61 @@ struct Mfma16<float> {
 // ---------------------------------------------------------------------------
 // Lower-triangle storage: only A[i][l], l <= i, is read and updated; reflector
 // c is kept in the (otherwise unused) upper part of row c.  TMAX bounds the
-// number of 64-column chunks of a row (n <= 64*TMAX).
-// Occupancy: two workgroups per CU for TMAX = 6 (N <= 384, 80 KB of LDS);
-// above that the LDS already holds one workgroup per CU, so the register
-// budget is left at two waves per SIMD (no spills of the 20 row accumulators).
-template <int TMAX>
-__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(TMAX <= 6 ? 4 : 2))) k_tridiag(const MolDesc* __restrict__ mols,
-                                                 const double* __restrict__ F,
-                                                 double* __restrict__ work,
-                                                 double* __restrict__ dA, double* __restrict__ eA,
-                                                 double* __restrict__ tauA,
-                                                 const int* __restrict__ active,
-                                                 const int* __restrict__ status,
-                                                 const int* __restrict__ order) {
-  extern __shared__ double lds[];
-  const int m = order ? order[blockIdx.x] : blockIdx.x;
-  if (!mol_live(active, status, m)) return;
-  const MolDesc M = mols[m];
+// number of 64-column chunks of a row (n <= 64*TMAX).  R is the working type:
+// double, or float for the FP32 steps of the mixed-precision policy (matrix,
+// panel and products in FP32; norms, the reflector scalars and d, e, tau in
+// FP64).  LDS: the V/W panel (panel_bytes), x (n R, rounded up to an even
+// count), 16 + 16 doubles (V^T v / W^T v, block reductions), the wavefronts'
+// partial transposed products (8 n R), the next column (n R, even count) and
+// 3 kNbMax doubles (the panel's d, e, tau).
+template <int TMAX, typename R>
+__device__ __forceinline__ void tridiag_mol(const MolDesc& M, const double* __restrict__ f,
+                                            R* __restrict__ A, double* __restrict__ d,
+                                            double* __restrict__ e, double* __restrict__ tau,
+                                            unsigned char* L) {
+  typedef Mfma16<R> Mf;
   const int n = M.n_ao, tid = threadIdx.x, nt = blockDim.x;
   const int wave = tid >> 6, lane = tid & 63, nwave = nt >> 6;
-  const int nb = panel_width(n);
-  double* V = lds;                   // nb columns of length n (column-major)
-  double* Wp = V + (size_t)nb * n;    // nb columns of length n; 2*nb*n <= kPanelDoubles
-  double* x = lds + kPanelDoubles;    // n
-  double* tv = x + n;                // 2 * nb <= 16
-  double* red = tv + 16;             // nwave
-  double* qb = red + 16;             // nwave x n partial transposed products
-  double* colbuf = qb + nwave * n;   // n: column c+1 of A, read during the product for c
-  double* pan = colbuf + n;          // 3 x kNbMax: d, e, tau of the panel's columns
-  double* A = work + 5 * M.mat0;
-  double* d = dA + M.vec0;
-  double* e = eA + M.vec0;
-  double* tau = tauA + M.vec0;
-  const double* f = F + M.mat0;
-  const __amdgpu_buffer_rsrc_t arsrc = make_rsrc(A, (int64_t)n * n);
-  for (int64_t t = tid; t < (int64_t)n * n; t += nt) A[t] = f[t];
+  const int nb = panel_width<R>(n);
+  R* V = (R*)L;                      // nb columns of length n (column-major)
+  R* Wp = V + (size_t)nb * n;        // nb columns of length n
+  const int nx = sizeof(R) == 8 ? n : (n + 1) & ~1;  // keeps the doubles 8-byte aligned
+  R* x = (R*)(L + panel_bytes<R>());  // n
+  double* tv = (double*)(x + nx);     // 2 * nb <= 16
+  double* red = tv + 16;              // 2 x nwave: the two block-sum buffers
+  R* qb = (R*)(red + 16);             // nwave x n partial transposed products
+  R* colbuf = qb + nwave * n;         // n: column c+1 of A, read during the product for c
+  double* pan = (double*)(colbuf + nx);  // 3 x kNbMax: d, e, tau of the panel's columns
+  const __amdgpu_buffer_rsrc_t arsrc = make_rsrc(A, (int64_t)n * n * (int64_t)sizeof(R));
+  for (int64_t t = tid; t < (int64_t)n * n; t += nt) A[t] = (R)f[t];
   __syncthreads();
   const int nref = n > 2 ? n - 2 : 0;
   for (int k0 = 0; k0 < nref; k0 += nb) {
     const int b = min(nb, nref - k0);
     for (int t = tid; t < nb * n; t += nt) {
-      V[t] = 0.0;
-      Wp[t] = 0.0;
+      V[t] = (R)0;
+      Wp[t] = (R)0;
     }
     __syncthreads();
+    // W[j-1][c] of the previous column, kept by every thread: the W column is
+    // written without a barrier after it, each thread reading back only its
+    // own rows, and this entry is the one it needs from another thread
+    R wprev = (R)0;
     for (int j = 0; j < b; ++j) {
       const int c = k0 + j;
       // column c of the partially reduced matrix (rows c..n-1, lower part)
-      for (int i = c + tid; i < n; i += nt) {
-        double s = j > 0 ? colbuf[i] : A[(int64_t)i * n + c];
-        for (int q = 0; q < j; ++q)
-          s -= V[q * n + i] * Wp[q * n + c] + Wp[q * n + i] * V[q * n + c];
-        x[i] = s;
-      }
-      __syncthreads();
+      // and the squared norm of its rows c+2..n-1 (the sum's barrier also
+      // publishes x)
       double s2 = 0.0;
-      for (int i = c + 2 + tid; i < n; i += nt) s2 += x[i] * x[i];
+      for (int i = c + tid; i < n; i += nt) {
+        R s = j > 0 ? colbuf[i] : A[(int64_t)i * n + c];
+        for (int q = 0; q < j; ++q) {
+          const R wqc = q == j - 1 ? wprev : Wp[q * n + c];
+          s -= V[q * n + i] * wqc + Wp[q * n + i] * V[q * n + c];
+        }
+        x[i] = s;
+        if (i >= c + 2) s2 += (double)s * (double)s;
+      }
       s2 = block_sum(s2, red);
       const double alpha = x[c + 1];
       // d, e, tau of the panel stay in LDS and the reflectors in V until the
@@ -220,13 +265,14 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(TMAX <=
         }
         if (j + 1 < b)
           for (int i = c + 1 + tid; i < n; i += nt) colbuf[i] = A[(int64_t)i * n + c + 1];
+        wprev = (R)0;
         __syncthreads();
         continue;
       }
       const double beta = -copysign(sqrt(alpha * alpha + s2), alpha);
       const double tk = (beta - alpha) / beta;
       const double sc = 1.0 / (alpha - beta);
-      for (int i = c + 1 + tid; i < n; i += nt) V[j * n + i] = i == c + 1 ? 1.0 : x[i] * sc;
+      for (int i = c + 1 + tid; i < n; i += nt) V[j * n + i] = i == c + 1 ? (R)1 : (R)(x[i] * sc);
       if (tid == 0) {
         pan[kNbMax + j] = beta;
         pan[2 * kNbMax + j] = tk;
@@ -241,49 +287,52 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(TMAX <=
       {
         const int m1 = c + 1;
         const int T = (n - m1 + 63) >> 6;
-        const double* vj = V + j * n;
+        const R* vj = V + j * n;
         const bool grab = j + 1 < b;
-        double q[TMAX];
+        constexpr int rs = (int)sizeof(R);
+        R q[TMAX];
 #pragma unroll
-        for (int t = 0; t < TMAX; ++t) q[t] = 0.0;
+        for (int t = 0; t < TMAX; ++t) q[t] = (R)0;
         for (int i0 = m1 + wave * 8; i0 < n; i0 += nwave * 8) {
-          double acc[8], vi[8];
+          R acc[8], vi[8];
 #pragma unroll
           for (int r = 0; r < 8; ++r) {
-            acc[r] = 0.0;
-            vi[r] = i0 + r < n ? vj[i0 + r] : 0.0;
+            acc[r] = (R)0;
+            vi[r] = i0 + r < n ? vj[i0 + r] : (R)0;
           }
           const int tcount = ((min(i0 + 7, n - 1) - m1) >> 6) + 1;
           // byte offset of (row i0+r, column m1+lane); rows >= n fall outside
           // the buffer and read 0, entries right of the diagonal are masked
           int voff[8];
 #pragma unroll
-          for (int r = 0; r < 8; ++r) voff[r] = ((i0 + r) * n + m1 + lane) * 8;
+          for (int r = 0; r < 8; ++r) voff[r] = ((i0 + r) * n + m1 + lane) * rs;
+          constexpr int SG = symv_group<TMAX, R>();
 #pragma unroll
-          for (int tg = 0; tg < TMAX; tg += kSymvGroup) {
+          for (int tg = 0; tg < TMAX; tg += SG) {
             if (tg < tcount) {
               // entries right of the diagonal get an offset past the buffer:
               // they read 0 and cost no memory traffic
-              double a[kSymvGroup][8];
+              R a[SG][8];
 #pragma unroll
-              for (int g = 0; g < kSymvGroup; ++g)
+              for (int g = 0; g < SG; ++g)
 #pragma unroll
                 for (int r = 0; r < 8; ++r) {
                   const int t = tg + g;
                   const bool in = lane + 64 * t <= i0 + r - m1;
-                  a[g][r] = t < TMAX ? load_f64(arsrc, in ? voff[r] + 512 * t : kOutside) : 0.0;
+                  a[g][r] = t < TMAX ? load_r<R>(arsrc, in ? voff[r] + 64 * rs * t : kOutside)
+                                     : (R)0;
                 }
 #pragma unroll
-              for (int g = 0; g < kSymvGroup; ++g) {
+              for (int g = 0; g < SG; ++g) {
                 const int t = tg + g;
                 if (t < TMAX && t < tcount) {
                   const int l = m1 + lane + 64 * t;
-                  const double vl = l < n ? vj[l] : 0.0;
-                  double qs = 0.0;
+                  const R vl = l < n ? vj[l] : (R)0;
+                  R qs = (R)0;
 #pragma unroll
                   for (int r = 0; r < 8; ++r) {
                     const int i = i0 + r;
-                    const double av = l <= i ? a[g][r] : 0.0;
+                    const R av = l <= i ? a[g][r] : (R)0;
                     acc[r] += av * vl;
                     if (l < i) qs += av * vi[r];
                     if (t == 0 && grab && lane == 0 && i < n) colbuf[i] = av;
@@ -294,27 +343,27 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(TMAX <=
             }
           }
           // lane l ends with the sum of row 4*b5 + 2*b4 + b3 (b = bits of l)
-          double v4[4], v2[2], v1;
+          R v4[4], v2[2], v1;
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
-            double x0 = acc[k], x1 = acc[k + 4];
+            R x0 = acc[k], x1 = acc[k + 4];
             swap32(x0, x1);
             v4[k] = x0 + x1;
           }
 #pragma unroll
           for (int k = 0; k < 2; ++k) {
-            double y0 = v4[k], y1 = v4[k + 2];
+            R y0 = v4[k], y1 = v4[k + 2];
             swap16(y0, y1);
             v2[k] = y0 + y1;
           }
           {
             const bool hi = lane & 8;
-            const double send = hi ? v2[0] : v2[1];
-            v1 = (hi ? v2[1] : v2[0]) + dpp_f64<kDppMirror16>(send);
+            const R send = hi ? v2[0] : v2[1];
+            v1 = (hi ? v2[1] : v2[0]) + dpp_r<kDppMirror16>(send);
           }
-          v1 += dpp_f64<kDppMirror8>(v1);
-          v1 += dpp_f64<kDppXor2>(v1);
-          v1 += dpp_f64<kDppXor1>(v1);
+          v1 += dpp_r<kDppMirror8>(v1);
+          v1 += dpp_r<kDppXor2>(v1);
+          v1 += dpp_r<kDppXor1>(v1);
           const int rho = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
           if ((lane & 7) == 0 && i0 + rho < n) x[i0 + rho] = v1;
         }
@@ -326,27 +375,30 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(TMAX <=
       }
       // W^T v and V^T v for the previous columns of the panel
       for (int qq = wave; qq < 2 * j; qq += nwave) {
-        const double* col = (qq < j) ? (Wp + qq * n) : (V + (qq - j) * n);
-        double s = 0.0;
+        const R* col = (qq < j) ? (Wp + qq * n) : (V + (qq - j) * n);
+        R s = (R)0;
         for (int i = c + 1 + lane; i < n; i += 64) s += col[i] * V[j * n + i];
-        s = wsum(s);
-        if (lane == 0) tv[qq] = s;
+        const double sw = wsum((double)s);
+        if (lane == 0) tv[qq] = sw;
       }
       __syncthreads();
       double pv = 0.0;
       for (int i = c + 1 + tid; i < n; i += nt) {
-        double s = x[i];
+        R s = x[i];
         for (int w = 0; w < nwave; ++w) s += qb[w * n + i];
-        for (int qq = 0; qq < j; ++qq) s -= V[qq * n + i] * tv[qq] + Wp[qq * n + i] * tv[j + qq];
-        s *= tk;
+        for (int qq = 0; qq < j; ++qq)
+          s -= V[qq * n + i] * (R)tv[qq] + Wp[qq * n + i] * (R)tv[j + qq];
+        s *= (R)tk;
         x[i] = s;
-        pv += s * V[j * n + i];
+        if (i == c + 1) tv[15] = (double)s;  // slot 15: 2 j <= 14 entries above
+        pv += (double)s * (double)V[j * n + i];
       }
-      pv = block_sum(pv, red);
+      pv = block_sum(pv, red + 8);
       const double al = -0.5 * tk * pv;
-      for (int i = c + 1 + tid; i < n; i += nt) Wp[j * n + i] = x[i] + al * V[j * n + i];
-      __syncthreads();
+      for (int i = c + 1 + tid; i < n; i += nt) Wp[j * n + i] = x[i] + (R)al * V[j * n + i];
+      wprev = (R)tv[15] + (R)al;  // row c+1 of the new W column (v there is 1)
     }
+    __syncthreads();  // the panel's W columns, for the update
     // the panel's reflectors into the upper part of their rows; d, e, tau
     for (int t = tid; t < b * n; t += nt) {
       const int q = t / n, i = t - q * n, c = k0 + q;
@@ -360,8 +412,8 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(TMAX <=
     // Trailing update A22 -= V W^T + W V^T on the matrix cores: one wave per
     // 16 x 16 tile of the lower triangle (diagonal tiles in full: their upper
     // part is unused until it receives a later reflector), K = 2 nb in steps of
-    // 4 with v_mfma_f64_16x16x4f64.  Operands at rows >= n only reach tile
-    // entries that are not stored.
+    // 4 with v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32.  Operands at rows
+    // >= n only reach tile entries that are not stored.
     const int r = k0 + b;
     if (r < n) {
       const int TT = (n - r + 15) >> 4;
@@ -370,17 +422,17 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(TMAX <=
       const int nsteps = (2 * nb + 3) >> 2;
       // this wave's tiles: linear indices wave, wave + nwave, ... of the lower
       // triangle in row order; the next tile loads before this one is updated
-      auto load = [&](int ti, int tj, Mfma16<double>::v4& v) {
+      auto load = [&](int ti, int tj, typename Mf::v4& v) {
         const int col = r + 16 * tj + lr;
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
-          const int row = r + 16 * ti + 4 * rr + lk;
-          v[rr] = (row < n && col < n) ? A[(int64_t)row * n + col] : 0.0;
+          const int row = r + 16 * ti + Mf::row(lane, rr);
+          v[rr] = (row < n && col < n) ? A[(int64_t)row * n + col] : (R)0;
         }
       };
       int ti = 0, tj = wave;
       while (tj > ti) tj -= ++ti;
-      Mfma16<double>::v4 cur = {0, 0, 0, 0}, nxt = {0, 0, 0, 0};
+      typename Mf::v4 cur = {0, 0, 0, 0}, nxt = {0, 0, 0, 0};
       if (wave < ntile) load(ti, tj, cur);
       for (int idx = wave; idx < ntile; idx += nwave) {
         int ni = ti, nj = tj + nwave;
@@ -389,7 +441,7 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(TMAX <=
         const int I0 = r + 16 * ti, J0 = r + 16 * tj;
         for (int st = 0; st < nsteps; ++st) {
           const int k = 4 * st + lk;
-          double av = 0.0, bv = 0.0;
+          R av = (R)0, bv = (R)0;
           if (k < nb) {
             av = -V[k * n + I0 + lr];
             bv = Wp[k * n + J0 + lr];
@@ -397,12 +449,12 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(TMAX <=
             av = -Wp[(k - nb) * n + I0 + lr];
             bv = V[(k - nb) * n + J0 + lr];
           }
-          cur = Mfma16<double>::op(av, bv, cur);
+          cur = Mf::op(av, bv, cur);
         }
         const int col = J0 + lr;
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
-          const int row = I0 + 4 * rr + lk;
+          const int row = I0 + Mf::row(lane, rr);
           if (row < n && col < n) A[(int64_t)row * n + col] = cur[rr];
         }
         cur = nxt;
@@ -420,6 +472,60 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(TMAX <=
     d[n - 1] = A[(int64_t)(n - 1) * n + (n - 1)];
     e[n - 1] = 0.0;
   }
+}
+
+// Waves per SIMD the register budget is sized for: two workgroups per CU for
+// N <= 384 (80 KB of LDS each in FP64, 40 KB in FP32; three FP32 workgroups
+// at 80 registers spill and measured slower); above N = 384 the LDS holds
+// one workgroup per CU, so the budget is left at two waves per SIMD (no
+// spills of the 20 row accumulators).
+template <int TMAX, typename R>
+constexpr int tridiag_waves() {
+  return TMAX > 6 ? 2 : 4;
+}
+
+// One workgroup per molecule, one launch per working type: prec (null for
+// the full decomposition) marks the molecules whose SCF step runs in FP32,
+// which the float instantiation reduces (in the first half of the
+// work-matrix room) and the double one skips.
+template <int TMAX, typename R>
+__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(tridiag_waves<TMAX, R>())))
+k_tridiag(const MolDesc* __restrict__ mols, const double* __restrict__ F,
+          double* __restrict__ work, double* __restrict__ dA, double* __restrict__ eA,
+          double* __restrict__ tauA, const int* __restrict__ active,
+          const int* __restrict__ status, const int* __restrict__ order,
+          const int* __restrict__ prec) {
+  extern __shared__ double lds[];
+  const int m = order ? order[blockIdx.x] : blockIdx.x;
+  if (!mol_live(active, status, m)) return;
+  const bool fp32 = prec && prec[m];
+  if (fp32 != (sizeof(R) == 4)) return;
+  const MolDesc M = mols[m];
+  tridiag_mol<TMAX, R>(M, F + M.mat0, (R*)(work + 5 * M.mat0), dA + M.vec0, eA + M.vec0,
+                       tauA + M.vec0, (unsigned char*)lds);
+}
+
+// Both working types in one launch, chosen per molecule: for launches whose
+// workgroups are all resident at once (a server lane's few molecules), where
+// two launches would run the FP32 and FP64 molecules one after the other.
+template <int TMAX>
+__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(tridiag_waves<TMAX, double>())))
+k_tridiag_both(const MolDesc* __restrict__ mols, const double* __restrict__ F,
+               double* __restrict__ work, double* __restrict__ dA, double* __restrict__ eA,
+               double* __restrict__ tauA, const int* __restrict__ active,
+               const int* __restrict__ status, const int* __restrict__ order,
+               const int* __restrict__ prec) {
+  extern __shared__ double lds[];
+  const int m = order ? order[blockIdx.x] : blockIdx.x;
+  if (!mol_live(active, status, m)) return;
+  const MolDesc M = mols[m];
+  double* base = work + 5 * M.mat0;
+  unsigned char* L = (unsigned char*)lds;
+  if (prec[m])
+    tridiag_mol<TMAX, float>(M, F + M.mat0, (float*)base, dA + M.vec0, eA + M.vec0,
+                             tauA + M.vec0, L);
+  else
+    tridiag_mol<TMAX, double>(M, F + M.mat0, base, dA + M.vec0, eA + M.vec0, tauA + M.vec0, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -899,82 +1005,78 @@ __global__ void __launch_bounds__(256) k_li_mgs(const MolDesc* __restrict__ mols
 // Y <- Q Y, Q = H_0 H_1 ... H_(nref-1), applied one group of G <= 16
 // reflectors at a time from the last group to the first, each group in
 // compact WY form I - V T V^T (LAPACK dlarft forward / dlarfb).  Every product
-// runs on the matrix cores (v_mfma_f64_16x16x4f64) in 16 x 16 tiles:
+// runs on the matrix cores (v_mfma_f64_16x16x4f64, or v_mfma_f32_16x16x4f32
+// in FP32 steps) in 16 x 16 tiles:
 //   Gram  V V^T (K split over the waves, partial tiles summed in order),
-//   W1 = V Y, W2 = T W1 (the f64 result registers of lane l are exactly the
-//   B operand of the next product for k-step r, so W1 never leaves the
-//   wave), Y -= V^T W2 (next tile loaded before the current one is stored).
+//   W1 = V Y, W2 = T W1 (the result registers of lane l are exactly the B
+//   operand of the next product for k-steps Mfma16::row(l, r), so W1 never
+//   leaves the wave), Y -= V^T W2 (next tile loaded before the current one is
+//   stored).
 // V is row q = reflector k0+q (stored in the upper part of row k0+q of the
 // work matrix, zero at columns <= k0+q).  LDS: V (G x n), W2 (G x k) sharing
 // room with the partial Gram tiles, T and the Gram matrix (16 x 16 each).
-__device__ __forceinline__ int bt_group(int n, int k, int lds_doubles) {
+__device__ __forceinline__ int bt_group(int n, int k, int lds_elems) {
   int g = 16;
-  while (g > 4 && g * n + max(g * k, 2048) + 512 > lds_doubles) g -= 4;
+  while (g > 4 && g * n + max(g * k, 2048) + 512 > lds_elems) g -= 4;
   return g;
 }
 
-__global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict__ mols,
-                                                       double* __restrict__ work,
-                                                       const double* __restrict__ tauA,
-                                                       int want_all,
-                                                       const int* __restrict__ active,
-                                                       const int* __restrict__ status,
-                                                       const int* __restrict__ order,
-                                                       int lds_doubles) {
-  extern __shared__ double lds[];
-  const int m = order ? order[blockIdx.x] : blockIdx.x;
-  if (!mol_live(active, status, m)) return;
-  const MolDesc M = mols[m];
+// In FP32 steps the reflectors are FP32 (first half of the work-matrix room)
+// and Y is converted once into FP32 vectors at 2 N^2 (the inverse-iteration
+// factors' room, free by then), where the density reads them.
+template <typename R>
+__device__ __forceinline__ void back_mol(const MolDesc& M, const R* __restrict__ A,
+                                         const double* __restrict__ Ysrc, R* __restrict__ Y,
+                                         const double* __restrict__ tau, int lo, int k,
+                                         int lds_elems, R* lds) {
+  typedef Mfma16<R> Mf;
+  typedef typename Mf::v4 v4;
   const int n = M.n_ao, tid = threadIdx.x, nt = blockDim.x;
   const int wave = tid >> 6, lane = tid & 63, nwave = nt >> 6;
   const int lr = lane & 15, lk = lane >> 4;
-  int lo, k;
-  selection(M, want_all, lo, k);
   const int nref = n > 2 ? n - 2 : 0;
-  if (k == 0 || nref == 0) return;
-  const int G = bt_group(n, k, lds_doubles);
+  const int G = bt_group(n, k, lds_elems);
   const int rg = max(G * k, 2048);
-  double* Vs = lds;          // G x n
-  double* R = Vs + G * n;    // W2 (G x k), before that nwave partial 16x16 Gram tiles
-  double* Tm = R + rg;       // 16 x 16, T[r][q] at r*16+q
-  double* Gm = Tm + 256;     // 16 x 16 Gram
-  const double* A = work + 5 * M.mat0;
-  double* Y = work + 5 * M.mat0 + (int64_t)n * n;
-  const double* tau = tauA + M.vec0;
+  R* Vs = lds;          // G x n
+  R* Rw = Vs + G * n;   // W2 (G x k), before that nwave partial 16x16 Gram tiles
+  R* Tm = Rw + rg;      // 16 x 16, T[r][q] at r*16+q
+  R* Gm = Tm + 256;     // 16 x 16 Gram
   const int KT = (k + 15) >> 4;
-  typedef Mfma16<double>::v4 v4;
+  if ((const void*)Ysrc != (const void*)Y)
+    for (int64_t t = tid; t < (int64_t)n * k; t += nt) Y[t] = (R)Ysrc[t];
+  if (nref == 0) return;  // Q = I
   for (int k0 = ((nref - 1) / G) * G; k0 >= 0; k0 -= G) {
     const int b = min(G, nref - k0);
     for (int t = tid; t < G * n; t += nt) {
       const int q = t / n, i = t - q * n, c = k0 + q;
-      Vs[t] = (q < b && i > c) ? A[(int64_t)c * n + i] : 0.0;
+      Vs[t] = (q < b && i > c) ? A[(int64_t)c * n + i] : (R)0;
     }
-    for (int t = tid; t < 256; t += nt) Tm[t] = 0.0;
+    for (int t = tid; t < 256; t += nt) Tm[t] = (R)0;
     __syncthreads();
     // Gram V V^T: wave w sums rows i = k0+1+4w+..., step 4 nwave
     {
       v4 acc = {0, 0, 0, 0};
       for (int i0 = k0 + 1 + 4 * wave; i0 < n; i0 += 4 * nwave) {
         const int i = i0 + lk;
-        const double a = (lr < G && i < n) ? Vs[lr * n + i] : 0.0;
-        acc = Mfma16<double>::op(a, a, acc);
+        const R a = (lr < G && i < n) ? Vs[lr * n + i] : (R)0;
+        acc = Mf::op(a, a, acc);
       }
 #pragma unroll
-      for (int rr = 0; rr < 4; ++rr) R[wave * 256 + (4 * rr + lk) * 16 + lr] = acc[rr];
+      for (int rr = 0; rr < 4; ++rr) Rw[wave * 256 + Mf::row(lane, rr) * 16 + lr] = acc[rr];
     }
     __syncthreads();
     for (int t = tid; t < 256; t += nt) {
-      double g = 0.0;
-      for (int w = 0; w < nwave; ++w) g += R[w * 256 + t];
+      R g = (R)0;
+      for (int w = 0; w < nwave; ++w) g += Rw[w * 256 + t];
       Gm[t] = g;
     }
     __syncthreads();
     // T: column i from columns < i (one wave, lanes r < i)
     if (wave == 0) {
       for (int i = 0; i < b; ++i) {
-        const double ti = tau[k0 + i];
+        const R ti = (R)tau[k0 + i];
         if (lane < i) {
-          double sacc = 0.0;
+          R sacc = (R)0;
           for (int q = lane; q < i; ++q) sacc += Tm[lane * 16 + q] * Gm[q * 16 + i];
           Tm[lane * 16 + i] = -ti * sacc;
         }
@@ -990,17 +1092,17 @@ __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict_
       v4 w1 = {0, 0, 0, 0};
       for (int i0 = k0 + 1; i0 < n; i0 += 4) {
         const int i = i0 + lk;
-        const double a = (lr < G && i < n) ? Vs[lr * n + i] : 0.0;
-        const double y = (i < n && j < k) ? Y[(int64_t)i * k + j] : 0.0;
-        w1 = Mfma16<double>::op(a, y, w1);
+        const R a = (lr < G && i < n) ? Vs[lr * n + i] : (R)0;
+        const R y = (i < n && j < k) ? Y[(int64_t)i * k + j] : (R)0;
+        w1 = Mf::op(a, y, w1);
       }
       v4 w2 = {0, 0, 0, 0};
 #pragma unroll
-      for (int st = 0; st < 4; ++st) w2 = Mfma16<double>::op(Tm[lr * 16 + 4 * st + lk], w1[st], w2);
+      for (int st = 0; st < 4; ++st) w2 = Mf::op(Tm[lr * 16 + Mf::row(lane, st)], w1[st], w2);
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
-        const int g = 4 * rr + lk;
-        if (g < G && j < k) R[g * k + j] = w2[rr];
+        const int g = Mf::row(lane, rr);
+        if (g < G && j < k) Rw[g * k + j] = w2[rr];
       }
     }
     __syncthreads();
@@ -1012,8 +1114,8 @@ __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict_
         const int I0 = k0 + 1 + 16 * (t / KT), col = 16 * (t % KT) + lr;
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
-          const int row = I0 + 4 * rr + lk;
-          v[rr] = (row < n && col < k) ? Y[(int64_t)row * k + col] : 0.0;
+          const int row = I0 + Mf::row(lane, rr);
+          v[rr] = (row < n && col < k) ? Y[(int64_t)row * k + col] : (R)0;
         }
       };
       v4 cur = {0, 0, 0, 0}, nxt = {0, 0, 0, 0};
@@ -1025,14 +1127,14 @@ __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict_
         for (int st = 0; st < 4; ++st) {
           const int g = 4 * st + lk;
           const int i = I0 + lr, j = J0 + lr;
-          const double a = (g < G && i < n) ? -Vs[g * n + i] : 0.0;
-          const double w = (g < G && j < k) ? R[g * k + j] : 0.0;
-          cur = Mfma16<double>::op(a, w, cur);
+          const R a = (g < G && i < n) ? -Vs[g * n + i] : (R)0;
+          const R w = (g < G && j < k) ? Rw[g * k + j] : (R)0;
+          cur = Mf::op(a, w, cur);
         }
         const int col = J0 + lr;
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
-          const int row = I0 + 4 * rr + lk;
+          const int row = I0 + Mf::row(lane, rr);
           if (row < n && col < k) Y[(int64_t)row * k + col] = cur[rr];
         }
         cur = nxt;
@@ -1042,6 +1144,32 @@ __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict_
   }
 }
 
+__global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict__ mols,
+                                                       double* __restrict__ work,
+                                                       const double* __restrict__ tauA,
+                                                       int want_all,
+                                                       const int* __restrict__ active,
+                                                       const int* __restrict__ status,
+                                                       const int* __restrict__ order,
+                                                       int lds_doubles,
+                                                       const int* __restrict__ prec) {
+  extern __shared__ double lds[];
+  const int m = order ? order[blockIdx.x] : blockIdx.x;
+  if (!mol_live(active, status, m)) return;
+  const MolDesc M = mols[m];
+  const int n = M.n_ao;
+  int lo, k;
+  selection(M, want_all, lo, k);
+  if (k == 0) return;
+  double* base = work + 5 * M.mat0;
+  double* Y = base + (int64_t)n * n;
+  if (prec && prec[m])
+    back_mol<float>(M, (const float*)base, Y, (float*)(base + 2 * (int64_t)n * n),
+                    tauA + M.vec0, lo, k, 2 * lds_doubles, (float*)lds);
+  else
+    back_mol<double>(M, base, Y, Y, tauA + M.vec0, lo, k, lds_doubles, lds);
+}
+
 // ---------------------------------------------------------------------------
 // P from the selected vectors Y (N x k, row-major): 2 Y Y^T (occupied side)
 // or 2 (I - Y Y^T) (virtual side), on the matrix cores with
@@ -1049,15 +1177,15 @@ __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict_
 // the mixed-precision policy).  One wave computes a 32 x 32 block of the lower
 // triangle as 2 x 2 tiles of 16 x 16 (each operand fragment feeds two
 // instructions), K = k in steps of 4, and stores it with its mirror image.
-template <typename T>
-__device__ __forceinline__ void density_block(const double* __restrict__ Y, int n, int k, int lo,
+template <typename T, typename YT>
+__device__ __forceinline__ void density_block(const YT* __restrict__ Y, int n, int k, int lo,
                                               int i0, int j0, int lane, double* __restrict__ p,
                                               double* __restrict__ tr) {
   typedef typename Mfma16<T>::v4 v4;
   v4 acc[2][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
   const int lr = lane & 15, kk = lane >> 4;
-  const double* ya[2];
-  const double* yb[2];
+  const YT* ya[2];
+  const YT* yb[2];
   bool oka[2], okb[2];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
@@ -1121,7 +1249,7 @@ __global__ void __launch_bounds__(256) k_density(const MolDesc* __restrict__ mol
                                                   const int* __restrict__ active,
                                                   const int* __restrict__ status,
                                                   const int* __restrict__ prec,
-                                                  const int* __restrict__ order) {
+                                                  const int* __restrict__ order, int float_y) {
   const int m = order ? order[blockIdx.y] : blockIdx.y;
   if (!mol_live(active, status, m)) return;
   const MolDesc M = mols[m];
@@ -1134,12 +1262,18 @@ __global__ void __launch_bounds__(256) k_density(const MolDesc* __restrict__ mol
   const int t32 = (n + 31) >> 5;
   const int nblk = t32 * (t32 + 1) / 2;  // lower-triangle 32 x 32 blocks, row order
   const bool fp32 = prec && prec[m];
+  // FP32 vectors of the one-workgroup back-transform (float_y), at 2 N^2
+  const float* Yf = (const float*)(work + 5 * M.mat0 + 2 * (int64_t)n * n);
   __shared__ double trans[4][32 * 33];  // per-wave transpose buffer
   for (int blk = blockIdx.x * 4 + wave; blk < nblk; blk += gridDim.x * 4) {
     int bi = 0, bj = blk;
     while (bj > bi) bj -= ++bi;
-    if (fp32) density_block<float>(Y, n, k, lo, 32 * bi, 32 * bj, lane, p, trans[wave]);
-    else density_block<double>(Y, n, k, lo, 32 * bi, 32 * bj, lane, p, trans[wave]);
+    if (fp32 && float_y)
+      density_block<float>(Yf, n, k, lo, 32 * bi, 32 * bj, lane, p, trans[wave]);
+    else if (fp32)
+      density_block<float>(Y, n, k, lo, 32 * bi, 32 * bj, lane, p, trans[wave]);
+    else
+      density_block<double>(Y, n, k, lo, 32 * bi, 32 * bj, lane, p, trans[wave]);
   }
 }
 
@@ -1177,9 +1311,12 @@ __global__ void k_write_eigvec(const MolDesc* __restrict__ mols, const double* _
   }
 }
 
+// Panel, 32 + 3 kNbMax scalars (FP64), x and the next column (each rounded up
+// to an even count), kT/64 partial products.
+template <typename R>
 static size_t tridiag_lds(int max_n) {
-  return sizeof(double) * ((size_t)kPanelDoubles + 2 * (size_t)max_n + 32 + 3 * kNbMax +
-                           (size_t)(kT / 64) * max_n);
+  return (size_t)panel_bytes<R>() + sizeof(double) * (32 + 3 * kNbMax) +
+         sizeof(R) * ((2 + kT / 64) * (size_t)max_n + 2);
 }
 
 // The one-workgroup kernels take the molecules below large_min: the tail of
@@ -1195,18 +1332,64 @@ Part small_part(const BatchDev& b) {
 }
 }  // namespace
 
-void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all) {
+// SCF steps: while every workgroup fits on the device at once (two per CU up
+// to N = 384, one above), one launch takes both working types; larger batches
+// of the mixed policy get one launch per type (the FP64 molecules' workgroups
+// exit at once in the FP32 launch and the other way round), each with its own
+// register budget.  The full decomposition is FP64.
+namespace {
+struct TridiagPlan {
+  Part p;
+  const int* prec;  // null: FP64 only
+  bool small, both;
+};
+TridiagPlan tridiag_plan(const BatchDev& b, bool all) {
+  TridiagPlan t;
+  t.p = small_part(b);
+  t.prec = (all || !b.mixed) ? nullptr : b.prec;
+  t.small = t.p.max_n <= 384;
+  t.both = t.prec && t.p.count <= (t.small ? 2 : 1) * kDeviceCUs;
+  return t;
+}
+}  // namespace
+
+bool tridiag_fp32_launch(const BatchDev& b, bool all) {
+  const TridiagPlan t = tridiag_plan(b, all);
+  return t.p.count > 0 && t.prec && !t.both;
+}
+
+void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all, int which) {
   const int* act = all ? nullptr : b.active;
-  const Part p = small_part(b);
-  if (p.count > 0) {
-    if (p.max_n <= 384)
-      k_tridiag<6><<<p.count, kT, tridiag_lds(p.max_n), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau,
-                                                             act, b.status, p.order);
+  const TridiagPlan t = tridiag_plan(b, all);
+  const Part& p = t.p;
+  const int* prec = t.prec;
+  if (which != 1 && p.count > 0) {
+    const size_t lds = tridiag_lds<double>(p.max_n);
+    if (t.both && t.small)
+      k_tridiag_both<6><<<p.count, kT, lds, s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act,
+                                                 b.status, p.order, prec);
+    else if (t.both)
+      k_tridiag_both<20><<<p.count, kT, lds, s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act,
+                                                  b.status, p.order, prec);
+    else if (t.small)
+      k_tridiag<6, double><<<p.count, kT, lds, s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act,
+                                                    b.status, p.order, prec);
     else
-      k_tridiag<20><<<p.count, kT, tridiag_lds(p.max_n), s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau,
-                                                              act, b.status, p.order);
+      k_tridiag<20, double><<<p.count, kT, lds, s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act,
+                                                     b.status, p.order, prec);
   }
-  for (int i = 0; i < b.n_large; ++i) launch_tridiag_large(b, s, b.large_idx[i], b.large_mols[i], act);
+  if (which != 0 && p.count > 0 && prec && !t.both) {
+    const size_t lds = tridiag_lds<float>(p.max_n);
+    if (t.small)
+      k_tridiag<6, float><<<p.count, kT, lds, s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act,
+                                                   b.status, p.order, prec);
+    else
+      k_tridiag<20, float><<<p.count, kT, lds, s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act,
+                                                    b.status, p.order, prec);
+  }
+  if (which != 1)
+    for (int i = 0; i < b.n_large; ++i)
+      launch_tridiag_large(b, s, b.large_idx[i], b.large_mols[i], act);
 }
 
 static void bisect_part(const BatchDev& b, hipStream_t s, bool all, const int* order, int count,
@@ -1264,7 +1447,8 @@ void launch_eigh_back(const BatchDev& b, hipStream_t s, bool all) {
     const int kmax = all ? N : N / 2 + 1;
     const int bt_doubles = 4 * (N + kmax) + 2560 > 10240 ? 20480 : 10240;
     k_backtransform<<<p.count, kT, sizeof(double) * bt_doubles, s>>>(
-        b.mols, b.W, b.tau, all ? 1 : 0, act, b.status, p.order, bt_doubles);
+        b.mols, b.W, b.tau, all ? 1 : 0, act, b.status, p.order, bt_doubles,
+        all ? nullptr : b.prec);
   }
   for (int i = 0; i < b.n_large; ++i) launch_back_large(b, s, b.large_idx[i], b.large_mols[i], act, all);
   if (all) {
@@ -1274,17 +1458,18 @@ void launch_eigh_back(const BatchDev& b, hipStream_t s, bool all) {
 }
 
 void launch_eigh(const BatchDev& b, hipStream_t s, bool all) {
-  launch_eigh_tridiag(b, s, all);
+  launch_eigh_tridiag(b, s, all, -1);
   launch_eigh_bisect(b, s, all);
   launch_eigh_invit(b, s, all);
   launch_eigh_back(b, s, all);
 }
 
-static void density_part(const BatchDev& b, hipStream_t s, const int* order, int count, int N) {
+static void density_part(const BatchDev& b, hipStream_t s, const int* order, int count, int N,
+                         int float_y) {
   // as many workgroups per molecule as the largest one has 4-block groups
   const int t32 = (N + 31) >> 5;
   dim3 grid((t32 * (t32 + 1) / 2 + 3) / 4, count);
-  k_density<<<grid, 256, 0, s>>>(b.mols, b.W, b.P, b.active, b.status, b.prec, order);
+  k_density<<<grid, 256, 0, s>>>(b.mols, b.W, b.P, b.active, b.status, b.prec, order, float_y);
 }
 
 void launch_density(const BatchDev& b, hipStream_t s, bool all) {
@@ -1293,8 +1478,8 @@ void launch_density(const BatchDev& b, hipStream_t s, bool all) {
     k_density_from_c<<<grid, 256, 0, s>>>(b.mols, b.C, b.P, 64);
   } else {
     const Part p = small_part(b);
-    if (p.count > 0) density_part(b, s, p.order, p.count, p.max_n);
-    if (b.n_large > 0) density_part(b, s, b.order, b.n_large, b.max_n);
+    if (p.count > 0) density_part(b, s, p.order, p.count, p.max_n, 1);
+    if (b.n_large > 0) density_part(b, s, b.order, b.n_large, b.max_n, 0);
   }
 }
 

@@ -560,6 +560,7 @@ int server_create(sg_context* ctx, double hbm_bytes, int max_slots, int max_atom
   }
   char* base = (char*)S->pool;
   BatchDev& d = S->shared;
+  d.mixed = policy == SG_POLICY_MIXED ? 1 : 0;
   d.blocks = (double*)(base + o_blk);
   d.blocks32 = (float*)(base + o_blk32);
   d.H = (double*)(base + o_H);

@@ -164,11 +164,28 @@ def test_mixed_policy_water_batch(engine):
     mols = geometry.water_batch(64, seed=11)
     b = engine.batch(mols)
     ref = b.scf(threshold=1e-6, policy="fp64")
+    assert not b.fp32_steps().any()
     mix = b.scf(threshold=1e-6, policy="mixed")
+    s32 = b.fp32_steps()
     worst = max(abs(a["e_tot"] - c["e_tot"]) for a, c in zip(ref, mix))
-    say("mixed vs fp64 worst |dE|", worst)
+    say("mixed vs fp64 worst |dE|", worst, "FP32 steps", s32.min(), s32.max())
     assert all(r["converged"] for r in mix)
     assert worst <= 1e-4
+    # every molecule starts in FP32 and ends with two FP64 steps
+    assert all(1 <= n32 <= r["steps"] - 2 for n32, r in zip(s32, mix))
+
+
+def test_mixed_policy_large_batch_equals_alone(engine):
+    """Above 512 molecules each SCF step reduces its FP32 and FP64 molecules
+    in separate launches (one per working type); alone, one launch takes
+    both.  The results agree bit for bit."""
+    mols = geometry.water_batch(600, seed=5, n_min=2, n_max=10)
+    big = engine.batch(mols).scf(threshold=1e-6, policy="mixed")
+    assert all(r["converged"] for r in big)
+    for i in (0, 1, 299, 599):
+        alone = engine.batch([mols[i]]).scf(threshold=1e-6, policy="mixed")[0]
+        say("mixed alone", i, alone["steps"], alone["e_tot"], big[i]["e_tot"])
+        assert alone == big[i], i
 
 
 def test_graph_replay_equals_eager(engine):
