@@ -281,7 +281,8 @@ __device__ __forceinline__ void tridiag_mol(const MolDesc& M, const double* __re
       // p = A22 v from the lower triangle of the trailing block.  A wave takes
       // 8 consecutive rows: row parts are dotted with v, and each row also adds
       // A[i][l] v_i to lane-owned accumulators for column l < i (the upper half).
-      // Tiles entirely left of the chunk's first row need no guards.  The 8 row
+      // Groups of tiles entirely left of the chunk's first row take a path
+      // without guards (two FMAs per loaded entry).  The 8 row
       // sums are reduced together (each butterfly step halves the values a lane
       // carries), and lane 0 keeps column c+1 for the next reflector.
       {
@@ -307,9 +308,32 @@ __device__ __forceinline__ void tridiag_mol(const MolDesc& M, const double* __re
 #pragma unroll
           for (int r = 0; r < 8; ++r) voff[r] = ((i0 + r) * n + m1 + lane) * rs;
           constexpr int SG = symv_group<TMAX, R>();
+          const int tfull = (i0 - m1) >> 6;  // tiles entirely left of row i0
 #pragma unroll
           for (int tg = 0; tg < TMAX; tg += SG) {
-            if (tg < tcount) {
+            if (tg + SG <= tfull) {
+              // every entry of these tiles lies left of the diagonal, in a row
+              // of the matrix or in a row >= n (which reads 0): no guards, and
+              // the tile offsets are immediates of the loads
+              R a[SG][8];
+#pragma unroll
+              for (int g = 0; g < SG; ++g)
+#pragma unroll
+                for (int r = 0; r < 8; ++r) a[g][r] = load_r<R>(arsrc, voff[r] + 64 * rs * (tg + g));
+#pragma unroll
+              for (int g = 0; g < SG; ++g) {
+                const int t = tg + g;
+                const R vl = vj[m1 + lane + 64 * t];
+                R qs = (R)0;
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                  acc[r] += a[g][r] * vl;
+                  qs += a[g][r] * vi[r];
+                  if (t == 0 && grab && lane == 0 && i0 + r < n) colbuf[i0 + r] = a[g][r];
+                }
+                q[t] += qs;
+              }
+            } else if (tg < tcount) {
               // entries right of the diagonal get an offset past the buffer:
               // they read 0 and cost no memory traffic
               R a[SG][8];
@@ -332,7 +356,7 @@ __device__ __forceinline__ void tridiag_mol(const MolDesc& M, const double* __re
 #pragma unroll
                   for (int r = 0; r < 8; ++r) {
                     const int i = i0 + r;
-                    const R av = l <= i ? a[g][r] : (R)0;
+                    const R av = a[g][r];  // 0 right of the diagonal
                     acc[r] += av * vl;
                     if (l < i) qs += av * vi[r];
                     if (t == 0 && grab && lane == 0 && i < n) colbuf[i] = av;
