@@ -498,14 +498,20 @@ __device__ __forceinline__ void tridiag_mol(const MolDesc& M, const double* __re
   }
 }
 
-// Waves per SIMD the register budget is sized for: two workgroups per CU for
-// N <= 384 (80 KB of LDS each in FP64, 40 KB in FP32; three FP32 workgroups
-// at 80 registers spill and measured slower); above N = 384 the LDS holds
-// one workgroup per CU, so the budget is left at two waves per SIMD (no
-// spills of the 20 row accumulators).
+// Wavefronts per molecule and the register budget (waves per SIMD).  Up to
+// N = 384: FP64 eight wavefronts, two workgroups per CU (80 KB of LDS each;
+// four wavefronts with half the panel and three workgroups per CU measured
+// slower); FP32 four wavefronts, four workgroups per CU (34 KB each), faster
+// than two of eight.  Above N = 384 eight wavefronts in either type and one
+// workgroup per CU (the LDS holds one), with the budget at two waves per SIMD
+// (no spills of the 20 row accumulators).
 template <int TMAX, typename R>
 constexpr int tridiag_waves() {
   return TMAX > 6 ? 2 : 4;
+}
+template <int TMAX, typename R>
+constexpr int tridiag_threads() {
+  return (TMAX <= 6 && sizeof(R) == 4) ? 256 : kT;
 }
 
 // One workgroup per molecule, one launch per working type: prec (null for
@@ -513,7 +519,8 @@ constexpr int tridiag_waves() {
 // which the float instantiation reduces (in the first half of the
 // work-matrix room) and the double one skips.
 template <int TMAX, typename R>
-__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(tridiag_waves<TMAX, R>())))
+__global__ void __launch_bounds__((tridiag_threads<TMAX, R>()))
+__attribute__((amdgpu_waves_per_eu(tridiag_waves<TMAX, R>())))
 k_tridiag(const MolDesc* __restrict__ mols, const double* __restrict__ F,
           double* __restrict__ work, double* __restrict__ dA, double* __restrict__ eA,
           double* __restrict__ tauA, const int* __restrict__ active,
@@ -533,7 +540,8 @@ k_tridiag(const MolDesc* __restrict__ mols, const double* __restrict__ F,
 // workgroups are all resident at once (a server lane's few molecules), where
 // two launches would run the FP32 and FP64 molecules one after the other.
 template <int TMAX>
-__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(tridiag_waves<TMAX, double>())))
+__global__ void __launch_bounds__((tridiag_threads<TMAX, double>()))
+__attribute__((amdgpu_waves_per_eu(tridiag_waves<TMAX, double>())))
 k_tridiag_both(const MolDesc* __restrict__ mols, const double* __restrict__ F,
                double* __restrict__ work, double* __restrict__ dA, double* __restrict__ eA,
                double* __restrict__ tauA, const int* __restrict__ active,
@@ -1336,11 +1344,11 @@ __global__ void k_write_eigvec(const MolDesc* __restrict__ mols, const double* _
 }
 
 // Panel, 32 + 3 kNbMax scalars (FP64), x and the next column (each rounded up
-// to an even count), kT/64 partial products.
-template <typename R>
+// to an even count), one partial product per wavefront.
+template <int TMAX, typename R>
 static size_t tridiag_lds(int max_n) {
   return (size_t)panel_bytes<R>() + sizeof(double) * (32 + 3 * kNbMax) +
-         sizeof(R) * ((2 + kT / 64) * (size_t)max_n + 2);
+         sizeof(R) * ((2 + tridiag_threads<TMAX, R>() / 64) * (size_t)max_n + 2);
 }
 
 // The one-workgroup kernels take the molecules below large_min: the tail of
@@ -1387,30 +1395,20 @@ void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all, int which) 
   const TridiagPlan t = tridiag_plan(b, all);
   const Part& p = t.p;
   const int* prec = t.prec;
+#define SG_TRIDIAG(KERNEL, TM, R)                                                      \
+  KERNEL<<<p.count, tridiag_threads<TM, R>(), tridiag_lds<TM, R>(p.max_n), s>>>(       \
+      b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status, p.order, prec)
   if (which != 1 && p.count > 0) {
-    const size_t lds = tridiag_lds<double>(p.max_n);
-    if (t.both && t.small)
-      k_tridiag_both<6><<<p.count, kT, lds, s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act,
-                                                 b.status, p.order, prec);
-    else if (t.both)
-      k_tridiag_both<20><<<p.count, kT, lds, s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act,
-                                                  b.status, p.order, prec);
-    else if (t.small)
-      k_tridiag<6, double><<<p.count, kT, lds, s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act,
-                                                    b.status, p.order, prec);
-    else
-      k_tridiag<20, double><<<p.count, kT, lds, s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act,
-                                                     b.status, p.order, prec);
+    if (t.both && t.small) SG_TRIDIAG(k_tridiag_both<6>, 6, double);
+    else if (t.both) SG_TRIDIAG(k_tridiag_both<20>, 20, double);
+    else if (t.small) SG_TRIDIAG((k_tridiag<6, double>), 6, double);
+    else SG_TRIDIAG((k_tridiag<20, double>), 20, double);
   }
   if (which != 0 && p.count > 0 && prec && !t.both) {
-    const size_t lds = tridiag_lds<float>(p.max_n);
-    if (t.small)
-      k_tridiag<6, float><<<p.count, kT, lds, s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act,
-                                                   b.status, p.order, prec);
-    else
-      k_tridiag<20, float><<<p.count, kT, lds, s>>>(b.mols, b.F, b.W, b.dT, b.eT, b.tau, act,
-                                                    b.status, p.order, prec);
+    if (t.small) SG_TRIDIAG((k_tridiag<6, float>), 6, float);
+    else SG_TRIDIAG((k_tridiag<20, float>), 20, float);
   }
+#undef SG_TRIDIAG
   if (which != 1)
     for (int i = 0; i < b.n_large; ++i)
       launch_tridiag_large(b, s, b.large_idx[i], b.large_mols[i], act);

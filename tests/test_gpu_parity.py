@@ -175,17 +175,25 @@ def test_mixed_policy_water_batch(engine):
     assert all(1 <= n32 <= r["steps"] - 2 for n32, r in zip(s32, mix))
 
 
-def test_mixed_policy_large_batch_equals_alone(engine):
+def test_mixed_policy_large_batch_vs_alone(engine):
     """Above 512 molecules each SCF step reduces its FP32 and FP64 molecules
-    in separate launches (one per working type); alone, one launch takes
-    both.  The results agree bit for bit."""
+    in separate launches, the FP32 one with four wavefronts per molecule;
+    alone, one launch takes both types with eight.  The FP32 steps then sum in
+    another order, so the results agree to the SCF threshold, not bit for bit
+    (FP64 steps and the FP64 policy are the same in both)."""
     mols = geometry.water_batch(600, seed=5, n_min=2, n_max=10)
-    big = engine.batch(mols).scf(threshold=1e-6, policy="mixed")
+    b = engine.batch(mols)
+    big = b.scf(threshold=1e-6, policy="mixed")
     assert all(r["converged"] for r in big)
+    ref = b.scf(threshold=1e-6, policy="fp64")
     for i in (0, 1, 299, 599):
         alone = engine.batch([mols[i]]).scf(threshold=1e-6, policy="mixed")[0]
-        say("mixed alone", i, alone["steps"], alone["e_tot"], big[i]["e_tot"])
-        assert alone == big[i], i
+        say("mixed alone", i, alone["steps"], alone["e_tot"] - big[i]["e_tot"],
+            alone["e_tot"] - ref[i]["e_tot"])
+        assert alone["converged"]
+        assert abs(alone["e_tot"] - big[i]["e_tot"]) <= 1e-6, i
+        assert abs(alone["e_tot"] - ref[i]["e_tot"]) <= 1e-4, i
+    b.close()
 
 
 def test_graph_replay_equals_eager(engine):
