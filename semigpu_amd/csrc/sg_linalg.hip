@@ -45,6 +45,11 @@ constexpr int kPanelLds = 48 * 1024;    // bytes of LDS for the V/W panel
 constexpr int kNbMax = 8;               // widest reflector panel
 constexpr int kPanelDoubles = kPanelLds / 8;
 constexpr double kEps = 2.220446049250313e-16;
+// Bisection width in FP32 steps, relative to ||T||: far below the FP32
+// rounding of the reduced matrix (~1e-7), and enough for inverse iteration to
+// settle a vector whose gap to the next eigenvalue is above 1e-5 ||T|| (the
+// closer ones are orthogonalised as a cluster).
+constexpr double kFp32BisectTol = 1e-10;
 // Inverse-iteration solves per vector.  With eigenvalues bisected to a few
 // ulps, one solve from a random start already leaves the wanted direction
 // dominant by ~1/eps; the second solve (after the cluster orthogonalisation)
@@ -623,7 +628,8 @@ __global__ void __launch_bounds__(256) k_bisect(const MolDesc* __restrict__ mols
                                                  const int* __restrict__ active,
                                                  const int* __restrict__ status,
                                                  const int* __restrict__ order,
-                                                 const int* __restrict__ steps) {
+                                                 const int* __restrict__ steps,
+                                                 const int* __restrict__ prec) {
   extern __shared__ double lds[];
   const int m = order ? order[blockIdx.y] : blockIdx.y;
   if (!mol_live(active, status, m)) return;
@@ -709,12 +715,14 @@ __global__ void __launch_bounds__(256) k_bisect(const MolDesc* __restrict__ mols
     lo_[u] = a;
     hi_[u] = b;
   }
-  // bisection keeping count(a) <= idx < count(b)
+  // bisection keeping count(a) <= idx < count(b), to a few ulps; in an FP32
+  // step to kFp32BisectTol ||T|| (T itself carries FP32 rounding there)
+  const double atol = pivmin + (prec && prec[m] ? kFp32BisectTol * tnorm : 0.0);
   bool done[2];
 #pragma unroll
   for (int u = 0; u < 2; ++u)
     done[u] = jj[u] >= k ||
-              hi_[u] - lo_[u] <= 2.0 * kEps * fmax(fabs(lo_[u]), fabs(hi_[u])) + pivmin;
+              hi_[u] - lo_[u] <= 2.0 * kEps * fmax(fabs(lo_[u]), fabs(hi_[u])) + atol;
   for (int it = 0; it < 128 && !(done[0] && done[1]); ++it) {
     const double m0 = 0.5 * (lo_[0] + hi_[0]), m1 = 0.5 * (lo_[1] + hi_[1]);
     int c[2];
@@ -725,7 +733,7 @@ __global__ void __launch_bounds__(256) k_bisect(const MolDesc* __restrict__ mols
       if (done[u]) continue;
       if (c[u] > lo + jj[u]) hi_[u] = mid[u];
       else lo_[u] = mid[u];
-      done[u] = hi_[u] - lo_[u] <= 2.0 * kEps * fmax(fabs(lo_[u]), fabs(hi_[u])) + pivmin;
+      done[u] = hi_[u] - lo_[u] <= 2.0 * kEps * fmax(fabs(lo_[u]), fabs(hi_[u])) + atol;
     }
   }
 #pragma unroll
@@ -1415,7 +1423,7 @@ void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all, int which) 
 }
 
 static void bisect_part(const BatchDev& b, hipStream_t s, bool all, const int* order, int count,
-                        int N) {
+                        int N, const int* prec) {
   // two eigenvalues per lane: (k+1)/2 lanes, k <= N/2 on the SCF path
   const int half = std::max(1, ((all ? N : N / 2) + 1) / 2);  // N = 1: no wanted value
   const int threads = min(256, max(64, (half + 63) / 64 * 64));
@@ -1423,13 +1431,16 @@ static void bisect_part(const BatchDev& b, hipStream_t s, bool all, const int* o
   k_bisect<<<gb, threads, sizeof(double) * (2 * N + 32), s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm,
                                                           all ? 1 : 0, all ? nullptr : b.active,
                                                           b.status, order,
-                                                          all ? nullptr : b.steps);
+                                                          all ? nullptr : b.steps, prec);
 }
 
 void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all) {
   const Part p = small_part(b);
-  if (p.count > 0) bisect_part(b, s, all, p.order, p.count, p.max_n);
-  if (b.n_large > 0) bisect_part(b, s, all, b.order, b.n_large, b.max_n);
+  // FP32 steps of the one-workgroup path bisect to the accuracy of their
+  // FP32-reduced T; the multi-workgroup path reduces in FP64 throughout
+  const int* prec = (all || !b.mixed) ? nullptr : b.prec;
+  if (p.count > 0) bisect_part(b, s, all, p.order, p.count, p.max_n, prec);
+  if (b.n_large > 0) bisect_part(b, s, all, b.order, b.n_large, b.max_n, nullptr);
 }
 
 void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all) {
