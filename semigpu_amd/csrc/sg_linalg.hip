@@ -928,7 +928,8 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(3))) k_
                                                double* __restrict__ work, int want_all,
                                                const int* __restrict__ active,
                                                const int* __restrict__ status,
-                                               const int* __restrict__ order) {
+                                               const int* __restrict__ order,
+                                               const int* __restrict__ prec) {
   extern __shared__ double lds[];
   const int m = order ? order[blockIdx.x] : blockIdx.x;
   if (!mol_live(active, status, m)) return;
@@ -938,6 +939,8 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(3))) k_
   int lo, k;
   selection(M, want_all, lo, k);
   if (k == 0) return;
+  // FP32 steps: one solve (their T and eigenvalues carry FP32-level error)
+  const int sweeps = prec && prec[m] ? 1 : kInvitSweeps;
   double* d = lds;
   double* e = d + n;
   double* sh = e + n;                     // shifts, k
@@ -961,11 +964,11 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(3))) k_
     Y[t] = start_entry(lo + col, row);
   }
   __syncthreads();
-  for (int sweep = 0; sweep < kInvitSweeps; ++sweep) {
+  for (int sweep = 0; sweep < sweeps; ++sweep) {
     for (int j = tid; j < k; j += nt) {
       const double nrm = tri_solve(d, e, n, sh[j], tiny, Y, k, j, fac + (j % L), L);
       const bool single = cstart[j] == j && (j + 1 == k || cstart[j + 1] != j);
-      if (single && sweep == kInvitSweeps - 1) scale_column(Y, n, k, j, 1.0 / nrm);
+      if (single && sweep == sweeps - 1) scale_column(Y, n, k, j, 1.0 / nrm);
     }
     __threadfence_block();
     __syncthreads();
@@ -1453,7 +1456,8 @@ void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all) {
     const int kmax = all ? N : N / 2;
     const int threads = min(kT, max(64, (kmax + 63) / 64 * 64));
     k_invit<<<p.count, threads, inv_lds, s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm, b.W, all ? 1 : 0,
-                                              act, b.status, p.order);
+                                              act, b.status, p.order,
+                                              (all || !b.mixed) ? nullptr : b.prec);
   }
   for (int i = 0; i < b.n_large; ++i) {
     const int m = b.large_idx[i];
