@@ -77,11 +77,13 @@ def run_step(b: api.Batch, threshold: float, policy: str = "mixed"):
     return b.scf(threshold=threshold, max_steps=100, policy=policy)
 
 
-def pmc_traffic():
-    """HBM bytes per tridiagonalisation launch from the committed PMC summary."""
+def pmc_traffic(key: str = "traffic_bytes_per_launch"):
+    """HBM bytes per tridiagonalisation launch (FP64 launch, or key
+    traffic_bytes_per_launch_fp32 for the FP32 one) from the committed PMC
+    summary."""
     try:
         with open(PMC_SUMMARY) as f:
-            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read())
+            m = re.search(key + r":\s*([0-9.eE+]+)", f.read())
         return float(m.group(1)) if m else None
     except OSError:
         return None
@@ -304,9 +306,13 @@ def main():
         t32_avg_ms = t32_ms / t32_cnt
         t32_flops_launch = float(np.sum(s32 * (4.0 / 3.0) * N3)) * K / t32_cnt
         t32_tflops = t32_flops_launch / (t32_avg_ms * 1e-3) / 1e12
+        traffic32 = pmc_traffic("traffic_bytes_per_launch_fp32")
         tri32_roof = {"kernel": "k_tridiag<6, float>", "bound": "mfma",
                       "achieved": round(t32_tflops, 4), "peak": FP32_PEAK_TFLOPS,
                       "unit": "TFLOP/s", "frac": round(t32_tflops / FP32_PEAK_TFLOPS, 6),
+                      "traffic": traffic32,
+                      "traffic_rate_GBs": (round(traffic32 / (t32_avg_ms * 1e-3) / 1e9, 1)
+                                           if traffic32 else None),
                       "avg_launch_ms": round(t32_avg_ms, 4),
                       "flops_per_launch": t32_flops_launch,
                       "fp32_steps_mean": round(float(s32.mean()), 3),

@@ -99,11 +99,17 @@ int sg_scf_run(sg_batch* b, double threshold_ev, int max_steps, sg_scf_result* r
 
 /* Same SCF with a precision policy (no counterpart in the reference, which is
  * FP64 throughout).  SG_POLICY_FP64 is sg_scf_run.  SG_POLICY_MIXED runs the
- * Fock contractions and the density product in FP32 (density on the matrix
- * cores) until a molecule's |dE| < switch_ev (1e-3 eV suggested), or |dE| <
- * 2^-17 |E_el| (below what FP32 contractions resolve on that energy), or 40
- * FP32 steps; then in FP64 to convergence.  Integrals, eigensolver and energies
- * stay FP64, and convergence is only tested between two FP64 steps. */
+ * Fock contractions, the Householder reduction and back-transformation of the
+ * one-workgroup eigensolver (N <= 1280) and the density product in FP32 (the
+ * products on the matrix cores), with bisection to 1e-10 ||T|| and one
+ * inverse-iteration solve, until a molecule's |dE| < switch_ev (1e-3 eV
+ * suggested), or |dE| < 2^-17 |E_el| (below what FP32 contractions resolve on
+ * that energy), or 40 FP32 steps; then in FP64 to convergence.  Integrals,
+ * energies and the multi-workgroup eigensolver stay FP64, and convergence is
+ * only tested between two FP64 steps.  Batches above 512 molecules reduce
+ * their FP32 molecules in a launch of their own, whose FP32 sums run in
+ * another order: such a batch agrees with its molecules alone to the SCF
+ * threshold, not bit for bit (SG_POLICY_FP64 is bit-identical). */
 #define SG_POLICY_FP64 0
 #define SG_POLICY_MIXED 1
 int sg_scf_run_policy(sg_batch* b, double threshold_ev, int max_steps, int policy,

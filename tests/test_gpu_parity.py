@@ -4,7 +4,8 @@ Bars (also stated in DESIGN.md):
   pair blocks            bit-identical to the reference
   Hcore, Fock            |gpu - ref| <= 1e-10 * max|ref|  (normwise relative)
   energies               |gpu - ref| <= 1e-6 eV at SCF threshold 1e-6 eV, same step count
-  batch vs alone         bit-identical (no floating-point atomics, fixed reduction order)
+  batch vs alone         bit-identical (no floating-point atomics, fixed reduction order;
+                         mixed-policy batches above 512 molecules: the SCF threshold)
 Reference values are the committed fixtures written by the compiled reference
 (tests/golden); sizes beyond the fixtures are checked against the oracle.
 """
