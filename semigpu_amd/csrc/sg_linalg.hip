@@ -229,7 +229,11 @@ __device__ __forceinline__ void tridiag_mol(const MolDesc& M, const double* __re
   R* colbuf = qb + nwave * n;         // n: column c+1 of A, read during the product for c
   double* pan = (double*)(colbuf + nx);  // 3 x kNbMax: d, e, tau of the panel's columns
   const __amdgpu_buffer_rsrc_t arsrc = make_rsrc(A, (int64_t)n * n * (int64_t)sizeof(R));
-  for (int64_t t = tid; t < (int64_t)n * n; t += nt) A[t] = (R)f[t];
+  // the lower triangle only: the upper part of a row is read only after its
+  // reflector is stored there (diagonal update tiles carry their unread upper
+  // entries along, entry by entry)
+  for (int i = wave; i < n; i += nwave)
+    for (int l = lane; l <= i; l += 64) A[(int64_t)i * n + l] = (R)f[(int64_t)i * n + l];
   __syncthreads();
   const int nref = n > 2 ? n - 2 : 0;
   for (int k0 = 0; k0 < nref; k0 += nb) {
