@@ -103,8 +103,9 @@ int sg_scf_run(sg_batch* b, double threshold_ev, int max_steps, sg_scf_result* r
  * one-workgroup eigensolver (N <= 1280) and the density product in FP32 (the
  * products on the matrix cores), with bisection to 1e-10 ||T|| and one
  * inverse-iteration solve, until a molecule's |dE| < switch_ev (1e-3 eV
- * suggested), or |dE| < 2^-17 |E_el| (below what FP32 contractions resolve on
- * that energy), or 40 FP32 steps; then in FP64 to convergence.  Integrals,
+ * suggested), or |dE| < 2^-20 |E_el| (8 FP32 ulps of that energy), or |dE|
+ * stops falling by half per step, or 40 FP32 steps; then in FP64 to
+ * convergence.  Integrals,
  * energies and the multi-workgroup eigensolver stay FP64, and convergence is
  * only tested between two FP64 steps.  Batches above 512 molecules reduce
  * their FP32 molecules in a launch of their own, whose FP32 sums run in

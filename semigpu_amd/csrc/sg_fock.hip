@@ -24,7 +24,7 @@
 namespace sg {
 
 constexpr int kEnergyChunk = 16384;  // entries of a large molecule's E_el per workgroup
-constexpr double kFp32Floor = 7.62939453125e-06;  // 2^-17 = 64 ulps of FP32 at 1
+constexpr double kFp32Floor = 9.5367431640625e-07;  // 2^-20 = 8 ulps of FP32 at 1
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -315,7 +315,8 @@ __global__ void k_scf_init(int n_mol, int max_steps, int policy, const double* _
                            int* __restrict__ active, int* __restrict__ converged,
                            int* __restrict__ status, int* __restrict__ n_active,
                            int* __restrict__ prec, int* __restrict__ prev_prec,
-                           int* __restrict__ steps32, const int* __restrict__ mask) {
+                           int* __restrict__ steps32, double* __restrict__ de_prev,
+                           const int* __restrict__ mask) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m == 0 && !mask) *n_active = 0;
   if (m >= n_mol) return;
@@ -323,6 +324,7 @@ __global__ void k_scf_init(int n_mol, int max_steps, int policy, const double* _
   e_el[m] = e0[m];
   steps[m] = 0;
   if (steps32) steps32[m] = 0;
+  de_prev[m] = INFINITY;
   active[m] = max_steps > 0 ? 1 : 0;
   converged[m] = 0;
   status[m] = 0;
@@ -347,7 +349,7 @@ __global__ void k_scf_update(int n_mol, double threshold, int max_steps, double 
                              int* __restrict__ active, int* __restrict__ converged,
                              const int* __restrict__ status, int* __restrict__ n_active,
                              int* __restrict__ prec, int* __restrict__ prev_prec,
-                             int* __restrict__ steps32) {
+                             int* __restrict__ steps32, double* __restrict__ de_prev) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n_mol || !active[m]) return;
   if (status[m]) {
@@ -364,9 +366,14 @@ __global__ void k_scf_update(int n_mol, double threshold, int max_steps, double 
   const int conv = step_prec == 0 && prev_prec[m] == 0 && de < threshold;
   prev_prec[m] = step_prec;
   // FP32 steps end at |dE| < switch_ev, or once |dE| is below what FP32
-  // contractions resolve on this energy (kFp32Floor |E_el|, 64 FP32 ulps), or
-  // after fp32_cap steps
-  if (step_prec && (de < switch_ev || de < kFp32Floor * fabs(e) || st >= fp32_cap)) prec[m] = 0;
+  // arithmetic resolves on this energy (kFp32Floor |E_el|, 8 FP32 ulps), or
+  // stops falling by half per step (FP32 rounding, or a slow SCF, sets the
+  // pace), or after fp32_cap steps
+  const double de_last = de_prev[m];
+  de_prev[m] = de;
+  if (step_prec && (de < switch_ev || de < kFp32Floor * fabs(e) || de > 0.5 * de_last ||
+                    st >= fp32_cap))
+    prec[m] = 0;
   converged[m] = conv;
   const int still = !conv && st < max_steps;
   active[m] = still;
@@ -444,7 +451,7 @@ void launch_scf_init(const BatchDev& b, hipStream_t s, int max_steps, int policy
   k_scf_init<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(b.n_mol, max_steps, policy, b.e_new, b.e_el,
                                                       b.steps, b.active, b.converged, b.status,
                                                       b.n_active, b.prec, b.prev_prec, b.steps32,
-                                                      mask);
+                                                      b.de_prev, mask);
 }
 
 void launch_set_prec(const BatchDev& b, hipStream_t s, int v, const int* mask) {
@@ -456,7 +463,8 @@ void launch_scf_update(const BatchDev& b, hipStream_t s, double threshold, int m
   k_scf_update<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(b.n_mol, threshold, max_steps, switch_ev,
                                                         fp32_cap, b.e_new, b.e_el, b.steps,
                                                         b.active, b.converged, b.status, b.n_active,
-                                                        b.prec, b.prev_prec, b.steps32);
+                                                        b.prec, b.prev_prec, b.steps32,
+                                                        b.de_prev);
 }
 
 }  // namespace sg

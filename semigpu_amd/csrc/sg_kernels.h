@@ -62,6 +62,7 @@ struct BatchDev {
   int* prec = nullptr;       // precision of the next SCF step: 1 = FP32 contractions, 0 = FP64
   int* prev_prec = nullptr;  // precision of the step just done
   int* steps32 = nullptr;    // FP32 steps of the current SCF per molecule (null: not counted)
+  double* de_prev = nullptr; // |dE| of the molecule's previous SCF step
   // The SCF runs the mixed-precision policy: only then can a step's molecules
   // be in FP32, and the eigensolver launches its FP32 kernels.
   int mixed = 0;
