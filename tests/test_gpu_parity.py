@@ -197,6 +197,27 @@ def test_mixed_policy_large_batch_vs_alone(engine):
     b.close()
 
 
+def test_mixed_policy_split_launches_above_384(engine):
+    """The FP32 reduction for N > 384 (eight wavefronts, one workgroup per CU):
+    300 molecules, one with N = 420, take separate FP32 and FP64 launches;
+    alone, the N = 420 molecule takes the combined one.  Both within the SCF
+    threshold of each other and within 1e-4 eV of FP64."""
+    mols = geometry.water_batch(299, seed=8, n_min=2, n_max=6)
+    mols.insert(150, geometry.water_cluster(70, geometry.SplitMix64(70)))  # N = 420
+    b = engine.batch(mols)
+    mix = b.scf(threshold=1e-6, policy="mixed")
+    s32 = b.fp32_steps()
+    ref = b.scf(threshold=1e-6, policy="fp64")
+    assert all(r["converged"] for r in mix)
+    assert max(abs(a["e_tot"] - c["e_tot"]) for a, c in zip(ref, mix)) <= 1e-4
+    assert s32[150] >= 1
+    alone = engine.batch([mols[150]]).scf(threshold=1e-6, policy="mixed")[0]
+    say("N=420 mixed batch", mix[150]["steps"], "alone", alone["steps"],
+        alone["e_tot"] - mix[150]["e_tot"], "vs fp64", mix[150]["e_tot"] - ref[150]["e_tot"])
+    assert alone["converged"] and abs(alone["e_tot"] - mix[150]["e_tot"]) <= 1e-6
+    b.close()
+
+
 def test_graph_replay_equals_eager(engine):
     """The captured SCF iteration (replayed once per step, with eager setup
     work and host synchronisation between replays) gives the eager results."""
