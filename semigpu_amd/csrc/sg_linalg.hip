@@ -168,11 +168,12 @@ __device__ __forceinline__ void swap16(float& x, float& y) {
   x = __int_as_float(r[0]);
   y = __int_as_float(r[1]);
 }
-// 64-column tiles of a row block whose loads are issued together (3 or 6
-// in either type measured slower: more registers, no fewer stalls)
+// 64-column tiles of a row block whose loads are issued together: one in
+// FP64 up to N = 384 (3 % faster than two: the 16 extra registers of a second
+// tile spill), else two (three or six measured slower in either type)
 template <int TMAX, typename R>
 constexpr int symv_group() {
-  return 2;
+  return (sizeof(R) == 8 && TMAX <= 6) ? 1 : 2;
 }
 constexpr int kOutside = 0x7ffffff0;  // byte offset past any matrix buffer
 
