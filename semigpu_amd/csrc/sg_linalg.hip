@@ -1063,9 +1063,17 @@ __global__ void __launch_bounds__(256) k_li_mgs(const MolDesc* __restrict__ mols
 // V is row q = reflector k0+q (stored in the upper part of row k0+q of the
 // work matrix, zero at columns <= k0+q).  LDS: V (G x n), W2 (G x k) sharing
 // room with the partial Gram tiles, T and the Gram matrix (16 x 16 each).
-__device__ __forceinline__ int bt_group(int n, int k, int lds_elems) {
+// G is the largest of 16, 12, 8, 4 whose LDS fits kBackBudget elements of the
+// working type: it depends on the molecule's n and k only (so a molecule's
+// result does not depend on its batch), and the launch allocates what its
+// largest molecule needs (two workgroups per CU up to N = 384, where G = 16).
+constexpr int kBackBudget = 20480;  // doubles (160 KB); twice as many floats
+__host__ __device__ inline int bt_need(int g, int n, int k) {
+  return g * n + (g * k > 2048 ? g * k : 2048) + 512;
+}
+__host__ __device__ inline int bt_group(int n, int k, int budget) {
   int g = 16;
-  while (g > 4 && g * n + max(g * k, 2048) + 512 > lds_elems) g -= 4;
+  while (g > 4 && bt_need(g, n, k) > budget) g -= 4;
   return g;
 }
 
@@ -1076,14 +1084,14 @@ template <typename R>
 __device__ __forceinline__ void back_mol(const MolDesc& M, const R* __restrict__ A,
                                          const double* __restrict__ Ysrc, R* __restrict__ Y,
                                          const double* __restrict__ tau, int lo, int k,
-                                         int lds_elems, R* lds) {
+                                         R* lds) {
   typedef Mfma16<R> Mf;
   typedef typename Mf::v4 v4;
   const int n = M.n_ao, tid = threadIdx.x, nt = blockDim.x;
   const int wave = tid >> 6, lane = tid & 63, nwave = nt >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int nref = n > 2 ? n - 2 : 0;
-  const int G = bt_group(n, k, lds_elems);
+  const int G = bt_group(n, k, kBackBudget * (int)(8 / sizeof(R)));
   const int rg = max(G * k, 2048);
   R* Vs = lds;          // G x n
   R* Rw = Vs + G * n;   // W2 (G x k), before that nwave partial 16x16 Gram tiles
@@ -1199,7 +1207,6 @@ __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict_
                                                        const int* __restrict__ active,
                                                        const int* __restrict__ status,
                                                        const int* __restrict__ order,
-                                                       int lds_doubles,
                                                        const int* __restrict__ prec) {
   extern __shared__ double lds[];
   const int m = order ? order[blockIdx.x] : blockIdx.x;
@@ -1213,9 +1220,9 @@ __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict_
   double* Y = base + (int64_t)n * n;
   if (prec && prec[m])
     back_mol<float>(M, (const float*)base, Y, (float*)(base + 2 * (int64_t)n * n),
-                    tauA + M.vec0, lo, k, 2 * lds_doubles, (float*)lds);
+                    tauA + M.vec0, lo, k, (float*)lds);
   else
-    back_mol<double>(M, base, Y, Y, tauA + M.vec0, lo, k, lds_doubles, lds);
+    back_mol<double>(M, base, Y, Y, tauA + M.vec0, lo, k, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1484,13 +1491,16 @@ void launch_eigh_back(const BatchDev& b, hipStream_t s, bool all) {
   const int* act = all ? nullptr : b.active;
   const Part p = small_part(b);
   if (p.count > 0) {
-    // 80 KB (two workgroups per CU) unless even groups of 4 reflectors need more
-    const int N = p.max_n;
-    const int kmax = all ? N : N / 2 + 1;
-    const int bt_doubles = 4 * (N + kmax) + 2560 > 10240 ? 20480 : 10240;
-    k_backtransform<<<p.count, kT, sizeof(double) * bt_doubles, s>>>(
-        b.mols, b.W, b.tau, all ? 1 : 0, act, b.status, p.order, bt_doubles,
-        all ? nullptr : b.prec);
+    // LDS for the largest need of any molecule size up to max_n, in either type
+    size_t bytes = 0;
+    for (int n = 1; n <= p.max_n; ++n) {
+      const int k = all ? n : n / 2 + 1;
+      const size_t b64 = 8 * (size_t)bt_need(bt_group(n, k, kBackBudget), n, k);
+      const size_t b32 = 4 * (size_t)bt_need(bt_group(n, k, 2 * kBackBudget), n, k);
+      bytes = std::max(bytes, std::max(b64, b32));
+    }
+    k_backtransform<<<p.count, kT, bytes, s>>>(b.mols, b.W, b.tau, all ? 1 : 0, act, b.status,
+                                               p.order, all ? nullptr : b.prec);
   }
   for (int i = 0; i < b.n_large; ++i) launch_back_large(b, s, b.large_idx[i], b.large_mols[i], act, all);
   if (all) {
