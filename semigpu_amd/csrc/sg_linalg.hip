@@ -119,14 +119,22 @@ __device__ double block_sum(double v, double* red) {
 }
 
 // Reflector panel: V and W (nb columns of length n each) fill at most
-// panel_bytes<R>() of LDS, the same width in either type.
+// panel_bytes<R>(n) of LDS, the same width in either type: 48 KB in FP64 up
+// to N = kWideN, 80 KB above (where the wavefronts' partial products take
+// half the room, one slot per pair of wavefronts), so that large molecules
+// get panels of 4..8 reflectors instead of 2..3 (each panel's update re-reads
+// and rewrites the trailing block).  Up to N = 384 the panel is 8 wide either
+// way, so a molecule's arithmetic does not depend on the launch it is in.
+constexpr int kWideN = 384;
+constexpr int kPanelLdsWide = 80 * 1024;
 template <typename R>
-__host__ __device__ constexpr int panel_bytes() {
-  return kPanelLds / 8 * (int)sizeof(R);
+__host__ __device__ constexpr int panel_bytes(int n) {
+  return (n > kWideN ? kPanelLdsWide : kPanelLds) / 8 * (int)sizeof(R);
 }
+// Width for n rows of a panel of panel_bytes<R>(nsize) bytes.
 template <typename R>
-__device__ __forceinline__ int panel_width(int n) {
-  int nb = panel_bytes<R>() / (2 * (int)sizeof(R) * (n > 0 ? n : 1));
+__device__ __forceinline__ int panel_width(int nsize, int n) {
+  int nb = panel_bytes<R>(nsize) / (2 * (int)sizeof(R) * (n > 0 ? n : 1));
   return nb < 1 ? 1 : (nb > kNbMax ? kNbMax : nb);
 }
 
@@ -208,9 +216,10 @@ struct Mfma16<float> {
 // double, or float for the FP32 steps of the mixed-precision policy (matrix,
 // panel and products in FP32; norms, the reflector scalars and d, e, tau in
 // FP64).  LDS: the V/W panel (panel_bytes), x (n R, rounded up to an even
-// count), 16 + 16 doubles (V^T v / W^T v, block reductions), the wavefronts'
-// partial transposed products (8 n R), the next column (n R, even count) and
-// 3 kNbMax doubles (the panel's d, e, tau).
+// count), 16 + 16 doubles (V^T v / W^T v, block reductions), the partial
+// transposed products (nq n R: one slot per wavefront, per pair above
+// kWideN), the next column (n R, even count) and 3 kNbMax doubles (the
+// panel's d, e, tau).
 template <int TMAX, typename R>
 __device__ __forceinline__ void tridiag_mol(const MolDesc& M, const double* __restrict__ f,
                                             R* __restrict__ A, double* __restrict__ d,
@@ -219,15 +228,17 @@ __device__ __forceinline__ void tridiag_mol(const MolDesc& M, const double* __re
   typedef Mfma16<R> Mf;
   const int n = M.n_ao, tid = threadIdx.x, nt = blockDim.x;
   const int wave = tid >> 6, lane = tid & 63, nwave = nt >> 6;
-  const int nb = panel_width<R>(n);
+  const int pbytes = panel_bytes<R>(TMAX > 6 ? n : 0);  // TMAX = 6: n <= kWideN
+  const int nb = panel_width<R>(TMAX > 6 ? n : 0, n);
   R* V = (R*)L;                      // nb columns of length n (column-major)
   R* Wp = V + (size_t)nb * n;        // nb columns of length n
   const int nx = sizeof(R) == 8 ? n : (n + 1) & ~1;  // keeps the doubles 8-byte aligned
-  R* x = (R*)(L + panel_bytes<R>());  // n
+  const int nq = (TMAX > 6 && n > kWideN) ? nwave / 2 : nwave;  // partial-product slots
+  R* x = (R*)(L + pbytes);            // n
   double* tv = (double*)(x + nx);     // 2 * nb <= 16
   double* red = tv + 16;              // 2 x nwave: the two block-sum buffers
-  R* qb = (R*)(red + 16);             // nwave x n partial transposed products
-  R* colbuf = qb + nwave * n;         // n: column c+1 of A, read during the product for c
+  R* qb = (R*)(red + 16);             // nq x n partial transposed products
+  R* colbuf = qb + nq * n;            // n: column c+1 of A, read during the product for c
   double* pan = (double*)(colbuf + nx);  // 3 x kNbMax: d, e, tau of the panel's columns
   const __amdgpu_buffer_rsrc_t arsrc = make_rsrc(A, (int64_t)n * n * (int64_t)sizeof(R));
   // the lower triangle only: the upper part of a row is read only after its
@@ -401,10 +412,30 @@ __device__ __forceinline__ void tridiag_mol(const MolDesc& M, const double* __re
           const int rho = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
           if ((lane & 7) == 0 && i0 + rho < n) x[i0 + rho] = v1;
         }
+        if (TMAX <= 6 || nq == nwave) {
 #pragma unroll
-        for (int t = 0; t < TMAX; ++t) {
-          const int l = m1 + lane + 64 * t;
-          if (t < T && l < n) qb[wave * n + l] = q[t];
+          for (int t = 0; t < TMAX; ++t) {
+            const int l = m1 + lane + 64 * t;
+            if (t < T && l < n) qb[wave * n + l] = q[t];
+          }
+        } else if constexpr (TMAX > 6) {
+          // the second half of the wavefronts adds its products to the first
+          // half's slots
+          if (wave < nq) {
+#pragma unroll
+            for (int t = 0; t < TMAX; ++t) {
+              const int l = m1 + lane + 64 * t;
+              if (t < T && l < n) qb[wave * n + l] = q[t];
+            }
+          }
+          __syncthreads();
+          if (wave >= nq) {
+#pragma unroll
+            for (int t = 0; t < TMAX; ++t) {
+              const int l = m1 + lane + 64 * t;
+              if (t < T && l < n) qb[(wave - nq) * n + l] += q[t];
+            }
+          }
         }
       }
       // W^T v and V^T v for the previous columns of the panel
@@ -419,7 +450,7 @@ __device__ __forceinline__ void tridiag_mol(const MolDesc& M, const double* __re
       double pv = 0.0;
       for (int i = c + 1 + tid; i < n; i += nt) {
         R s = x[i];
-        for (int w = 0; w < nwave; ++w) s += qb[w * n + i];
+        for (int w = 0; w < nq; ++w) s += qb[w * n + i];
         for (int qq = 0; qq < j; ++qq)
           s -= V[qq * n + i] * (R)tv[qq] + Wp[qq * n + i] * (R)tv[j + qq];
         s *= (R)tk;
@@ -1367,11 +1398,18 @@ __global__ void k_write_eigvec(const MolDesc* __restrict__ mols, const double* _
 }
 
 // Panel, 32 + 3 kNbMax scalars (FP64), x and the next column (each rounded up
-// to an even count), one partial product per wavefront.
+// to an even count), the partial-product slots: the most any molecule size up
+// to max_n needs (the panel and the slots change at kWideN).
 template <int TMAX, typename R>
 static size_t tridiag_lds(int max_n) {
-  return (size_t)panel_bytes<R>() + sizeof(double) * (32 + 3 * kNbMax) +
-         sizeof(R) * ((2 + tridiag_threads<TMAX, R>() / 64) * (size_t)max_n + 2);
+  const int nw = tridiag_threads<TMAX, R>() / 64;
+  size_t bytes = 0;
+  for (int n = 1; n <= max_n; ++n) {
+    const int nq = n > kWideN ? nw / 2 : nw;
+    bytes = std::max(bytes, (size_t)panel_bytes<R>(n) + sizeof(double) * (32 + 3 * kNbMax) +
+                                sizeof(R) * ((2 + nq) * (size_t)n + 2));
+  }
+  return bytes;
 }
 
 // The one-workgroup kernels take the molecules below large_min: the tail of
