@@ -178,10 +178,11 @@ __device__ __forceinline__ void swap16(float& x, float& y) {
 }
 // 64-column tiles of a row block whose loads are issued together: one in
 // FP64 up to N = 384 (3 % faster than two: the 16 extra registers of a second
-// tile spill), else two (three or six measured slower in either type)
+// tile spill), four in FP32 above N = 384 (one CU's molecule: more loads in
+// flight pay there), else two (three to six measured slower)
 template <int TMAX, typename R>
 constexpr int symv_group() {
-  return (sizeof(R) == 8 && TMAX <= 6) ? 1 : 2;
+  return TMAX <= 6 ? (sizeof(R) == 8 ? 1 : 2) : (sizeof(R) == 4 ? 4 : 2);
 }
 constexpr int kOutside = 0x7ffffff0;  // byte offset past any matrix buffer
 
