@@ -62,6 +62,24 @@ def test_server_mixed_policy_equals_alone(engine):
         assert {k: out[i][k] for k in KEYS} == alone, i
 
 
+def test_server_mixed_policy_large_molecules_equal_alone(engine):
+    """Mixed policy with molecules above N = 384 (wide reduction panels, FP32
+    steps in the combined launch) in one class with small ones."""
+    mols = [geometry.water_cluster(80, geometry.SplitMix64(80)),      # N = 480
+            geometry.water_cluster(110, geometry.SplitMix64(110))]    # N = 660
+    mols += geometry.water_batch(6, seed=31, n_min=2, n_max=20)
+    srv = engine.server(hbm_bytes=512e6, max_slots=8, max_atoms=1024, threshold=1e-6,
+                        policy="mixed")
+    for i, m in enumerate(mols):
+        srv.submit(i, m)
+    out = {r["id"]: r for r in srv.drain()}
+    srv.close()
+    for i, m in enumerate(mols):
+        alone = engine.batch([m]).scf(threshold=1e-6, policy="mixed")[0]
+        say("mixed server", i, geometry.n_basis(m), out[i]["steps"], alone["steps"])
+        assert {k: out[i][k] for k in KEYS} == alone, i
+
+
 def test_server_fixtures(engine):
     names = golden_names()
     srv = engine.server(hbm_bytes=256e6, max_slots=16, max_atoms=4096, threshold=1e-6)
