@@ -545,8 +545,8 @@ __device__ __forceinline__ void tridiag_mol(const MolDesc& M, const double* __re
 // four wavefronts with half the panel and three workgroups per CU measured
 // slower); FP32 four wavefronts, four workgroups per CU (34 KB each), faster
 // than two of eight.  Above N = 384 eight wavefronts in either type and one
-// workgroup per CU (the LDS holds one), with the budget at two waves per SIMD
-// (no spills of the 20 row accumulators).
+// workgroup per CU (in FP64 the LDS holds one), with the budget at two waves
+// per SIMD (no spills of the 20 row accumulators).
 template <int TMAX, typename R>
 constexpr int tridiag_waves() {
   return TMAX > 6 ? 2 : 4;
