@@ -266,7 +266,7 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
     return o;
   };
   const size_t o_mols = take(sizeof(MolDesc) * n_mol), o_atoms = take(sizeof(AtomDesc) * n_atoms),
-               o_partners = take(sizeof(int) * n_atoms);
+               o_partners = take(sizeof(AtomDesc) * n_atoms);
   const size_t o_xx = take(sizeof(int2) * pxx.size()), o_hx = take(sizeof(int2) * phx.size()),
                o_hh = take(sizeof(int2) * phh.size()), o_all = take(sizeof(int2) * pall.size()),
                o_diag = take(sizeof(int2) * diag.size());
@@ -294,7 +294,7 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   char* base = (char*)b->dev_mem;
   d.mols = (MolDesc*)(base + o_mols);
   d.atoms = (AtomDesc*)(base + o_atoms);
-  d.partners = (int*)(base + o_partners);
+  d.partners = (AtomDesc*)(base + o_partners);
   d.pairs_xx = (int2*)(base + o_xx);
   d.pairs_hx = (int2*)(base + o_hx);
   d.pairs_hh = (int2*)(base + o_hh);
@@ -334,15 +334,15 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   e = up(d.mols, b->mols.data(), sizeof(MolDesc) * n_mol);
   if (e == hipSuccess) e = up(d.atoms, b->atoms.data(), sizeof(AtomDesc) * n_atoms);
   // partner order of the Fock build: each molecule's heavy atoms, then its hydrogens
-  std::vector<int> partners;
+  std::vector<AtomDesc> partners;
   partners.reserve(n_atoms);
   for (int m = 0; m < n_mol; ++m) {
     const MolDesc& M = b->mols[m];
     for (int pass = 0; pass < 2; ++pass)
       for (int a = 0; a < M.n_atoms; ++a)
-        if ((b->atoms[M.atom0 + a].zn > 1) == (pass == 0)) partners.push_back(M.atom0 + a);
+        if ((b->atoms[M.atom0 + a].zn > 1) == (pass == 0)) partners.push_back(b->atoms[M.atom0 + a]);
   }
-  if (e == hipSuccess) e = up(d.partners, partners.data(), sizeof(int) * n_atoms);
+  if (e == hipSuccess) e = up(d.partners, partners.data(), sizeof(AtomDesc) * n_atoms);
   if (e == hipSuccess) e = up(d.pairs_xx, pxx.data(), sizeof(int2) * pxx.size());
   if (e == hipSuccess) e = up(d.pairs_hx, phx.data(), sizeof(int2) * phx.size());
   if (e == hipSuccess) e = up(d.pairs_hh, phh.data(), sizeof(int2) * phh.size());

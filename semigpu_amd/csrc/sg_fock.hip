@@ -116,7 +116,7 @@ __device__ __forceinline__ void write_offdiag(const double* __restrict__ h,
 // registers while the other partners run).
 template <bool HA, typename T, typename S>
 __device__ double fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc* __restrict__ atoms,
-                            const int* __restrict__ partners, const S* __restrict__ blk,
+                            const AtomDesc* __restrict__ partners, const S* __restrict__ blk,
                             const double* __restrict__ h, const double* __restrict__ p,
                             double* __restrict__ f, double* __restrict__ kaa, int lane) {
   const int N = M.n_ao;
@@ -128,7 +128,7 @@ __device__ double fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc*
   for (int b0 = 0; b0 < M.n_atoms; b0 += 64) {
     const int bl = b0 + lane;
     if (bl >= M.n_atoms) continue;
-    const AtomDesc B = atoms[partners[M.atom0 + bl]];
+    const AtomDesc B = partners[M.atom0 + bl];
     const bool row = B.local <= A.local;
     const S* b = blk + (row ? block_offset(A, B) : block_offset(B, A));
     if (row) {
@@ -177,7 +177,7 @@ __device__ double fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc*
 }
 
 __global__ void __launch_bounds__(256) k_fock(int n_atoms, const AtomDesc* __restrict__ atoms,
-                                               const int* __restrict__ partners,
+                                               const AtomDesc* __restrict__ partners,
                                                const MolDesc* __restrict__ mols,
                                                const double* __restrict__ blocks,
                                                const float* __restrict__ blocks32,

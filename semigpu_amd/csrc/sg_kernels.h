@@ -29,9 +29,10 @@ struct BatchDev {
   int64_t n_blocks = 0;  // sum of pair-block doubles
   MolDesc* mols = nullptr;
   AtomDesc* atoms = nullptr;
-  // per molecule, over its atom range: global atom indices of its heavy atoms,
-  // then its hydrogens (the Fock build's partner order)
-  int* partners = nullptr;
+  // per molecule, over its atom range: copies of the descriptors of its heavy
+  // atoms, then its hydrogens (the Fock build's partner order; a copy rather
+  // than an index saves the Fock wavefronts one dependent load)
+  AtomDesc* partners = nullptr;
   // type-sorted atom-pair lists (row atom, column atom), row local > column local
   int2* pairs_xx = nullptr; int n_xx = 0;
   int2* pairs_hx = nullptr; int n_hx = 0;

@@ -147,7 +147,7 @@ struct Lane {
   // pinned mirrors: descriptors, mask, order, pair-list staging, readback
   Pinned<MolDesc> mols;
   Pinned<AtomDesc> atoms;
-  Pinned<int> partners;  // the Fock build's partner order per molecule
+  Pinned<AtomDesc> partners;  // the Fock build's partner order per molecule (descriptor copies)
   Pinned<int> mask, order;
   Pinned<int2> pairs;
   Pinned<int> rb_act, rb_steps, rb_conv, rb_stat;
@@ -212,7 +212,8 @@ int lane_create(const BatchDev& shared, Lane* L, int max_slots, int max_atoms, i
   };
   const size_t ns = max_slots + 1;  // + dead slot
   const size_t o_mols = take(sizeof(MolDesc) * ns), o_atoms = take(sizeof(AtomDesc) * max_atoms);
-  const size_t o_eatom = take(8 * (size_t)max_atoms), o_partners = take(4 * (size_t)max_atoms);
+  const size_t o_eatom = take(8 * (size_t)max_atoms),
+               o_partners = take(sizeof(AtomDesc) * (size_t)max_atoms);
   const size_t o_pairs = take(sizeof(int2) * L->pair_cap);
   const size_t o_tn = take(8 * ns), o_ecc = take(8 * ns), o_eel = take(8 * ns), o_enew = take(8 * ns);
   const size_t o_steps = take(4 * ns), o_act = take(4 * ns), o_conv = take(4 * ns),
@@ -233,7 +234,7 @@ int lane_create(const BatchDev& shared, Lane* L, int max_slots, int max_atoms, i
   d.mols = (MolDesc*)(base + o_mols);
   d.atoms = (AtomDesc*)(base + o_atoms);
   d.e_atom = (double*)(base + o_eatom);
-  d.partners = (int*)(base + o_partners);
+  d.partners = (AtomDesc*)(base + o_partners);
   L->pair_scratch = (int2*)(base + o_pairs);
   d.tnorm = (double*)(base + o_tn);
   d.e_cc = (double*)(base + o_ecc);
@@ -400,7 +401,7 @@ int lane_launch(sg_server* S, Lane* L) {
     int np = 0;
     for (int pass = 0; pass < 2; ++pass)
       for (int a = 0; a < na; ++a)
-        if ((r.Z[a] > 1) == (pass == 0)) L->partners[a0 + np++] = (int)a0 + a;
+        if ((r.Z[a] > 1) == (pass == 0)) L->partners[a0 + np++] = L->atoms[a0 + a];
     sl.req = std::move(r);
     L->queue.pop_front();
     L->fresh.push_back(slot);
@@ -441,7 +442,7 @@ int lane_launch(sg_server* S, Lane* L) {
       HIP_TRY(hipMemcpyAsync(L->d.atoms + M.atom0, L->atoms.p + M.atom0,
                              sizeof(AtomDesc) * M.n_atoms, hipMemcpyHostToDevice, s));
       HIP_TRY(hipMemcpyAsync(L->d.partners + M.atom0, L->partners.p + M.atom0,
-                             sizeof(int) * M.n_atoms, hipMemcpyHostToDevice, s));
+                             sizeof(AtomDesc) * M.n_atoms, hipMemcpyHostToDevice, s));
     }
     HIP_TRY(hipMemcpyAsync(L->mask_new, L->mask.p, sizeof(int) * (L->max_slots + 1),
                            hipMemcpyHostToDevice, s));
