@@ -283,7 +283,8 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
                o_conv = take(sizeof(int) * n_mol), o_stat = take(sizeof(int) * n_mol),
                o_nact = take(sizeof(int) * 64), o_prec = take(sizeof(int) * n_mol),
                o_pprec = take(sizeof(int) * n_mol), o_order = take(sizeof(int) * n_mol),
-               o_s32 = take(sizeof(int) * n_mol), o_dep = take(sizeof(double) * n_mol);
+               o_s32 = take(sizeof(int) * n_mol), o_dep = take(sizeof(double) * n_mol),
+               o_wdiag = take(sizeof(double) * 10 * (size_t)n_atoms);
   (void)hipSetDevice(ctx->device);
   hipError_t e = hipMalloc(&b->dev_mem, off);
   if (e != hipSuccess) {
@@ -325,6 +326,7 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   d.prev_prec = (int*)(base + o_pprec);
   d.steps32 = (int*)(base + o_s32);
   d.de_prev = (double*)(base + o_dep);
+  d.wdiag = (double*)(base + o_wdiag);
   int* order = (int*)(base + o_order);
   d.order = order;
   hipStream_t s = ctx->stream;

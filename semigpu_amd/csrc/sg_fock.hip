@@ -52,16 +52,13 @@ __device__ __forceinline__ S blk_entry(const S* __restrict__ b, int ka, int kb) 
 // mixed-precision policy, where S = float reads the FP32 copy of the blocks).
 template <bool HA, bool HB, bool ROW, typename T, typename S>
 __device__ __forceinline__ void partner(const S* __restrict__ b, const double* __restrict__ p,
-                                        int N, int ao_a, int ao_b, T (&J)[10], T (&K)[4][4]) {
+                                        const double* __restrict__ wb, int N, int ao_a, int ao_b,
+                                        T (&J)[10], T (&K)[4][4]) {
   constexpr int NA = HA ? 4 : 1, NB = HB ? 4 : 1;
   constexpr int SA = HA ? 10 : 1, SB = HB ? 10 : 1;
   T w[SB];
 #pragma unroll
-  for (int l = 0; l < SB; ++l) {
-    const int x = mop_a(l), y = mop_b(l);
-    const T v = (T)p[(int64_t)(ao_b + x) * N + ao_b + y];
-    w[l] = x == y ? v : (T)2 * v;
-  }
+  for (int l = 0; l < SB; ++l) w[l] = (T)wb[l];
   T pab[NA][NB];
   if constexpr (ROW) {
 #pragma unroll
@@ -116,7 +113,8 @@ __device__ __forceinline__ void write_offdiag(const double* __restrict__ h,
 // registers while the other partners run).
 template <bool HA, typename T, typename S>
 __device__ double fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc* __restrict__ atoms,
-                            const AtomDesc* __restrict__ partners, const S* __restrict__ blk,
+                            const AtomDesc* __restrict__ partners, const double* __restrict__ wdiag,
+                            const S* __restrict__ blk,
                             const double* __restrict__ h, const double* __restrict__ p,
                             double* __restrict__ f, double* __restrict__ kaa, int lane) {
   const int N = M.n_ao;
@@ -129,6 +127,7 @@ __device__ double fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc*
     const int bl = b0 + lane;
     if (bl >= M.n_atoms) continue;
     const AtomDesc B = partners[M.atom0 + bl];
+    const double* wb = wdiag + (int64_t)(M.atom0 + B.local) * 10;
     const bool row = B.local <= A.local;
     const S* b = blk + (row ? block_offset(A, B) : block_offset(B, A));
     if (row) {
@@ -137,8 +136,8 @@ __device__ double fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc*
       for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 4; ++y) K[x][y] = (T)0;
-      if (B.zn > 1) partner<HA, true, true>(b, p, N, A.ao0, B.ao0, J, K);
-      else partner<HA, false, true>(b, p, N, A.ao0, B.ao0, J, K);
+      if (B.zn > 1) partner<HA, true, true>(b, p, wb, N, A.ao0, B.ao0, J, K);
+      else partner<HA, false, true>(b, p, wb, N, A.ao0, B.ao0, J, K);
       if (B.local == A.local) {
 #pragma unroll
         for (int x = 0; x < 4; ++x)
@@ -152,8 +151,8 @@ __device__ double fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc*
       }
     } else {
       T K[4][4];
-      if (B.zn > 1) partner<HA, true, false>(b, p, N, A.ao0, B.ao0, J, K);
-      else partner<HA, false, false>(b, p, N, A.ao0, B.ao0, J, K);
+      if (B.zn > 1) partner<HA, true, false>(b, p, wb, N, A.ao0, B.ao0, J, K);
+      else partner<HA, false, false>(b, p, wb, N, A.ao0, B.ao0, J, K);
     }
   }
   constexpr int SA = HA ? 10 : 1;
@@ -176,8 +175,30 @@ __device__ double fock_atom(const AtomDesc& A, const MolDesc& M, const AtomDesc*
   return wave_sum(e);
 }
 
+// Coulomb weights of each atom's diagonal density block, in the Fock build's
+// order: w(l) = P_l, doubled for the off-diagonal AO pairs (exact in FP32 too).
+__global__ void k_wdiag(int n_atoms, const AtomDesc* __restrict__ atoms,
+                        const MolDesc* __restrict__ mols, const double* __restrict__ P,
+                        double* __restrict__ wdiag, const int* __restrict__ active,
+                        const int* __restrict__ status) {
+  const int ia = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= n_atoms) return;
+  const AtomDesc A = atoms[ia];
+  if (active && (!active[A.mol] || status[A.mol])) return;
+  const MolDesc M = mols[A.mol];
+  const int N = M.n_ao;
+  const double* p = P + M.mat0;
+  const int sb = A.zn > 1 ? 10 : 1;
+  for (int l = 0; l < sb; ++l) {
+    const int x = mop_a(l), y = mop_b(l);
+    const double v = p[(int64_t)(A.ao0 + x) * N + A.ao0 + y];
+    wdiag[(int64_t)ia * 10 + l] = x == y ? v : 2.0 * v;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_fock(int n_atoms, const AtomDesc* __restrict__ atoms,
                                                const AtomDesc* __restrict__ partners,
+                                               const double* __restrict__ wdiag,
                                                const MolDesc* __restrict__ mols,
                                                const double* __restrict__ blocks,
                                                const float* __restrict__ blocks32,
@@ -201,12 +222,12 @@ __global__ void __launch_bounds__(256) k_fock(int n_atoms, const AtomDesc* __res
   double e;
   if (prec && prec[A.mol]) {
     const float* blk = blocks32 + M.blk0;
-    if (A.zn > 1) e = fock_atom<true, float>(A, M, atoms, partners, blk, h, p, f, kaa, lane);
-    else e = fock_atom<false, float>(A, M, atoms, partners, blk, h, p, f, kaa, lane);
+    if (A.zn > 1) e = fock_atom<true, float>(A, M, atoms, partners, wdiag, blk, h, p, f, kaa, lane);
+    else e = fock_atom<false, float>(A, M, atoms, partners, wdiag, blk, h, p, f, kaa, lane);
   } else {
     const double* blk = blocks + M.blk0;
-    if (A.zn > 1) e = fock_atom<true, double>(A, M, atoms, partners, blk, h, p, f, kaa, lane);
-    else e = fock_atom<false, double>(A, M, atoms, partners, blk, h, p, f, kaa, lane);
+    if (A.zn > 1) e = fock_atom<true, double>(A, M, atoms, partners, wdiag, blk, h, p, f, kaa, lane);
+    else e = fock_atom<false, double>(A, M, atoms, partners, wdiag, blk, h, p, f, kaa, lane);
   }
   if (lane == 0) e_atom[ia] = e;
 }
@@ -409,10 +430,12 @@ void launch_guess_density(const BatchDev& b, hipStream_t s, const int* mask) {
 }
 
 void launch_fock(const BatchDev& b, hipStream_t s, bool all, const int* mask) {
-  k_fock<<<blocks_for(b.n_atoms, 4), 256, 0, s>>>(b.n_atoms, b.atoms, b.partners, b.mols, b.blocks,
-                                                  b.blocks32, b.H, b.P, b.F, b.e_atom,
-                                                  mask ? mask : (all ? nullptr : b.active), b.status,
-                                                  all ? nullptr : b.prec);
+  const int* act = mask ? mask : (all ? nullptr : b.active);
+  k_wdiag<<<blocks_for(b.n_atoms, 256), 256, 0, s>>>(b.n_atoms, b.atoms, b.mols, b.P, b.wdiag, act,
+                                                     b.status);
+  k_fock<<<blocks_for(b.n_atoms, 4), 256, 0, s>>>(b.n_atoms, b.atoms, b.partners, b.wdiag, b.mols,
+                                                  b.blocks, b.blocks32, b.H, b.P, b.F, b.e_atom,
+                                                  act, b.status, all ? nullptr : b.prec);
 }
 
 void launch_energy_fock(const BatchDev& b, hipStream_t s, double* out, bool all, const int* mask) {

@@ -64,6 +64,9 @@ struct BatchDev {
   int* prev_prec = nullptr;  // precision of the step just done
   int* steps32 = nullptr;    // FP32 steps of the current SCF per molecule (null: not counted)
   double* de_prev = nullptr; // |dE| of the molecule's previous SCF step
+  // per atom, 10 entries: the Coulomb weights w(l) of its diagonal density
+  // block (P_l, doubled off the diagonal), gathered before each Fock build
+  double* wdiag = nullptr;
   // The SCF runs the mixed-precision policy: only then can a step's molecules
   // be in FP32, and the eigensolver launches its FP32 kernels.
   int mixed = 0;

@@ -219,7 +219,7 @@ int lane_create(const BatchDev& shared, Lane* L, int max_slots, int max_atoms, i
   const size_t o_steps = take(4 * ns), o_act = take(4 * ns), o_conv = take(4 * ns),
                o_stat = take(4 * ns), o_nact = take(256), o_prec = take(4 * ns),
                o_pprec = take(4 * ns), o_mask = take(4 * ns), o_order = take(4 * ns),
-               o_dep = take(8 * ns);
+               o_dep = take(8 * ns), o_wdiag = take(80 * (size_t)max_atoms);
   HIP_TRY(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreateWithFlags(&L->ev, hipEventDisableTiming));
   hipError_t e = hipMalloc(&L->dev, off);
@@ -248,6 +248,7 @@ int lane_create(const BatchDev& shared, Lane* L, int max_slots, int max_atoms, i
   d.prec = (int*)(base + o_prec);
   d.prev_prec = (int*)(base + o_pprec);
   d.de_prev = (double*)(base + o_dep);
+  d.wdiag = (double*)(base + o_wdiag);
   L->mask_new = (int*)(base + o_mask);
   L->order_dev = (int*)(base + o_order);
   HIP_TRY(L->mols.alloc(ns));
