@@ -218,6 +218,35 @@ def test_mixed_policy_split_launches_above_384(engine):
     b.close()
 
 
+def test_fp32_eigensolver_step_density(engine):
+    """The first SCF step of the mixed policy diagonalises the FP64 guess Fock
+    matrix on the FP32 path (reduction, bisection to 1e-10 ||T||, one solve,
+    back-transform and density in FP32).  Its density is held to the FP64
+    step's: |dP| <= 1e-4, trace 2 n_occ to 1e-4, and P P = 2 P to 1e-4."""
+    mols = [geometry.water_cluster(64, geometry.SplitMix64(4)),   # N = 384
+            geometry.water_cluster(100, geometry.SplitMix64(5)),  # N = 600 (eight wavefronts)
+            geometry.alkane(12), geometry.pyridine()]
+    b = engine.batch(mols)
+    b.scf(threshold=1e-6, max_steps=1, policy="mixed")
+    assert all(n32 == 1 for n32 in b.fp32_steps())
+    p32 = [b.packed(i, api.SG_DENSITY) for i in range(len(mols))]
+    b.scf(threshold=1e-6, max_steps=1, policy="fp64")
+    for i, m in enumerate(mols):
+        n = b.n_ao(i)
+        p64 = b.packed(i, api.SG_DENSITY)
+        P = np.zeros((n, n))
+        r, c = np.tril_indices(n)
+        P[r, c] = p32[i]
+        P[c, r] = p32[i]
+        dp = np.abs(p32[i] - p64).max()
+        idem = np.abs(P @ P - 2.0 * P).max()
+        say("fp32 step N", n, "dP", dp, "idempotency", idem, "trace", np.trace(P) - 2 * b.info[i][2])
+        assert dp <= 1e-4
+        assert abs(np.trace(P) - 2 * b.info[i][2]) <= 1e-4
+        assert idem <= 1e-4
+    b.close()
+
+
 def test_graph_replay_equals_eager(engine):
     """The captured SCF iteration (replayed once per step, with eager setup
     work and host synchronisation between replays) gives the eager results."""
