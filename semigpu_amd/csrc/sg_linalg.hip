@@ -796,26 +796,30 @@ __device__ __forceinline__ double start_entry(int vec, int row) {
 // the 2-norm of the solution.  The right-hand side
 // is not rescaled: pivots below `tiny` are raised to it, so each solve grows
 // the vector by at most ~1/tiny and the sweeps stay far from overflow.
+// START: the right-hand side is vector `vec`'s start vector (start_entry),
+// generated instead of read, which saves writing and reading it once.
+template <bool START>
 __device__ double tri_solve(const double* __restrict__ d, const double* __restrict__ e, int n,
                             double lam, double tiny, double* __restrict__ Y, int ldy, int col,
-                            double* __restrict__ fac, int fs) {
+                            double* __restrict__ fac, int fs, int vec) {
   constexpr int B = kSolveBatch;
+  auto rhs = [&](int i) { return START ? start_entry(vec, i) : Y[(int64_t)i * ldy + col]; };
   double ca = d[0] - lam, cb = n > 1 ? e[0] : 0.0;
-  double y0 = Y[col];
+  double y0 = rhs(0);
   // forward elimination; the right-hand side of the next B rows is loaded
   // while the current B rows are eliminated
   double yb[B];
 #pragma unroll
   for (int r = 0; r < B; ++r) {
     const int i = 1 + r;
-    yb[r] = i < n ? Y[(int64_t)i * ldy + col] : 0.0;
+    yb[r] = i < n ? rhs(i) : 0.0;
   }
   for (int i0 = 0; i0 < n - 1; i0 += B) {
     double yn[B];
 #pragma unroll
     for (int r = 0; r < B; ++r) {
       const int i = i0 + B + 1 + r;
-      yn[r] = i < n ? Y[(int64_t)i * ldy + col] : 0.0;
+      yn[r] = i < n ? rhs(i) : 0.0;
     }
 #pragma unroll
     for (int r = 0; r < B; ++r) {
@@ -996,14 +1000,11 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(3))) k_
   }
   if (tid == 0) n_clusters = make_clusters(lam, k, tnorm, want_all, sh, cstart, clist);
   __syncthreads();
-  for (int64_t t = tid; t < (int64_t)n * k; t += nt) {
-    const int row = (int)(t / k), col = (int)(t % k);
-    Y[t] = start_entry(lo + col, row);
-  }
-  __syncthreads();
   for (int sweep = 0; sweep < sweeps; ++sweep) {
     for (int j = tid; j < k; j += nt) {
-      const double nrm = tri_solve(d, e, n, sh[j], tiny, Y, k, j, fac + (j % L), L);
+      const double nrm =
+          sweep == 0 ? tri_solve<true>(d, e, n, sh[j], tiny, Y, k, j, fac + (j % L), L, lo + j)
+                     : tri_solve<false>(d, e, n, sh[j], tiny, Y, k, j, fac + (j % L), L, lo + j);
       const bool single = cstart[j] == j && (j + 1 == k || cstart[j + 1] != j);
       if (single && sweep == sweeps - 1) scale_column(Y, n, k, j, 1.0 / nrm);
     }
@@ -1057,9 +1058,9 @@ __global__ void __launch_bounds__(64) k_li_solve(const MolDesc* __restrict__ mol
   const double* e = eA + M.vec0;
   const double tiny = kEps * fmax(tnormA[m], 1e-300);
   for (int j = j0; j < k; j += L) {
-    if (sweep == 0)
-      for (int i = 0; i < n; ++i) W.Y[(int64_t)i * k + j] = start_entry(lo + j, i);
-    const double nrm = tri_solve(d, e, n, W.sh[j], tiny, W.Y, k, j, W.fac + j0, L);
+    const double nrm =
+        sweep == 0 ? tri_solve<true>(d, e, n, W.sh[j], tiny, W.Y, k, j, W.fac + j0, L, lo + j)
+                   : tri_solve<false>(d, e, n, W.sh[j], tiny, W.Y, k, j, W.fac + j0, L, lo + j);
     const bool single = W.cstart[j] == j && (j + 1 == k || W.cstart[j + 1] != j);
     if (single && sweep == kInvitSweeps - 1) scale_column(W.Y, n, k, j, 1.0 / nrm);
   }
