@@ -102,7 +102,7 @@ int sg_scf_run(sg_batch* b, double threshold_ev, int max_steps, sg_scf_result* r
  * Fock contractions, the Householder reduction and back-transformation of the
  * one-workgroup eigensolver (N <= 1280) and the density product in FP32 (the
  * products on the matrix cores), with bisection to 1e-10 ||T|| and one
- * inverse-iteration solve, until a molecule's |dE| < switch_ev (1e-3 eV
+ * inverse-iteration solve (factors stored in FP32), until a molecule's |dE| < switch_ev (1e-3 eV
  * suggested), or |dE| < 2^-20 |E_el| (8 FP32 ulps of that energy), or |dE|
  * stops falling by half per step, or 40 FP32 steps; then in FP64 to
  * convergence.  Integrals,

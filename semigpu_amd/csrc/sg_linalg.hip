@@ -797,11 +797,13 @@ __device__ __forceinline__ double start_entry(int vec, int row) {
 // is not rescaled: pivots below `tiny` are raised to it, so each solve grows
 // the vector by at most ~1/tiny and the sweeps stay far from overflow.
 // START: the right-hand side is vector `vec`'s start vector (start_entry),
-// generated instead of read, which saves writing and reading it once.
-template <bool START>
+// generated instead of read, which saves writing and reading it once.  FT:
+// the factors' storage type (float in FP32 steps, where the solve needs FP32
+// accuracy only: half their traffic; the arithmetic stays FP64).
+template <bool START, typename FT>
 __device__ double tri_solve(const double* __restrict__ d, const double* __restrict__ e, int n,
                             double lam, double tiny, double* __restrict__ Y, int ldy, int col,
-                            double* __restrict__ fac, int fs, int vec) {
+                            FT* __restrict__ fac, int fs, int vec) {
   constexpr int B = kSolveBatch;
   auto rhs = [&](int i) { return START ? start_entry(vec, i) : Y[(int64_t)i * ldy + col]; };
   double ca = d[0] - lam, cb = n > 1 ? e[0] : 0.0;
@@ -827,16 +829,16 @@ __device__ double tri_solve(const double* __restrict__ d, const double* __restri
       if (i < n - 1) {
         const double sub = e[i], na = d[i + 1] - lam, nb = i + 2 < n ? e[i + 1] : 0.0;
         const double y1 = yb[r];
-        double* fr = fac + (int64_t)i * 3 * fs;
+        FT* fr = fac + (int64_t)i * 3 * fs;
         // partial pivoting without a branch: pv keeps row i as the pivot row,
         // else rows i and i+1 swap; one reciprocal (v_rcp_f64 + 2 Newton
         // steps) per row
         const bool pv = fabs(ca) >= fabs(sub);
         const double num = pv ? sub : ca, den = pv ? ca : sub;
         const double mlt = den != 0.0 ? num * rcp_nr(den) : 0.0;
-        fr[0] = den;
-        fr[fs] = pv ? cb : na;
-        fr[2 * fs] = pv ? 0.0 : nb;
+        fr[0] = (FT)den;
+        fr[fs] = (FT)(pv ? cb : na);
+        fr[2 * fs] = (FT)(pv ? 0.0 : nb);
         Y[(int64_t)i * ldy + col] = pv ? y0 : y1;
         const double ny0 = pv ? y1 - mlt * y0 : y0 - mlt * y1;
         const double nca = pv ? na - mlt * cb : cb - mlt * na;
@@ -849,10 +851,10 @@ __device__ double tri_solve(const double* __restrict__ d, const double* __restri
     for (int r = 0; r < B; ++r) yb[r] = yn[r];
   }
   {
-    double* fr = fac + (int64_t)(n - 1) * 3 * fs;
-    fr[0] = ca;
-    fr[fs] = 0.0;
-    fr[2 * fs] = 0.0;
+    FT* fr = fac + (int64_t)(n - 1) * 3 * fs;
+    fr[0] = (FT)ca;
+    fr[fs] = (FT)0;
+    fr[2 * fs] = (FT)0;
     Y[(int64_t)(n - 1) * ldy + col] = y0;
   }
   // back substitution, the factors and right-hand side of the next B rows
@@ -863,7 +865,7 @@ __device__ double tri_solve(const double* __restrict__ d, const double* __restri
 #pragma unroll
     for (int r = 0; r < B; ++r) {
       const int i = i0 - r >= 0 ? i0 - r : 0;
-      const double* g = fac + (int64_t)i * 3 * fs;
+      const FT* g = fac + (int64_t)i * 3 * fs;
       a0[r] = g[0];
       a1[r] = g[fs];
       a2[r] = g[2 * fs];
@@ -981,7 +983,8 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(3))) k_
   selection(M, want_all, lo, k);
   if (k == 0) return;
   // FP32 steps: one solve (their T and eigenvalues carry FP32-level error)
-  const int sweeps = prec && prec[m] ? 1 : kInvitSweeps;
+  const bool fp32 = prec && prec[m];
+  const int sweeps = fp32 ? 1 : kInvitSweeps;
   double* d = lds;
   double* e = d + n;
   double* sh = e + n;                     // shifts, k
@@ -1003,8 +1006,12 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(3))) k_
   for (int sweep = 0; sweep < sweeps; ++sweep) {
     for (int j = tid; j < k; j += nt) {
       const double nrm =
-          sweep == 0 ? tri_solve<true>(d, e, n, sh[j], tiny, Y, k, j, fac + (j % L), L, lo + j)
-                     : tri_solve<false>(d, e, n, sh[j], tiny, Y, k, j, fac + (j % L), L, lo + j);
+          fp32
+              ? tri_solve<true, float>(d, e, n, sh[j], tiny, Y, k, j, (float*)fac + (j % L), L,
+                                       lo + j)
+          : sweep == 0
+              ? tri_solve<true, double>(d, e, n, sh[j], tiny, Y, k, j, fac + (j % L), L, lo + j)
+              : tri_solve<false, double>(d, e, n, sh[j], tiny, Y, k, j, fac + (j % L), L, lo + j);
       const bool single = cstart[j] == j && (j + 1 == k || cstart[j + 1] != j);
       if (single && sweep == sweeps - 1) scale_column(Y, n, k, j, 1.0 / nrm);
     }
@@ -1059,8 +1066,9 @@ __global__ void __launch_bounds__(64) k_li_solve(const MolDesc* __restrict__ mol
   const double tiny = kEps * fmax(tnormA[m], 1e-300);
   for (int j = j0; j < k; j += L) {
     const double nrm =
-        sweep == 0 ? tri_solve<true>(d, e, n, W.sh[j], tiny, W.Y, k, j, W.fac + j0, L, lo + j)
-                   : tri_solve<false>(d, e, n, W.sh[j], tiny, W.Y, k, j, W.fac + j0, L, lo + j);
+        sweep == 0
+            ? tri_solve<true, double>(d, e, n, W.sh[j], tiny, W.Y, k, j, W.fac + j0, L, lo + j)
+            : tri_solve<false, double>(d, e, n, W.sh[j], tiny, W.Y, k, j, W.fac + j0, L, lo + j);
     const bool single = W.cstart[j] == j && (j + 1 == k || W.cstart[j + 1] != j);
     if (single && sweep == kInvitSweeps - 1) scale_column(W.Y, n, k, j, 1.0 / nrm);
   }
