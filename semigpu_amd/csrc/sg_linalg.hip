@@ -11,27 +11,33 @@
 //                   lower triangle of the trailing block per column, 8 rows per
 //                   wavefront), then the trailing block gets one rank-2nb update
 //                   on the matrix cores.  Reflector c is stored in the upper part
-//                   of row c of the work matrix.
+//                   of row c of the work matrix.  In the FP32 steps of the
+//                   mixed-precision policy the same reduction runs in FP32
+//                   (k_tridiag<., float>, or per molecule in k_tridiag_both).
 //  k_bisect         eigenvalues of the tridiagonal matrix by bisection on the
-//                   Sturm count, one lane per eigenvalue (only the indices the
+//                   Sturm count, two eigenvalues per lane (only the indices the
 //                   caller selected).
 //  k_invit          eigenvectors of the tridiagonal matrix by inverse iteration
-//                   (2 sweeps, Gaussian elimination with partial pivoting, one
-//                   lane per vector).  Eigenvalues closer than 1e-3*||T|| (full
+//                   (2 sweeps, 1 in FP32 steps; Gaussian elimination with
+//                   partial pivoting, one lane per vector).  Eigenvalues closer
+//                   than 1e-3*||T|| (full
 //                   decomposition) or 1e-5*||T|| (SCF subspace) form a
 //                   cluster; after each sweep a cluster's vectors are
 //                   orthonormalised in order (modified Gram-Schmidt) by one
 //                   wavefront, as LAPACK dstein does.
 //  k_backtransform  Y <- Q Y with the reflectors grouped in compact WY form
-//                   (I - V T V^T), so Y is read twice and written once per panel.
+//                   (I - V T V^T), so Y is read twice and written once per group
+//                   (FP32 reflectors and an FP32 copy of Y in FP32 steps).
 //
 // The SCF only needs the density P = 2 C_occ C_occ^T, i.e. the invariant
 // subspace of the occupied levels.  It selects whichever side is smaller:
 // the n_occ lowest vectors (P = 2 Y Y^T) or the n - n_occ highest
 // (P = 2 (I - Y Y^T)); sg_eigh selects all n for the full eigendecomposition.
 //
-// Workspace per molecule (5 N^2 doubles at 5*mat0): work matrix A, vectors
-// Y (N x k, row-major), inverse-iteration factors (3 N x min(k,512)).
+// Workspace per molecule (5 N^2 doubles at 5*mat0): work matrix A (N^2
+// floats in FP32 steps), vectors Y (N x k, row-major), inverse-iteration
+// factors (3 N x min(k,512), floats in FP32 steps) at 2 N^2, where FP32 steps
+// also put the FP32 copy of Y after the inverse iteration.
 // Golub & Van Loan, Matrix Computations (4th ed.) 5.1.7, 8.3, 8.4.
 #include "sg_common.h"
 #include "sg_kernels.h"
