@@ -115,6 +115,13 @@ int sg_scf_run(sg_batch* b, double threshold_ev, int max_steps, sg_scf_result* r
 #define SG_POLICY_MIXED 1
 int sg_scf_run_policy(sg_batch* b, double threshold_ev, int max_steps, int policy,
                       double switch_ev, sg_scf_result* results);
+/* Density damping for the batch's later sg_scf_run / sg_scf_run_policy calls
+ * (no counterpart in the reference, whose SCF takes each new density as it
+ * is; opt-in for oscillating cases): every step's density becomes
+ * beta P_old + (1 - beta) P_new before the Fock build, beta in [0, 1), 0 (the
+ * default) = off.  The converged energy is the same fixed point; the server
+ * does not damp. */
+int sg_scf_set_damping(sg_batch* b, double beta);
 /* FP32 steps of each molecule in the last sg_scf_run_policy (0 under
  * SG_POLICY_FP64); steps32 gets n_mol entries.  No counterpart in the
  * reference: it splits the steps for per-precision rates. */

@@ -50,6 +50,7 @@ struct sg_batch {
   hipGraphExec_t iter_exec = nullptr;  // captured SCF iteration
   double g_threshold = -1.0, g_switch = -1.0;
   int g_max_steps = -1, g_mixed = -1;
+  double g_damping = -1.0;
 };
 
 // Plain copy, 16 bytes per lane per step: reads and writes exactly n16 * 16 bytes.
@@ -513,6 +514,13 @@ int sg_scf_run(sg_batch* b, double threshold, int max_steps, sg_scf_result* res)
   return sg_scf_run_policy(b, threshold, max_steps, SG_POLICY_FP64, 1e-3, res);
 }
 
+int sg_scf_set_damping(sg_batch* b, double beta) {
+  if (!b) return fail(SG_ERR_INVALID, "null batch");
+  if (!(beta >= 0.0 && beta < 1.0)) return fail(SG_ERR_INVALID, "damping must lie in [0, 1)");
+  b->d.damping = beta;
+  return SG_OK;
+}
+
 int sg_scf_fp32_steps(const sg_batch* b, int* steps32) {
   if (!b || !steps32) return fail(SG_ERR_INVALID, "null batch or output");
   HIP_TRY(hipSetDevice(b->ctx->device));
@@ -569,7 +577,8 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
   b->d.mixed = policy == SG_POLICY_MIXED ? 1 : 0;
   const bool use_graph = !c->timing && b->d.n_large == 0;
   if (use_graph && (!b->iter_exec || b->g_threshold != threshold || b->g_max_steps != max_steps ||
-                    b->g_switch != switch_ev || b->g_mixed != b->d.mixed)) {
+                    b->g_switch != switch_ev || b->g_mixed != b->d.mixed ||
+                    b->g_damping != b->d.damping)) {
     if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
     b->iter_exec = nullptr;
     hipGraph_t g = nullptr;
@@ -590,6 +599,7 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
     b->g_max_steps = max_steps;
     b->g_switch = switch_ev;
     b->g_mixed = b->d.mixed;
+    b->g_damping = b->d.damping;
   }
   for (int it = 0; it < max_steps && n_active > 0; ++it) {
     if (use_graph)

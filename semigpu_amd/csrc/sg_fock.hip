@@ -417,6 +417,32 @@ __global__ void k_clear_hf(const MolDesc* __restrict__ mols, double* __restrict_
 
 static inline int blocks_for(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
+// Density damping: mode 0 keeps P in C, mode 1 mixes P <- beta C + (1 - beta) P
+// (full N x N matrices of the active molecules).
+__global__ void k_damp(const MolDesc* __restrict__ mols, double* __restrict__ P,
+                       double* __restrict__ C, double beta, int mode,
+                       const int* __restrict__ active, const int* __restrict__ status) {
+  const int m = blockIdx.y;
+  if (!mol_live(active, status, m)) return;
+  const MolDesc M = mols[m];
+  const int64_t nn = (int64_t)M.n_ao * M.n_ao;
+  double* p = P + M.mat0;
+  double* c = C + M.mat0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nn;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    if (mode == 0) c[t] = p[t];
+    else p[t] = beta * c[t] + (1.0 - beta) * p[t];
+  }
+}
+
+void launch_damp_keep(const BatchDev& b, hipStream_t s) {
+  if (b.n_mol) k_damp<<<dim3(16, b.n_mol), 256, 0, s>>>(b.mols, b.P, b.C, b.damping, 0, b.active, b.status);
+}
+
+void launch_damp_mix(const BatchDev& b, hipStream_t s) {
+  if (b.n_mol) k_damp<<<dim3(16, b.n_mol), 256, 0, s>>>(b.mols, b.P, b.C, b.damping, 1, b.active, b.status);
+}
+
 __global__ void k_reset_count(int* __restrict__ n) { *n = 0; }
 
 void launch_reset_count(const BatchDev& b, hipStream_t s) { k_reset_count<<<1, 64, 0, s>>>(b.n_active); }

@@ -100,7 +100,9 @@ inline void scf_iteration(sg_context* c, const BatchDev& d, hipStream_t s, doubl
   timed_eigh(c, d, s, false);
   {
     StageTimer t(c, ST_DENSITY, s);
+    if (d.damping > 0.0) launch_damp_keep(d, s);
     launch_density(d, s, false);
+    if (d.damping > 0.0) launch_damp_mix(d, s);
   }
   {
     StageTimer t(c, ST_FOCK, s);

@@ -70,6 +70,9 @@ struct BatchDev {
   // The SCF runs the mixed-precision policy: only then can a step's molecules
   // be in FP32, and the eigensolver launches its FP32 kernels.
   int mixed = 0;
+  // Density damping of the SCF (opt-in, 0 = off): each step's density is
+  // beta P_old + (1 - beta) P_new, with P_old kept in C meanwhile.
+  double damping = 0.0;
   int max_n = 0;             // largest N in the batch
   // Molecule indices by decreasing N (or null for 0..n_mol-1): the per-molecule
   // eigensolver and density launches take their workgroups in this order, so
@@ -167,6 +170,10 @@ void launch_scf_update(const BatchDev& b, hipStream_t s, double threshold, int m
 void launch_blocks_to_fp32(const BatchDev& b, hipStream_t s);
 void launch_set_prec(const BatchDev& b, hipStream_t s, int v, const int* mask = nullptr);
 void launch_clear_hf(const BatchDev& b, hipStream_t s, const int* mask);
+// Density damping around launch_density (active molecules): keep P in C, then
+// P <- damping C + (1 - damping) P.
+void launch_damp_keep(const BatchDev& b, hipStream_t s);
+void launch_damp_mix(const BatchDev& b, hipStream_t s);
 // Eigensolver of F (all = every eigenpair, else the selected subspace of the
 // active molecules): tridiagonalisation, bisection, inverse iteration,
 // back-transformation; launch_eigh runs the four in order.  The reduction's

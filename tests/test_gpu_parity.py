@@ -247,6 +247,35 @@ def test_fp32_eigensolver_step_density(engine):
     b.close()
 
 
+def test_damped_scf(engine):
+    """Opt-in density damping reaches the undamped SCF's energy; batch equals
+    alone and graph replay equals eager under damping; damping 0 is the
+    reference's SCF again, bit for bit."""
+    mols = [geometry.pyridine(), geometry.water_cluster(20, geometry.SplitMix64(20)),
+            geometry.alkane(10)]
+    ref = engine.batch(mols).scf(threshold=1e-6)
+    b = engine.batch(mols)
+    b.set_damping(0.3)
+    damp = b.scf(threshold=1e-6)
+    for r, d in zip(ref, damp):
+        say("damped", d["steps"], "undamped", r["steps"], d["e_tot"] - r["e_tot"])
+        assert d["converged"] and abs(d["e_tot"] - r["e_tot"]) <= 1e-5
+    alone = engine.batch([mols[0]])
+    alone.set_damping(0.3)
+    assert alone.scf(threshold=1e-6)[0] == damp[0]
+    alone.close()
+    engine.timing(True)
+    eager = b.scf(threshold=1e-6)
+    engine.timing(False)
+    engine.timing_reset()
+    assert eager == damp
+    with pytest.raises(api.SemiGpuError, match="damping"):
+        b.set_damping(1.0)
+    b.set_damping(0.0)
+    assert b.scf(threshold=1e-6) == ref
+    b.close()
+
+
 def test_graph_replay_equals_eager(engine):
     """The captured SCF iteration (replayed once per step, with eager setup
     work and host synchronisation between replays) gives the eager results."""
