@@ -122,6 +122,17 @@ int sg_scf_run_policy(sg_batch* b, double threshold_ev, int max_steps, int polic
  * default) = off.  The converged energy is the same fixed point; the server
  * does not damp. */
 int sg_scf_set_damping(sg_batch* b, double beta);
+/* Pulay DIIS for the batch's later sg_scf_run / sg_scf_run_policy calls (no
+ * counterpart in the reference, whose SCF diagonalises each new Fock matrix
+ * as it is; opt-in for slow or oscillating cases): after each step's
+ * convergence test, a molecule that goes on iterating stores F and its error
+ * F P - P F (the last `depth` of each), and the next eigensolve takes
+ * sum_j c_j F_j, c minimising |sum_j c_j e_j| with sum_j c_j = 1.  depth 0
+ * (the default) = off, else 2..8; the histories take 2 depth sum N^2 doubles,
+ * allocated here.  Under SG_POLICY_MIXED only FP64 steps are stored (the
+ * history starts at the first FP64 step).  A converged molecule keeps the
+ * Fock matrix of its final density; the server does not extrapolate. */
+int sg_scf_set_diis(sg_batch* b, int depth);
 /* FP32 steps of each molecule in the last sg_scf_run_policy (0 under
  * SG_POLICY_FP64); steps32 gets n_mol entries.  No counterpart in the
  * reference: it splits the steps for per-precision rates. */

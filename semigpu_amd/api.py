@@ -32,7 +32,7 @@ EXPORTS = (
     "sg_last_error", "sg_version", "sg_ctx_create", "sg_ctx_destroy", "sg_batch_create",
     "sg_batch_destroy", "sg_batch_molecule_info", "sg_two_center", "sg_hcore", "sg_core_core",
     "sg_guess_density", "sg_fock", "sg_eigh", "sg_density", "sg_energy", "sg_scf_run",
-    "sg_scf_run_policy", "sg_scf_fp32_steps", "sg_scf_set_damping",
+    "sg_scf_run_policy", "sg_scf_fp32_steps", "sg_scf_set_damping", "sg_scf_set_diis",
     "sg_pair_block_count", "sg_get_pair_blocks", "sg_get_packed", "sg_set_packed",
     "sg_get_eigen", "sg_set_eigen", "sg_batch_create_bohr", "sg_set_pair_blocks",
     "sg_get_status", "sg_calibration_copy", "sg_timing_enable", "sg_timing_read", "sg_timing_reset",
@@ -101,6 +101,7 @@ def load_library(path: str = LIB_PATH):
                                     ctypes.c_double, ctypes.POINTER(ScfResult)]
     L.sg_scf_fp32_steps.argtypes = [_vp, _ip]
     L.sg_scf_set_damping.argtypes = [_vp, ctypes.c_double]
+    L.sg_scf_set_diis.argtypes = [_vp, ctypes.c_int]
     L.sg_pair_block_count.argtypes = [_vp, ctypes.c_int]
     L.sg_pair_block_count.restype = ctypes.c_long
     L.sg_get_pair_blocks.argtypes = [_vp, ctypes.c_int, _dp]
@@ -298,6 +299,11 @@ class Batch:
         """Density damping of later scf() calls: P <- beta P_old + (1 - beta) P_new
         each step (0 = off, the reference's SCF)."""
         _check(self.L.sg_scf_set_damping(self.h, float(beta)))
+
+    def set_diis(self, depth: int):
+        """Pulay DIIS of later scf() calls over the last `depth` Fock matrices
+        (0 = off, the reference's SCF; else 2..8)."""
+        _check(self.L.sg_scf_set_diis(self.h, int(depth)))
 
     def fp32_steps(self) -> np.ndarray:
         """FP32 steps of each molecule in the last scf() (0 for policy fp64)."""

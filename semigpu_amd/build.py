@@ -36,6 +36,7 @@ UNITS = {
     "sg_fock.hip": [],
     "sg_linalg.hip": [],
     "sg_large.hip": [],
+    "sg_diis.hip": [],
     "sg_api.cpp": ["-x", "hip"],
     "sg_server.cpp": ["-x", "hip"],
 }

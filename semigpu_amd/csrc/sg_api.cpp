@@ -51,6 +51,7 @@ struct sg_batch {
   double g_threshold = -1.0, g_switch = -1.0;
   int g_max_steps = -1, g_mixed = -1;
   double g_damping = -1.0;
+  void* diis_mem = nullptr;  // DIIS histories (sg_scf_set_diis)
 };
 
 // Plain copy, 16 bytes per lane per step: reads and writes exactly n16 * 16 bytes.
@@ -394,6 +395,7 @@ int sg_batch_destroy(sg_batch* b) {
   (void)hipStreamSynchronize(b->ctx->stream);
   if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
   if (b->dev_mem) (void)hipFree(b->dev_mem);
+  if (b->diis_mem) (void)hipFree(b->diis_mem);
   delete b;
   return SG_OK;
 }
@@ -521,6 +523,46 @@ int sg_scf_set_damping(sg_batch* b, double beta) {
   return SG_OK;
 }
 
+int sg_scf_set_diis(sg_batch* b, int depth) {
+  if (!b) return fail(SG_ERR_INVALID, "null batch");
+  if (depth != 0 && (depth < 2 || depth > kDiisMax))
+    return fail(SG_ERR_INVALID, "DIIS depth must be 0 or lie in [2, " + std::to_string(kDiisMax) + "]");
+  if (depth == b->d.diis_m) return SG_OK;
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  // the captured iteration holds the old depth and buffers
+  if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
+  b->iter_exec = nullptr;
+  if (b->diis_mem) (void)hipFree(b->diis_mem);
+  b->diis_mem = nullptr;
+  BatchDev& d = b->d;
+  d.diis_m = 0;
+  d.diis_F = d.diis_E = d.diis_B = d.diis_part = d.diis_c = nullptr;
+  d.diis_k = nullptr;
+  if (depth == 0) return SG_OK;
+  const size_t n_mol = (size_t)d.n_mol;
+  const size_t hist = sizeof(double) * (size_t)depth * (size_t)d.n_mat;
+  const size_t o_F = 0, o_E = o_F + hist, o_B = o_E + hist,
+               o_part = o_B + sizeof(double) * n_mol * kDiisMax * kDiisMax,
+               o_c = o_part + sizeof(double) * (size_t)diis_part_doubles(d.n_mol),
+               o_k = o_c + sizeof(double) * n_mol * kDiisMax, end = o_k + sizeof(int) * n_mol;
+  hipError_t e = hipMalloc(&b->diis_mem, end ? end : 256);
+  if (e != hipSuccess) {
+    b->diis_mem = nullptr;
+    return fail(SG_ERR_HIP, std::string("hipMalloc(") + std::to_string(end) + ") for DIIS: " +
+                                hipGetErrorString(e));
+  }
+  char* base = (char*)b->diis_mem;
+  d.diis_F = (double*)(base + o_F);
+  d.diis_E = (double*)(base + o_E);
+  d.diis_B = (double*)(base + o_B);
+  d.diis_part = (double*)(base + o_part);
+  d.diis_c = (double*)(base + o_c);
+  d.diis_k = (int*)(base + o_k);
+  d.diis_m = depth;
+  return SG_OK;
+}
+
 int sg_scf_fp32_steps(const sg_batch* b, int* steps32) {
   if (!b || !steps32) return fail(SG_ERR_INVALID, "null batch or output");
   HIP_TRY(hipSetDevice(b->ctx->device));
@@ -565,6 +607,7 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
     StageTimer t(c, ST_CONTROL);
     launch_set_prec(b->d, s, policy == SG_POLICY_MIXED ? 1 : 0);
     launch_scf_init(b->d, s, max_steps, policy);
+    launch_diis_reset(b->d, s);
   }
   if ((r = check_launch("scf init"))) return r;
   int n_active = max_steps > 0 ? b->d.n_mol : 0;

@@ -117,6 +117,10 @@ inline void scf_iteration(sg_context* c, const BatchDev& d, hipStream_t s, doubl
     launch_reset_count(d, s);
     launch_scf_update(d, s, threshold, max_steps, switch_ev, fp32_cap);
   }
+  if (d.diis_m) {  // the next eigensolve takes the extrapolated Fock matrix
+    StageTimer t(c, ST_FOCK, s);
+    launch_diis(d, s);
+  }
 }
 
 }  // namespace sg

@@ -17,6 +17,7 @@ constexpr int kMaxLarge = 8192;
 // tables share the last N^2 doubles of the molecule's workspace).
 constexpr int kMinLarge = 128;
 constexpr int kLargeNb = 32;  // reflectors per panel of the multi-workgroup solver
+constexpr int kDiisMax = 8;   // largest DIIS depth
 
 // Device view of a batch of molecules.  All matrices are dense N x N per
 // molecule, row-major (symmetric ones are stored in full); eigenvectors are
@@ -73,6 +74,18 @@ struct BatchDev {
   // Density damping of the SCF (opt-in, 0 = off): each step's density is
   // beta P_old + (1 - beta) P_new, with P_old kept in C meanwhile.
   double damping = 0.0;
+  // Pulay DIIS of the SCF (opt-in, depth 0 = off): the last diis_m Fock and
+  // error matrices of each molecule (slot j at diis_F / diis_E + j n_mat +
+  // mat0), the B matrices (kDiisMax^2 per molecule), the dot products'
+  // partial sums, the coefficients (kDiisMax per molecule) and the number of
+  // vectors stored in the current SCF.
+  int diis_m = 0;
+  double* diis_F = nullptr;
+  double* diis_E = nullptr;
+  double* diis_B = nullptr;
+  double* diis_part = nullptr;
+  double* diis_c = nullptr;
+  int* diis_k = nullptr;
   int max_n = 0;             // largest N in the batch
   // Molecule indices by decreasing N (or null for 0..n_mol-1): the per-molecule
   // eigensolver and density launches take their workgroups in this order, so
@@ -174,6 +187,11 @@ void launch_clear_hf(const BatchDev& b, hipStream_t s, const int* mask);
 // P <- damping C + (1 - damping) P.
 void launch_damp_keep(const BatchDev& b, hipStream_t s);
 void launch_damp_mix(const BatchDev& b, hipStream_t s);
+// Pulay DIIS after the convergence test (molecules still iterating, FP64
+// steps only): store F and F P - P F, F <- the extrapolated Fock matrix.
+void launch_diis(const BatchDev& b, hipStream_t s);
+void launch_diis_reset(const BatchDev& b, hipStream_t s);  // at the start of an SCF
+int64_t diis_part_doubles(int n_mol);
 // Eigensolver of F (all = every eigenpair, else the selected subspace of the
 // active molecules): tridiagonalisation, bisection, inverse iteration,
 // back-transformation; launch_eigh runs the four in order.  The reduction's

@@ -276,6 +276,49 @@ def test_damped_scf(engine):
     b.close()
 
 
+def test_diis_scf(engine):
+    """Opt-in Pulay DIIS reaches the plain SCF's energy in no more steps;
+    batch equals alone and graph replay equals eager under DIIS; a converged
+    molecule's Fock matrix is the one of its final density; depth 0 is the
+    reference's SCF again, bit for bit."""
+    mols = [geometry.pyridine(), geometry.water_cluster(20, geometry.SplitMix64(20)),
+            geometry.alkane(10), geometry.water_cluster(60, geometry.SplitMix64(61))]
+    ref = engine.batch(mols).scf(threshold=1e-8)
+    b = engine.batch(mols)
+    b.set_diis(6)
+    dis = b.scf(threshold=1e-8)
+    for r, d in zip(ref, dis):
+        say("diis", d["steps"], "plain", r["steps"], d["e_tot"] - r["e_tot"])
+        assert d["converged"] and abs(d["e_tot"] - r["e_tot"]) <= 1e-6
+        assert d["steps"] <= r["steps"]
+    assert sum(d["steps"] for d in dis) < sum(r["steps"] for r in ref)
+    f_scf = b.packed(3, api.SG_FOCK)
+    b.fock()
+    np.testing.assert_allclose(b.packed(3, api.SG_FOCK), f_scf, rtol=0, atol=1e-12)
+    alone = engine.batch([mols[1]])
+    alone.set_diis(6)
+    assert alone.scf(threshold=1e-8)[0] == dis[1]
+    alone.close()
+    engine.timing(True)
+    eager = b.scf(threshold=1e-8)
+    engine.timing(False)
+    engine.timing_reset()
+    assert eager == dis
+    mixed = b.scf(threshold=1e-8, policy="mixed")
+    for r, d in zip(ref, mixed):
+        say("diis mixed", d["steps"], d["e_tot"] - r["e_tot"])
+        assert d["converged"] and abs(d["e_tot"] - r["e_tot"]) <= 1e-6
+    for bad in (1, 9, -1):
+        with pytest.raises(api.SemiGpuError, match="DIIS"):
+            b.set_diis(bad)
+    b.set_diis(3)
+    for r, d in zip(ref, b.scf(threshold=1e-8)):
+        assert d["converged"] and abs(d["e_tot"] - r["e_tot"]) <= 1e-6
+    b.set_diis(0)
+    assert b.scf(threshold=1e-8) == ref
+    b.close()
+
+
 def test_graph_replay_equals_eager(engine):
     """The captured SCF iteration (replayed once per step, with eager setup
     work and host synchronisation between replays) gives the eager results."""
