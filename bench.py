@@ -234,6 +234,9 @@ def main():
     ap.add_argument("--water-box", action="store_true",
                     help="instead: the 1000-water box (N = 6000), FP64 and mixed SCF, one JSON line")
     ap.add_argument("--box-max-steps", type=int, default=60)
+    ap.add_argument("--diis", type=int, default=0,
+                    help="also time the batch with Pulay DIIS of this depth (opt-in SCF aid, "
+                         "not in the reference; reported beside value, never as it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -354,6 +357,24 @@ def main():
     fp64_s = time.perf_counter() - t
     dE = np.array([abs(r["e_tot"] - q["e_tot"]) for r, q in zip(res, res64)])
     ok = int(np.sum((dE <= 1e-4) & conv))
+    diis = None
+    if args.diis:
+        b.set_diis(args.diis)
+        run_step(b, args.threshold, args.policy)
+        dt = []
+        for k in range(args.steps):
+            t = time.perf_counter()
+            rd = run_step(b, args.threshold, args.policy)
+            dt.append(time.perf_counter() - t)
+            print("diis step %d: %.3f s" % (k, dt[-1]), file=sys.stderr, flush=True)
+        d_s = float(np.mean(dt))
+        dEd = np.array([abs(r["e_tot"] - q["e_tot"]) for r, q in zip(rd, res64)])
+        okd = int(np.sum((dEd <= 1e-4) & np.array([r["converged"] for r in rd])))
+        diis = {"depth": args.diis, "value": round(okd / d_s, 3), "unit": "molecules/s",
+                "ms_per_step": round(d_s * 1e3, 3), "speedup_vs_value": round(step_s / d_s, 3),
+                "scf_steps_mean": round(float(np.mean([r["steps"] for r in rd])), 3),
+                "scf_steps_max": int(max(r["steps"] for r in rd)),
+                "max_abs_dE_vs_fp64_eV": float(dEd.max()), "policy": args.policy}
     b.close()
 
     # plain FP64 path: one molecule at a time through the same kernels
@@ -408,6 +429,8 @@ def main():
         "scf_steps_mean": round(float(steps.mean()), 3),
         "batch_latency": batch_latency,
     }
+    if diis is not None:
+        out["diis"] = diis
     if plain_value is not None:
         out["plain_fp64"] = {"value": round(plain_value, 3), "unit": "molecules/s",
                              "sample": len(idx), "max_abs_dE_vs_batched_eV": max_dE_plain,

@@ -6,7 +6,8 @@
 // molecules that go on iterating:
 //   1. k_diis_err   error matrix e = F P - P F (the commutator; the ZDO basis
 //                   has S = 1), written with F into ring slot k mod m;
-//                   16 x 16 output tiles of the lower triangle, e_ji = -e_ij;
+//                   64 x 64 blocks of the lower triangle on the FP64 matrix
+//                   cores, e_ji = -e_ij;
 //   2. k_diis_dot   <e_k, e_j> for the stored slots, 16 chunks per molecule,
 //                   one partial sum per chunk and slot (no atomics: the sums
 //                   run in a fixed order, so a molecule's result does not
@@ -28,6 +29,15 @@ constexpr int kChunks = 16;
 
 __device__ __forceinline__ int diis_slot(const int* k, int m, int depth) { return k[m] % depth; }
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int kErrTile = 64;  // output block of one workgroup (lower triangle of 64 x 64 blocks)
+constexpr int kErrK = 16;     // k columns staged through LDS per pass
+
+// Four wavefronts per workgroup, each a 32 x 32 quarter of the block as 2 x 2
+// v_mfma_f64_16x16x4f64 tiles (operand lane (r, q) = row r & 15, k column q;
+// result lane (c, q) holds rows 4 i + q, column c):
+//   e_ij = sum_k F_ik P_jk - P_ik F_jk   (P_kj = P_jk, F_kj = F_jk)
 __global__ void __launch_bounds__(256) k_diis_err(const MolDesc* __restrict__ mols,
                                                   const double* __restrict__ F,
                                                   const double* __restrict__ P,
@@ -40,48 +50,81 @@ __global__ void __launch_bounds__(256) k_diis_err(const MolDesc* __restrict__ mo
   const int m = blockIdx.y;
   if (!mol_live(active, status, m) || fp32_step[m]) return;
   const MolDesc M = mols[m];
-  const int n = M.n_ao, nt = (n + 15) / 16;
-  // tile t of the lower triangle: row tile ti, column tile tj <= ti
+  const int n = M.n_ao, nt = (n + kErrTile - 1) / kErrTile;
   const int t = blockIdx.x;
   if (t >= nt * (nt + 1) / 2) return;
-  int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-  while (ti * (ti + 1) / 2 > t) --ti;
-  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-  const int tj = t - ti * (ti + 1) / 2;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int i = ti * 16 + ty, j = tj * 16 + tx;
+  int bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (bi * (bi + 1) / 2 > t) --bi;
+  while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+  const int bj = t - bi * (bi + 1) / 2;
+  const int I0 = bi * kErrTile, J0 = bj * kErrTile;
   const double* f = F + M.mat0;
   const double* p = P + M.mat0;
-  __shared__ double Fi[16][17], Pi[16][17], Fj[16][17], Pj[16][17];
-  double acc = 0.0;
-  for (int k0 = 0; k0 < n; k0 += 16) {
-    const int k = k0 + tx;
-    const int ri = ti * 16 + ty, rj = tj * 16 + ty;
-    const bool ki = ri < n && k < n, kj = rj < n && k < n;
-    Fi[ty][tx] = ki ? f[(int64_t)ri * n + k] : 0.0;
-    Pi[ty][tx] = ki ? p[(int64_t)ri * n + k] : 0.0;
-    Fj[ty][tx] = kj ? f[(int64_t)rj * n + k] : 0.0;
-    Pj[ty][tx] = kj ? p[(int64_t)rj * n + k] : 0.0;
-    __syncthreads();
-    // e_ij = sum_k F_ik P_kj - P_ik F_kj, with P_kj = P_jk and F_kj = F_jk
+  __shared__ double sFi[kErrTile][kErrK + 1], sPi[kErrTile][kErrK + 1];
+  __shared__ double sFj[kErrTile][kErrK + 1], sPj[kErrTile][kErrK + 1];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int wr = wave >> 1, wc = wave & 1;
+  v4d acc[2][2];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      acc = fma(Fi[ty][q], Pj[tx][q], acc);
-      acc = fma(-Pi[ty][q], Fj[tx][q], acc);
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[a][c] = v4d{0.0, 0.0, 0.0, 0.0};
+  const int lrow = tid >> 4, lcol = tid & 15;  // staging: 16 rows of 16 columns per pass
+  for (int k0 = 0; k0 < n; k0 += kErrK) {
+    const int k = k0 + lcol;
+#pragma unroll
+    for (int q = 0; q < kErrTile / 16; ++q) {
+      const int rr = lrow + 16 * q, ri = I0 + rr, rj = J0 + rr;
+      const bool ki = ri < n && k < n, kj = rj < n && k < n;
+      sFi[rr][lcol] = ki ? f[(int64_t)ri * n + k] : 0.0;
+      sPi[rr][lcol] = ki ? p[(int64_t)ri * n + k] : 0.0;
+      sFj[rr][lcol] = kj ? f[(int64_t)rj * n + k] : 0.0;
+      sPj[rr][lcol] = kj ? p[(int64_t)rj * n + k] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int st = 0; st < kErrK / 4; ++st) {
+      const int kk = 4 * st + lk;
+      double aF[2], aP[2], bF[2], bP[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const int r = 32 * wr + 16 * a + lr, c = 32 * wc + 16 * a + lr;
+        aF[a] = sFi[r][kk];
+        aP[a] = -sPi[r][kk];
+        bP[a] = sPj[c][kk];
+        bF[a] = sFj[c][kk];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(aF[a], bP[c], acc[a][c], 0, 0, 0);
+          acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(aP[a], bF[c], acc[a][c], 0, 0, 0);
+        }
     }
     __syncthreads();
   }
-  if (i >= n || j >= n || j > i) return;
   const int s = diis_slot(kcount, m, depth);
   double* e = histE + (int64_t)s * n_mat + M.mat0;
   double* h = histF + (int64_t)s * n_mat + M.mat0;
-  const int64_t ij = (int64_t)i * n + j, ji = (int64_t)j * n + i;
-  // one writer per entry: the thread of the lower-triangle position writes
-  // both halves (the diagonal of a commutator of symmetric matrices is 0)
-  e[ij] = i == j ? 0.0 : acc;
-  e[ji] = i == j ? 0.0 : -acc;
-  h[ij] = f[ij];
-  h[ji] = f[ji];
+  // one writer per entry: the lane of the lower-triangle position writes both
+  // halves (the diagonal of a commutator of symmetric matrices is 0)
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int i = I0 + 32 * wr + 16 * a + 4 * rr + lk, j = J0 + 32 * wc + 16 * c + lr;
+        if (i >= n || j >= n || j > i) continue;
+        const int64_t ij = (int64_t)i * n + j, ji = (int64_t)j * n + i;
+        const double v = acc[a][c][rr];
+        e[ij] = i == j ? 0.0 : v;
+        e[ji] = i == j ? 0.0 : -v;
+        h[ij] = f[ij];
+        h[ji] = f[ji];
+      }
 }
 
 // part[(m * kChunks + chunk) * kDiisMax + j] = chunk's share of <e_s, e_j>.
@@ -239,7 +282,7 @@ void launch_diis_reset(const BatchDev& b, hipStream_t s) {
 
 void launch_diis(const BatchDev& b, hipStream_t s) {
   if (!b.n_mol || !b.diis_m) return;
-  const int nt = (b.max_n + 15) / 16;
+  const int nt = (b.max_n + kErrTile - 1) / kErrTile;
   k_diis_err<<<dim3(nt * (nt + 1) / 2, b.n_mol), 256, 0, s>>>(
       b.mols, b.F, b.P, b.diis_F, b.diis_E, b.n_mat, b.diis_k, b.diis_m, b.active, b.status, b.prev_prec);
   k_diis_dot<<<dim3(kChunks, b.n_mol), 256, 0, s>>>(b.mols, b.diis_E, b.n_mat, b.diis_k, b.diis_m,
