@@ -129,9 +129,10 @@ int sg_scf_set_damping(sg_batch* b, double beta);
  * F P - P F (the last `depth` of each), and the next eigensolve takes
  * sum_j c_j F_j, c minimising |sum_j c_j e_j| with sum_j c_j = 1.  depth 0
  * (the default) = off, else 2..8; the histories take 2 depth sum N^2 doubles,
- * allocated here.  Under SG_POLICY_MIXED only FP64 steps are stored (the
- * history starts at the first FP64 step).  A converged molecule keeps the
- * Fock matrix of its final density; the server does not extrapolate. */
+ * allocated here.  Under SG_POLICY_MIXED the FP32 steps extrapolate over
+ * FP32 steps only, and the history restarts at the first FP64 step.  A
+ * converged molecule keeps the Fock matrix of its final density; the server
+ * does not extrapolate. */
 int sg_scf_set_diis(sg_batch* b, int depth);
 /* FP32 steps of each molecule in the last sg_scf_run_policy (0 under
  * SG_POLICY_FP64); steps32 gets n_mol entries.  No counterpart in the

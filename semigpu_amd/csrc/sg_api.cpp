@@ -538,14 +538,15 @@ int sg_scf_set_diis(sg_batch* b, int depth) {
   BatchDev& d = b->d;
   d.diis_m = 0;
   d.diis_F = d.diis_E = d.diis_B = d.diis_part = d.diis_c = nullptr;
-  d.diis_k = nullptr;
+  d.diis_k = d.diis_p = nullptr;
   if (depth == 0) return SG_OK;
   const size_t n_mol = (size_t)d.n_mol;
   const size_t hist = sizeof(double) * (size_t)depth * (size_t)d.n_mat;
   const size_t o_F = 0, o_E = o_F + hist, o_B = o_E + hist,
                o_part = o_B + sizeof(double) * n_mol * kDiisMax * kDiisMax,
                o_c = o_part + sizeof(double) * (size_t)diis_part_doubles(d.n_mol),
-               o_k = o_c + sizeof(double) * n_mol * kDiisMax, end = o_k + sizeof(int) * n_mol;
+               o_k = o_c + sizeof(double) * n_mol * kDiisMax, o_p = o_k + sizeof(int) * n_mol,
+               end = o_p + sizeof(int) * n_mol;
   hipError_t e = hipMalloc(&b->diis_mem, end ? end : 256);
   if (e != hipSuccess) {
     b->diis_mem = nullptr;
@@ -559,6 +560,7 @@ int sg_scf_set_diis(sg_batch* b, int depth) {
   d.diis_part = (double*)(base + o_part);
   d.diis_c = (double*)(base + o_c);
   d.diis_k = (int*)(base + o_k);
+  d.diis_p = (int*)(base + o_p);
   d.diis_m = depth;
   return SG_OK;
 }

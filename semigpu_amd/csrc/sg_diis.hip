@@ -45,10 +45,9 @@ __global__ void __launch_bounds__(256) k_diis_err(const MolDesc* __restrict__ mo
                                                   double* __restrict__ histE, int64_t n_mat,
                                                   const int* __restrict__ kcount, int depth,
                                                   const int* __restrict__ active,
-                                                  const int* __restrict__ status,
-                                                  const int* __restrict__ fp32_step) {
+                                                  const int* __restrict__ status) {
   const int m = blockIdx.y;
-  if (!mol_live(active, status, m) || fp32_step[m]) return;
+  if (!mol_live(active, status, m)) return;
   const MolDesc M = mols[m];
   const int n = M.n_ao, nt = (n + kErrTile - 1) / kErrTile;
   const int t = blockIdx.x;
@@ -133,10 +132,9 @@ __global__ void __launch_bounds__(256) k_diis_dot(const MolDesc* __restrict__ mo
                                                   const int* __restrict__ kcount, int depth,
                                                   double* __restrict__ part,
                                                   const int* __restrict__ active,
-                                                  const int* __restrict__ status,
-                                                  const int* __restrict__ fp32_step) {
+                                                  const int* __restrict__ status) {
   const int m = blockIdx.y;
-  if (!mol_live(active, status, m) || fp32_step[m]) return;
+  if (!mol_live(active, status, m)) return;
   const MolDesc M = mols[m];
   const int64_t nn = (int64_t)M.n_ao * M.n_ao;
   const int s = diis_slot(kcount, m, depth);
@@ -170,13 +168,9 @@ __global__ void __launch_bounds__(256) k_diis_dot(const MolDesc* __restrict__ mo
 __global__ void k_diis_solve(int n_mol, int* __restrict__ kcount, int depth,
                              const double* __restrict__ part, double* __restrict__ B,
                              double* __restrict__ coef, const int* __restrict__ active,
-                             const int* __restrict__ status, const int* __restrict__ fp32_step) {
+                             const int* __restrict__ status) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n_mol || !mol_live(active, status, m)) return;
-  if (fp32_step[m]) {  // history restarts at the first FP64 step
-    kcount[m] = 0;
-    return;
-  }
   const int kc = kcount[m];
   const int s = kc % depth, nv = min(kc + 1, depth);
   double* b = B + (int64_t)m * kDiisMax * kDiisMax;
@@ -249,10 +243,9 @@ __global__ void __launch_bounds__(256) k_diis_mix(const MolDesc* __restrict__ mo
                                                   const double* __restrict__ histF, int64_t n_mat,
                                                   const double* __restrict__ coef,
                                                   const int* __restrict__ active,
-                                                  const int* __restrict__ status,
-                                                  const int* __restrict__ fp32_step) {
+                                                  const int* __restrict__ status) {
   const int m = blockIdx.y;
-  if (!mol_live(active, status, m) || fp32_step[m]) return;
+  if (!mol_live(active, status, m)) return;
   const MolDesc M = mols[m];
   const int64_t nn = (int64_t)M.n_ao * M.n_ao;
   double c[kDiisMax];
@@ -269,29 +262,47 @@ __global__ void __launch_bounds__(256) k_diis_mix(const MolDesc* __restrict__ mo
   }
 }
 
-__global__ void k_diis_reset(int n_mol, int* __restrict__ kcount) {
+__global__ void k_diis_reset(int n_mol, int* __restrict__ kcount, int* __restrict__ hist_prec) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m < n_mol) kcount[m] = 0;
+  if (m < n_mol) {
+    kcount[m] = 0;
+    hist_prec[m] = -1;
+  }
+}
+
+// The history holds Fock matrices of one precision: it restarts when the
+// precision of the step just done differs from the stored ones'.
+__global__ void k_diis_begin(int n_mol, int* __restrict__ kcount, int* __restrict__ hist_prec,
+                             const int* __restrict__ step_prec, const int* __restrict__ active,
+                             const int* __restrict__ status) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_mol || !mol_live(active, status, m)) return;
+  if (hist_prec[m] != step_prec[m]) {
+    kcount[m] = 0;
+    hist_prec[m] = step_prec[m];
+  }
 }
 
 }  // namespace
 
 void launch_diis_reset(const BatchDev& b, hipStream_t s) {
-  if (b.n_mol && b.diis_m) k_diis_reset<<<(b.n_mol + 255) / 256, 256, 0, s>>>(b.n_mol, b.diis_k);
+  if (b.n_mol && b.diis_m)
+    k_diis_reset<<<(b.n_mol + 255) / 256, 256, 0, s>>>(b.n_mol, b.diis_k, b.diis_p);
 }
 
 void launch_diis(const BatchDev& b, hipStream_t s) {
   if (!b.n_mol || !b.diis_m) return;
   const int nt = (b.max_n + kErrTile - 1) / kErrTile;
+  k_diis_begin<<<(b.n_mol + 255) / 256, 256, 0, s>>>(b.n_mol, b.diis_k, b.diis_p, b.prev_prec,
+                                                     b.active, b.status);
   k_diis_err<<<dim3(nt * (nt + 1) / 2, b.n_mol), 256, 0, s>>>(
-      b.mols, b.F, b.P, b.diis_F, b.diis_E, b.n_mat, b.diis_k, b.diis_m, b.active, b.status, b.prev_prec);
+      b.mols, b.F, b.P, b.diis_F, b.diis_E, b.n_mat, b.diis_k, b.diis_m, b.active, b.status);
   k_diis_dot<<<dim3(kChunks, b.n_mol), 256, 0, s>>>(b.mols, b.diis_E, b.n_mat, b.diis_k, b.diis_m,
-                                                    b.diis_part, b.active, b.status, b.prev_prec);
+                                                    b.diis_part, b.active, b.status);
   k_diis_solve<<<(b.n_mol + 63) / 64, 64, 0, s>>>(b.n_mol, b.diis_k, b.diis_m, b.diis_part,
-                                                  b.diis_B, b.diis_c, b.active, b.status,
-                                                  b.prev_prec);
+                                                  b.diis_B, b.diis_c, b.active, b.status);
   k_diis_mix<<<dim3(kChunks, b.n_mol), 256, 0, s>>>(b.mols, b.F, b.diis_F, b.n_mat, b.diis_c,
-                                                    b.active, b.status, b.prev_prec);
+                                                    b.active, b.status);
 }
 
 int64_t diis_part_doubles(int n_mol) { return (int64_t)n_mol * kChunks * kDiisMax; }

@@ -77,8 +77,8 @@ struct BatchDev {
   // Pulay DIIS of the SCF (opt-in, depth 0 = off): the last diis_m Fock and
   // error matrices of each molecule (slot j at diis_F / diis_E + j n_mat +
   // mat0), the B matrices (kDiisMax^2 per molecule), the dot products'
-  // partial sums, the coefficients (kDiisMax per molecule) and the number of
-  // vectors stored in the current SCF.
+  // partial sums, the coefficients (kDiisMax per molecule), the number of
+  // vectors stored in the current SCF and their precision (1 = FP32 steps).
   int diis_m = 0;
   double* diis_F = nullptr;
   double* diis_E = nullptr;
@@ -86,6 +86,7 @@ struct BatchDev {
   double* diis_part = nullptr;
   double* diis_c = nullptr;
   int* diis_k = nullptr;
+  int* diis_p = nullptr;
   int max_n = 0;             // largest N in the batch
   // Molecule indices by decreasing N (or null for 0..n_mol-1): the per-molecule
   // eigensolver and density launches take their workgroups in this order, so
@@ -187,8 +188,9 @@ void launch_clear_hf(const BatchDev& b, hipStream_t s, const int* mask);
 // P <- damping C + (1 - damping) P.
 void launch_damp_keep(const BatchDev& b, hipStream_t s);
 void launch_damp_mix(const BatchDev& b, hipStream_t s);
-// Pulay DIIS after the convergence test (molecules still iterating, FP64
-// steps only): store F and F P - P F, F <- the extrapolated Fock matrix.
+// Pulay DIIS after the convergence test (molecules still iterating; the
+// history restarts when the step precision changes): store F and F P - P F,
+// F <- the extrapolated Fock matrix.
 void launch_diis(const BatchDev& b, hipStream_t s);
 void launch_diis_reset(const BatchDev& b, hipStream_t s);  // at the start of an SCF
 int64_t diis_part_doubles(int n_mol);
