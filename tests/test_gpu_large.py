@@ -124,6 +124,29 @@ def test_large_path_batch_equals_alone(large_engine):
         assert alone == res[i], i
 
 
+def test_diis_large_path(engine, large_engine):
+    """DIIS over the multi-workgroup eigensolver (eager iteration, no graph)
+    reaches the one-workgroup path's DIIS energy and the plain SCF's, with
+    pyridine below the threshold in the same batch."""
+    mols = [geometry.water_cluster(40, geometry.SplitMix64(1)), geometry.pyridine()]
+    say("case diis both solvers", [geometry.n_basis(m) for m in mols])
+    plain = engine.batch(mols).scf(threshold=1e-8)
+    small = engine.batch(mols)
+    small.set_diis(6)
+    ref = small.scf(threshold=1e-8)
+    big = large_engine.batch(mols)
+    big.set_diis(6)
+    res = big.scf(threshold=1e-8)
+    for p, a, c in zip(plain, ref, res):
+        say("diis steps", a["steps"], c["steps"], "plain", p["steps"], c["e_tot"] - a["e_tot"])
+        assert c["converged"] and c["status"] == 0
+        assert abs(c["e_tot"] - a["e_tot"]) <= 1e-7
+        assert abs(c["e_tot"] - p["e_tot"]) <= 1e-6
+        assert c["steps"] < p["steps"]
+    small.close()
+    big.close()
+
+
 def test_mixed_policy_large_path(large_engine):
     mol = geometry.water_cluster(50, geometry.SplitMix64(12))
     say("case mixed policy large path N", geometry.n_basis(mol))
