@@ -206,6 +206,19 @@ def water_box(args):
                     "tridiag_ms_per_iteration": round(st["tridiag"][0] / max(1, st["tridiag"][1]), 2)}
         print("water box %s: %d steps, %.2f s" % (pol, r["steps"], dt), file=sys.stderr, flush=True)
     out["mixed_minus_fp64_eV"] = out["mixed"]["e_tot_eV"] - out["fp64"]["e_tot_eV"]
+    if args.diis:
+        b.set_diis(args.diis)
+        for pol in ("fp64", "mixed"):
+            eng.timing_reset()
+            t = time.perf_counter()
+            r = b.scf(threshold=args.threshold, max_steps=args.box_max_steps, policy=pol)[0]
+            dt = time.perf_counter() - t
+            key = pol + "_diis%d" % args.diis
+            out[key] = {"seconds": round(dt, 3), "steps": r["steps"], "converged": r["converged"],
+                        "e_tot_eV": r["e_tot"],
+                        "minus_fp64_eV": r["e_tot"] - out["fp64"]["e_tot_eV"]}
+            print("water box %s: %d steps, %.2f s" % (key, r["steps"], dt), file=sys.stderr,
+                  flush=True)
     eng.timing(False)
     b.close()
     eng.close()

@@ -249,4 +249,10 @@ def test_water_box_1000(engine):
     c = b.scf(threshold=1e-6, max_steps=60, policy="mixed")[0]
     say("box mixed", c, "dE", c["e_tot"] - r["e_tot"])
     assert c["converged"] and abs(c["e_tot"] - r["e_tot"]) <= 1e-4
+    say("case water box fp64 scf with DIIS")
+    b.set_diis(6)
+    d = b.scf(threshold=1e-6, max_steps=60, policy="fp64")[0]
+    say("box diis", d, "dE", d["e_tot"] - r["e_tot"])
+    assert d["converged"] and d["steps"] < r["steps"]
+    assert abs(d["e_tot"] - r["e_tot"]) <= 1e-6
     b.close()
