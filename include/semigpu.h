@@ -208,6 +208,11 @@ int sg_server_create_classes(sg_context* ctx, double hbm_bytes, int max_slots, i
                              double threshold_ev, int max_steps, int policy, double switch_ev,
                              int n_classes, const int* bounds, sg_server** out);
 int sg_server_destroy(sg_server* srv);
+/* Pulay DIIS for the server's SCFs (no counterpart in the reference; the
+ * batch's sg_scf_set_diis for a stream): depth 0 = off (the default), else
+ * 2..8.  Allocates 2 depth doubles per element of the server's matrix pool
+ * beyond hbm_bytes.  Only while no request is queued or in flight. */
+int sg_server_set_diis(sg_server* srv, int depth);
 /* Queues one molecule (Angstrom); rejects elements outside H, C, N, O and
  * molecules larger than the server's pools. */
 int sg_server_submit(sg_server* srv, long long request_id, int n_atoms, const int* Z,

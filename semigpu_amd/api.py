@@ -37,7 +37,7 @@ EXPORTS = (
     "sg_get_eigen", "sg_set_eigen", "sg_batch_create_bohr", "sg_set_pair_blocks",
     "sg_get_status", "sg_calibration_copy", "sg_timing_enable", "sg_timing_read", "sg_timing_reset",
     "sg_server_create", "sg_server_destroy", "sg_server_submit", "sg_server_step",
-    "sg_server_poll", "sg_server_clock", "sg_ctx_set_large_basis", "sg_server_create_classes",
+    "sg_server_poll", "sg_server_clock", "sg_server_set_diis", "sg_ctx_set_large_basis", "sg_server_create_classes",
     "sg_heat_of_formation",
 )
 
@@ -102,6 +102,7 @@ def load_library(path: str = LIB_PATH):
     L.sg_scf_fp32_steps.argtypes = [_vp, _ip]
     L.sg_scf_set_damping.argtypes = [_vp, ctypes.c_double]
     L.sg_scf_set_diis.argtypes = [_vp, ctypes.c_int]
+    L.sg_server_set_diis.argtypes = [_vp, ctypes.c_int]
     L.sg_pair_block_count.argtypes = [_vp, ctypes.c_int]
     L.sg_pair_block_count.restype = ctypes.c_long
     L.sg_get_pair_blocks.argtypes = [_vp, ctypes.c_int, _dp]
@@ -364,6 +365,10 @@ class Server:
             self.close()
         except Exception:
             pass
+
+    def set_diis(self, depth: int):
+        """Pulay DIIS of the server's SCFs (0 = off, else 2..8); only while idle."""
+        _check(self.L.sg_server_set_diis(self.h, int(depth)))
 
     def submit(self, request_id: int, molecule) -> None:
         Z = np.asarray([int(a[0]) for a in molecule], dtype=np.int32)

@@ -561,6 +561,7 @@ int sg_scf_set_diis(sg_batch* b, int depth) {
   d.diis_c = (double*)(base + o_c);
   d.diis_k = (int*)(base + o_k);
   d.diis_p = (int*)(base + o_p);
+  d.diis_stride = d.n_mat;
   d.diis_m = depth;
   return SG_OK;
 }

@@ -18,7 +18,8 @@
 //                   (c = 1 on the newest slot);
 //   4. k_diis_mix   F <- sum_j c_j F_j.
 // The matrices are the full N x N row-major ones of the batch; history slot j
-// of a molecule is at hist + j * n_mat + mat0.
+// of a molecule is at hist + j * stride + mat0 (stride: the batch's sum N^2,
+// or the server's matrix pool).
 #include "sg_kernels.h"
 
 namespace sg {
@@ -42,7 +43,7 @@ __global__ void __launch_bounds__(256) k_diis_err(const MolDesc* __restrict__ mo
                                                   const double* __restrict__ F,
                                                   const double* __restrict__ P,
                                                   double* __restrict__ histF,
-                                                  double* __restrict__ histE, int64_t n_mat,
+                                                  double* __restrict__ histE, int64_t stride,
                                                   const int* __restrict__ kcount, int depth,
                                                   const int* __restrict__ active,
                                                   const int* __restrict__ status) {
@@ -105,8 +106,8 @@ __global__ void __launch_bounds__(256) k_diis_err(const MolDesc* __restrict__ mo
     __syncthreads();
   }
   const int s = diis_slot(kcount, m, depth);
-  double* e = histE + (int64_t)s * n_mat + M.mat0;
-  double* h = histF + (int64_t)s * n_mat + M.mat0;
+  double* e = histE + (int64_t)s * stride + M.mat0;
+  double* h = histF + (int64_t)s * stride + M.mat0;
   // one writer per entry: the lane of the lower-triangle position writes both
   // halves (the diagonal of a commutator of symmetric matrices is 0)
 #pragma unroll
@@ -128,7 +129,7 @@ __global__ void __launch_bounds__(256) k_diis_err(const MolDesc* __restrict__ mo
 
 // part[(m * kChunks + chunk) * kDiisMax + j] = chunk's share of <e_s, e_j>.
 __global__ void __launch_bounds__(256) k_diis_dot(const MolDesc* __restrict__ mols,
-                                                  const double* __restrict__ histE, int64_t n_mat,
+                                                  const double* __restrict__ histE, int64_t stride,
                                                   const int* __restrict__ kcount, int depth,
                                                   double* __restrict__ part,
                                                   const int* __restrict__ active,
@@ -139,7 +140,7 @@ __global__ void __launch_bounds__(256) k_diis_dot(const MolDesc* __restrict__ mo
   const int64_t nn = (int64_t)M.n_ao * M.n_ao;
   const int s = diis_slot(kcount, m, depth);
   const int nv = min(kcount[m] + 1, depth);
-  const double* es = histE + (int64_t)s * n_mat + M.mat0;
+  const double* es = histE + (int64_t)s * stride + M.mat0;
   double acc[kDiisMax];
 #pragma unroll
   for (int j = 0; j < kDiisMax; ++j) acc[j] = 0.0;
@@ -149,7 +150,7 @@ __global__ void __launch_bounds__(256) k_diis_dot(const MolDesc* __restrict__ mo
     const double a = es[t];
 #pragma unroll
     for (int j = 0; j < kDiisMax; ++j)
-      if (j < nv) acc[j] = fma(a, histE[(int64_t)j * n_mat + M.mat0 + t], acc[j]);
+      if (j < nv) acc[j] = fma(a, histE[(int64_t)j * stride + M.mat0 + t], acc[j]);
   }
   __shared__ double red[kDiisMax][256];
 #pragma unroll
@@ -240,7 +241,7 @@ __global__ void k_diis_solve(int n_mol, int* __restrict__ kcount, int depth,
 
 __global__ void __launch_bounds__(256) k_diis_mix(const MolDesc* __restrict__ mols,
                                                   double* __restrict__ F,
-                                                  const double* __restrict__ histF, int64_t n_mat,
+                                                  const double* __restrict__ histF, int64_t stride,
                                                   const double* __restrict__ coef,
                                                   const int* __restrict__ active,
                                                   const int* __restrict__ status) {
@@ -257,14 +258,15 @@ __global__ void __launch_bounds__(256) k_diis_mix(const MolDesc* __restrict__ mo
     double v = 0.0;
 #pragma unroll
     for (int j = 0; j < kDiisMax; ++j)  // unused slots hold c = 0 and undefined matrices
-      if (c[j] != 0.0) v = fma(c[j], histF[(int64_t)j * n_mat + M.mat0 + t], v);
+      if (c[j] != 0.0) v = fma(c[j], histF[(int64_t)j * stride + M.mat0 + t], v);
     f[t] = v;
   }
 }
 
-__global__ void k_diis_reset(int n_mol, int* __restrict__ kcount, int* __restrict__ hist_prec) {
+__global__ void k_diis_reset(int n_mol, int* __restrict__ kcount, int* __restrict__ hist_prec,
+                             const int* __restrict__ mask) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m < n_mol) {
+  if (m < n_mol && (!mask || mask[m])) {
     kcount[m] = 0;
     hist_prec[m] = -1;
   }
@@ -285,9 +287,9 @@ __global__ void k_diis_begin(int n_mol, int* __restrict__ kcount, int* __restric
 
 }  // namespace
 
-void launch_diis_reset(const BatchDev& b, hipStream_t s) {
+void launch_diis_reset(const BatchDev& b, hipStream_t s, const int* mask) {
   if (b.n_mol && b.diis_m)
-    k_diis_reset<<<(b.n_mol + 255) / 256, 256, 0, s>>>(b.n_mol, b.diis_k, b.diis_p);
+    k_diis_reset<<<(b.n_mol + 255) / 256, 256, 0, s>>>(b.n_mol, b.diis_k, b.diis_p, mask);
 }
 
 void launch_diis(const BatchDev& b, hipStream_t s) {
@@ -296,12 +298,12 @@ void launch_diis(const BatchDev& b, hipStream_t s) {
   k_diis_begin<<<(b.n_mol + 255) / 256, 256, 0, s>>>(b.n_mol, b.diis_k, b.diis_p, b.prev_prec,
                                                      b.active, b.status);
   k_diis_err<<<dim3(nt * (nt + 1) / 2, b.n_mol), 256, 0, s>>>(
-      b.mols, b.F, b.P, b.diis_F, b.diis_E, b.n_mat, b.diis_k, b.diis_m, b.active, b.status);
-  k_diis_dot<<<dim3(kChunks, b.n_mol), 256, 0, s>>>(b.mols, b.diis_E, b.n_mat, b.diis_k, b.diis_m,
+      b.mols, b.F, b.P, b.diis_F, b.diis_E, b.diis_stride, b.diis_k, b.diis_m, b.active, b.status);
+  k_diis_dot<<<dim3(kChunks, b.n_mol), 256, 0, s>>>(b.mols, b.diis_E, b.diis_stride, b.diis_k, b.diis_m,
                                                     b.diis_part, b.active, b.status);
   k_diis_solve<<<(b.n_mol + 63) / 64, 64, 0, s>>>(b.n_mol, b.diis_k, b.diis_m, b.diis_part,
                                                   b.diis_B, b.diis_c, b.active, b.status);
-  k_diis_mix<<<dim3(kChunks, b.n_mol), 256, 0, s>>>(b.mols, b.F, b.diis_F, b.n_mat, b.diis_c,
+  k_diis_mix<<<dim3(kChunks, b.n_mol), 256, 0, s>>>(b.mols, b.F, b.diis_F, b.diis_stride, b.diis_c,
                                                     b.active, b.status);
 }
 

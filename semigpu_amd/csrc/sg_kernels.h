@@ -80,6 +80,7 @@ struct BatchDev {
   // partial sums, the coefficients (kDiisMax per molecule), the number of
   // vectors stored in the current SCF and their precision (1 = FP32 steps).
   int diis_m = 0;
+  int64_t diis_stride = 0;  // doubles between history slots (>= the molecules' matrix range)
   double* diis_F = nullptr;
   double* diis_E = nullptr;
   double* diis_B = nullptr;
@@ -192,7 +193,8 @@ void launch_damp_mix(const BatchDev& b, hipStream_t s);
 // history restarts when the step precision changes): store F and F P - P F,
 // F <- the extrapolated Fock matrix.
 void launch_diis(const BatchDev& b, hipStream_t s);
-void launch_diis_reset(const BatchDev& b, hipStream_t s);  // at the start of an SCF
+// At the start of an SCF: every molecule, or the masked ones.
+void launch_diis_reset(const BatchDev& b, hipStream_t s, const int* mask = nullptr);
 int64_t diis_part_doubles(int n_mol);
 // Eigensolver of F (all = every eigenpair, else the selected subspace of the
 // active molecules): tridiagonalisation, bisection, inverse iteration,

@@ -158,6 +158,7 @@ struct Lane {
   RangeAlloc atom;
   bool busy = false;        // an iteration and its readback are in flight
   int hw = 0;               // slots covered by that iteration
+  void* diis_dev = nullptr;  // DIIS B matrices, partial sums, coefficients, counters per slot
   std::vector<int> fresh;   // slots it set up
 };
 
@@ -170,6 +171,7 @@ struct sg_server {
   // shared pools: N^2 matrices (H, F, P, C, 5 N^2 eigensolver workspace),
   // pair blocks (FP64 + FP32), length-N vectors; first fit over each
   void* pool = nullptr;
+  void* diis_pool = nullptr;  // DIIS histories over the matrix pool (sg_server_set_diis)
   BatchDev shared;
   int64_t mat_cap = 0, blk_cap = 0, vec_cap = 0;
   RangeAlloc mat, blk, vec;
@@ -183,11 +185,13 @@ struct sg_server {
     for (Lane* L : lanes) {
       if (L->stream) (void)hipStreamSynchronize(L->stream);
       if (L->dev) (void)hipFree(L->dev);
+      if (L->diis_dev) (void)hipFree(L->diis_dev);
       if (L->ev) (void)hipEventDestroy(L->ev);
       if (L->stream) (void)hipStreamDestroy(L->stream);
       delete L;
     }
     if (pool) (void)hipFree(pool);
+    if (diis_pool) (void)hipFree(diis_pool);
   }
 };
 
@@ -491,6 +495,7 @@ int lane_launch(sg_server* S, Lane* L) {
       StageTimer t(c, ST_CONTROL, s);
       launch_set_prec(nb, s, S->policy == SG_POLICY_MIXED ? 1 : 0, L->mask_new);
       launch_scf_init(nb, s, S->max_steps, S->policy, L->mask_new);
+      launch_diis_reset(nb, s, L->mask_new);
     }
     int r = check_launch("server setup");
     if (r) return r;
@@ -613,6 +618,68 @@ int sg_server_create_classes(sg_context* ctx, double hbm_bytes, int max_slots, i
                              int n_classes, const int* bounds, sg_server** out) {
   return server_create(ctx, hbm_bytes, max_slots, max_atoms, threshold_ev, max_steps, policy,
                        switch_ev, n_classes, bounds, out);
+}
+
+int sg_server_set_diis(sg_server* S, int depth) {
+  if (!S) return fail(SG_ERR_INVALID, "null server");
+  if (depth != 0 && (depth < 2 || depth > kDiisMax))
+    return fail(SG_ERR_INVALID, "DIIS depth must be 0 or lie in [2, " + std::to_string(kDiisMax) + "]");
+  for (Lane* L : S->lanes)
+    for (const Slot& sl : L->slots)
+      if (sl.used || L->busy || !L->queue.empty())
+        return fail(SG_ERR_INVALID, "sg_server_set_diis: requests in flight (set the DIIS depth "
+                                    "before submitting or after draining)");
+  HIP_TRY(hipSetDevice(S->ctx->device));
+  for (Lane* L : S->lanes) HIP_TRY(hipStreamSynchronize(L->stream));
+  auto clear = [](BatchDev& d) {
+    d.diis_m = 0;
+    d.diis_stride = 0;
+    d.diis_F = d.diis_E = d.diis_B = d.diis_part = d.diis_c = nullptr;
+    d.diis_k = d.diis_p = nullptr;
+  };
+  if (S->diis_pool) (void)hipFree(S->diis_pool);
+  S->diis_pool = nullptr;
+  clear(S->shared);
+  for (Lane* L : S->lanes) {
+    if (L->diis_dev) (void)hipFree(L->diis_dev);
+    L->diis_dev = nullptr;
+    clear(L->d);
+  }
+  if (depth == 0) return SG_OK;
+  const size_t hist = sizeof(double) * (size_t)depth * (size_t)S->mat_cap;
+  hipError_t e = hipMalloc(&S->diis_pool, 2 * hist);
+  if (e != hipSuccess) {
+    S->diis_pool = nullptr;
+    return fail(SG_ERR_HIP, std::string("sg_server_set_diis: hipMalloc(") +
+                                std::to_string(2 * hist) + "): " + hipGetErrorString(e));
+  }
+  double* hF = (double*)S->diis_pool;
+  double* hE = (double*)((char*)S->diis_pool + hist);
+  for (Lane* L : S->lanes) {
+    const size_t ns = (size_t)L->max_slots + 1;
+    const size_t o_B = 0, o_part = o_B + sizeof(double) * ns * kDiisMax * kDiisMax,
+                 o_c = o_part + sizeof(double) * (size_t)diis_part_doubles((int)ns),
+                 o_k = o_c + sizeof(double) * ns * kDiisMax, o_p = o_k + sizeof(int) * ns,
+                 end = o_p + sizeof(int) * ns;
+    e = hipMalloc(&L->diis_dev, end);
+    if (e != hipSuccess) {
+      L->diis_dev = nullptr;
+      return fail(SG_ERR_HIP, std::string("sg_server_set_diis: hipMalloc(") +
+                                  std::to_string(end) + "): " + hipGetErrorString(e));
+    }
+    char* base = (char*)L->diis_dev;
+    BatchDev& d = L->d;
+    d.diis_F = hF;
+    d.diis_E = hE;
+    d.diis_B = (double*)(base + o_B);
+    d.diis_part = (double*)(base + o_part);
+    d.diis_c = (double*)(base + o_c);
+    d.diis_k = (int*)(base + o_k);
+    d.diis_p = (int*)(base + o_p);
+    d.diis_stride = S->mat_cap;
+    d.diis_m = depth;
+  }
+  return SG_OK;
 }
 
 int sg_server_destroy(sg_server* S) {

@@ -158,3 +158,34 @@ def test_server_edge_molecules(engine):
     by_id = {r["id"]: r for r in out}
     for i, m in enumerate(mols):
         assert {k: by_id[i][k] for k in KEYS} == engine.batch([m]).scf(threshold=1e-6)[0], i
+
+
+def test_server_diis_equals_alone(engine):
+    """DIIS in the server: every result equals the molecule alone as a batch of
+    one with the same DIIS depth, bit for bit (FP64 and mixed policy), in no
+    more steps than the plain SCF and at its energy; the depth is only set
+    while the server is idle."""
+    mols = stream()
+    for policy in ("fp64", "mixed"):
+        srv = engine.server(hbm_bytes=48e6, max_slots=6, max_atoms=600, threshold=1e-6,
+                            policy=policy)
+        srv.set_diis(6)
+        for i, m in enumerate(mols):
+            srv.submit(i, m)
+        with pytest.raises(api.SemiGpuError, match="in flight"):
+            srv.set_diis(4)
+        out = {r["id"]: r for r in srv.drain()}
+        srv.set_diis(0)
+        srv.close()
+        assert sorted(out) == list(range(len(mols)))
+        for i, m in enumerate(mols):
+            b = engine.batch([m])
+            plain = b.scf(threshold=1e-6, policy=policy)[0]
+            b.set_diis(6)
+            alone = b.scf(threshold=1e-6, policy=policy)[0]
+            b.close()
+            say("server diis", policy, i, out[i]["steps"], "plain", plain["steps"],
+                out[i]["e_tot"] - plain["e_tot"])
+            assert {k: out[i][k] for k in KEYS} == alone, (policy, i)
+            assert alone["converged"] and alone["steps"] <= plain["steps"]
+            assert abs(alone["e_tot"] - plain["e_tot"]) <= 1e-5
