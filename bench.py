@@ -134,11 +134,13 @@ SERVE_CLASSES = (300, 700)
 
 
 def serve(eng, rate: float, seconds: float, threshold: float, policy: str, seed: int = 7,
-          limit_s: float = 90.0):
+          limit_s: float = 90.0, diis: int = 0):
     """Poisson arrivals of alkanes C_nH_2n+2 (n <= 60) and water clusters (n <= 200)."""
     reqs = geometry.serving_stream(rate, seconds, seed)
     srv = eng.server(hbm_bytes=64e9, max_slots=16384, max_atoms=1 << 21, threshold=threshold,
                      max_steps=100, policy=policy, classes=SERVE_CLASSES)
+    if diis:
+        srv.set_diis(diis)
     done = []
     i = 0
     t0 = srv.clock()
@@ -454,6 +456,10 @@ def main():
                                args.policy)
         out["latency"] = {"p50_ms": out["serving"]["p50_ms"], "p99_ms": out["serving"]["p99_ms"],
                           "basis": "serving stream, arrival to result"}
+        if args.diis:
+            out["serving_diis"] = serve(eng, args.serve_rate, args.serve_seconds, args.threshold,
+                                        args.policy, diis=args.diis)
+            out["serving_diis"]["diis_depth"] = args.diis
         print("serving: %.1f s" % (time.time() - t), file=sys.stderr, flush=True)
         extra = [x for x in args.serve_extra.split(",") if x]
         if extra:

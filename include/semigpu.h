@@ -131,8 +131,8 @@ int sg_scf_set_damping(sg_batch* b, double beta);
  * (the default) = off, else 2..8; the histories take 2 depth sum N^2 doubles,
  * allocated here.  Under SG_POLICY_MIXED the FP32 steps extrapolate over
  * FP32 steps only, and the history restarts at the first FP64 step.  A
- * converged molecule keeps the Fock matrix of its final density; the server
- * does not extrapolate. */
+ * converged molecule keeps the Fock matrix of its final density; for the
+ * server, sg_server_set_diis. */
 int sg_scf_set_diis(sg_batch* b, int depth);
 /* FP32 steps of each molecule in the last sg_scf_run_policy (0 under
  * SG_POLICY_FP64); steps32 gets n_mol entries.  No counterpart in the
