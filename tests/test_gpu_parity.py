@@ -404,3 +404,22 @@ def test_empty_molecule_rejected(engine):
                            Z.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                            xyz.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(h))
     assert rc == -1 and b"no atoms" in L.sg_last_error()
+
+
+def test_diis_against_numpy_restatement(engine):
+    """The device DIIS (FP64 policy) takes the steps of the numpy restatement
+    over the oracle's Fock builds and lands on its energy."""
+    import diis_ref
+    mols = [geometry.pyridine(), geometry.water_cluster(10, geometry.SplitMix64(5)),
+            geometry.alkane(6), geometry.water_cluster(20, geometry.SplitMix64(20))]
+    for depth in (3, 6):
+        b = engine.batch(mols)
+        b.set_diis(depth)
+        res = b.scf(threshold=1e-8)
+        b.close()
+        for m, r in zip(mols, res):
+            e, steps, conv, _ = diis_ref.diis_scf(m, depth, 1e-8)
+            say("diis depth", depth, len(m), "gpu", r["steps"], "numpy", steps, r["e_tot"] - e)
+            assert r["converged"] and conv
+            assert r["steps"] == steps
+            assert abs(r["e_tot"] - e) <= 1e-8
