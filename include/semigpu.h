@@ -149,6 +149,19 @@ int sg_get_status(const sg_batch* b, int* status);
  * MNDO_parameters.cpp:58-121).  Host arithmetic only (no device); no
  * counterpart in the reference, which stops at E_tot (semiGPU.cc:95). */
 int sg_heat_of_formation(int n_atoms, const int* Z, double e_tot_ev, double* dhf_kcal);
+/* Nuclear gradient of the MNDO total energy by central differences (no
+ * counterpart in the reference, whose README lists gradients and a geometry
+ * optimiser as to do; analytic gradients are not built): one batch of the
+ * molecule and its 6 n_atoms displaced copies (each atom moved by +-step_ang
+ * along x, y, z), FP64 SCF to threshold_ev (200 steps at most; DIIS of
+ * diis_depth, 0 = off), grad[3 a + k] = (E(+) - E(-)) / (2 step) in eV/A.
+ * The truncation error is O(step^2), the SCF's is about threshold / step:
+ * step 1e-3 A with threshold 1e-9 eV gives about 1e-6 eV/A.  e_tot (may be
+ * null) gets the energy of the undisplaced geometry, all_converged (may be
+ * null) whether every SCF converged. */
+int sg_gradient_fd(sg_context* ctx, int n_atoms, const int* Z, const double* xyz_ang,
+                   double step_ang, double threshold_ev, int diis_depth, double* e_tot,
+                   double* grad, int* all_converged);
 
 /* Host copies for the reference's host-side views. */
 long sg_pair_block_count(const sg_batch* b, int mol);

@@ -32,7 +32,7 @@ EXPORTS = (
     "sg_last_error", "sg_version", "sg_ctx_create", "sg_ctx_destroy", "sg_batch_create",
     "sg_batch_destroy", "sg_batch_molecule_info", "sg_two_center", "sg_hcore", "sg_core_core",
     "sg_guess_density", "sg_fock", "sg_eigh", "sg_density", "sg_energy", "sg_scf_run",
-    "sg_scf_run_policy", "sg_scf_fp32_steps", "sg_scf_set_damping", "sg_scf_set_diis",
+    "sg_scf_run_policy", "sg_scf_fp32_steps", "sg_scf_set_damping", "sg_scf_set_diis", "sg_gradient_fd",
     "sg_pair_block_count", "sg_get_pair_blocks", "sg_get_packed", "sg_set_packed",
     "sg_get_eigen", "sg_set_eigen", "sg_batch_create_bohr", "sg_set_pair_blocks",
     "sg_get_status", "sg_calibration_copy", "sg_timing_enable", "sg_timing_read", "sg_timing_reset",
@@ -102,6 +102,8 @@ def load_library(path: str = LIB_PATH):
     L.sg_scf_fp32_steps.argtypes = [_vp, _ip]
     L.sg_scf_set_damping.argtypes = [_vp, ctypes.c_double]
     L.sg_scf_set_diis.argtypes = [_vp, ctypes.c_int]
+    L.sg_gradient_fd.argtypes = [_vp, ctypes.c_int, _ip, _dp, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_int, _dp, _dp, _ip]
     L.sg_server_set_diis.argtypes = [_vp, ctypes.c_int]
     L.sg_pair_block_count.argtypes = [_vp, ctypes.c_int]
     L.sg_pair_block_count.restype = ctypes.c_long
@@ -191,6 +193,20 @@ class Engine:
 
     def server(self, **kw) -> "Server":
         return Server(self, **kw)
+
+    def gradient_fd(self, molecule, step: float = 1e-3, threshold: float = 1e-9,
+                    diis: int = 6):
+        """Nuclear gradient (eV/A, n_atoms x 3) of the MNDO total energy by
+        central differences over one batched SCF of the 6 n_atoms displaced
+        geometries (sg_gradient_fd).  Returns (gradient, e_tot, all converged)."""
+        Z = np.ascontiguousarray([int(a[0]) for a in molecule], dtype=np.int32)
+        xyz = np.ascontiguousarray([a[1:4] for a in molecule], dtype=np.float64).reshape(-1)
+        g = np.zeros(3 * len(Z))
+        e = ctypes.c_double()
+        conv = ctypes.c_int()
+        _check(self.L.sg_gradient_fd(self.h, len(Z), _i(Z), _d(xyz), float(step), float(threshold),
+                                     int(diis), ctypes.byref(e), _d(g), ctypes.byref(conv)))
+        return g.reshape(-1, 3), e.value, bool(conv.value)
 
     def set_large_basis(self, n_min: int):
         """Molecules of later batches with N >= n_min take the multi-workgroup

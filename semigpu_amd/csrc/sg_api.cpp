@@ -785,6 +785,44 @@ int sg_heat_of_formation(int n_atoms, const int* Z, double e_tot_ev, double* dhf
   return SG_OK;
 }
 
+int sg_gradient_fd(sg_context* ctx, int n_atoms, const int* Z, const double* xyz_ang,
+                   double step_ang, double threshold_ev, int diis_depth, double* e_tot,
+                   double* grad, int* all_converged) {
+  if (!ctx || n_atoms <= 0 || !Z || !xyz_ang || !grad || !(step_ang > 0.0) ||
+      !(threshold_ev >= 0.0))
+    return fail(SG_ERR_INVALID, "sg_gradient_fd: bad arguments");
+  // molecule 0: the geometry; 1 + 2 (3 a + k) and 2 + 2 (3 a + k): atom a
+  // moved by +step and -step along axis k
+  const int n_mol = 1 + 6 * n_atoms;
+  std::vector<int> offs(n_mol + 1), zz((size_t)n_mol * n_atoms);
+  std::vector<double> xyz((size_t)n_mol * n_atoms * 3);
+  for (int m = 0; m < n_mol; ++m) {
+    offs[m] = m * n_atoms;
+    std::copy(Z, Z + n_atoms, zz.begin() + (size_t)m * n_atoms);
+    std::copy(xyz_ang, xyz_ang + 3 * n_atoms, xyz.begin() + (size_t)m * n_atoms * 3);
+    if (m) {
+      const int q = (m - 1) / 2, a = q / 3, k = q % 3;
+      xyz[((size_t)m * n_atoms + a) * 3 + k] += (m - 1) % 2 == 0 ? step_ang : -step_ang;
+    }
+  }
+  offs[n_mol] = n_mol * n_atoms;
+  sg_batch* b = nullptr;
+  int r = sg_batch_create(ctx, n_mol, offs.data(), zz.data(), xyz.data(), &b);
+  if (r) return r;
+  std::vector<sg_scf_result> res(n_mol);
+  r = diis_depth ? sg_scf_set_diis(b, diis_depth) : SG_OK;
+  if (!r) r = sg_scf_run_policy(b, threshold_ev, 200, SG_POLICY_FP64, 1e-3, res.data());
+  sg_batch_destroy(b);
+  if (r) return r;
+  int conv = 1;
+  for (const sg_scf_result& x : res) conv = conv && x.converged && x.status == 0;
+  for (int q = 0; q < 3 * n_atoms; ++q)
+    grad[q] = (res[1 + 2 * q].e_tot - res[2 + 2 * q].e_tot) / (2.0 * step_ang);
+  if (e_tot) *e_tot = res[0].e_tot;
+  if (all_converged) *all_converged = conv;
+  return SG_OK;
+}
+
 int sg_get_status(const sg_batch* b, int* status) {
   if (!b || !status) return fail(SG_ERR_INVALID, "bad arguments");
   HIP_TRY(hipSetDevice(b->ctx->device));
