@@ -80,3 +80,35 @@ def test_gradient_fd_against_oracle(engine):
     g1 = engine.gradient_fd(water(), step=1e-3)[0]
     g0 = engine.gradient_fd(water(), step=1e-3, diis=0)[0]
     assert np.abs(g1 - g0).max() <= 1e-5
+
+
+@pytest.mark.gpu
+def test_geometry_optimisation_reaches_published_mndo_geometry(engine):
+    """BFGS on the device energy and gradient from distorted starts lands on
+    the published MNDO equilibrium: H2O r_OH 0.943 A, HOH 106.8 deg, dH_f
+    -60.9 kcal/mol; CH4 r_CH 1.104 A, dH_f -11.9 kcal/mol."""
+    import math
+    from semigpu_amd import api
+    from semigpu_amd.optimize import optimize
+    t = math.radians(100.0) / 2
+    start = [(8, 0.0, 0.0, 0.0), (1, 1.0 * math.sin(t), 1.0 * math.cos(t), 0.0),
+             (1, -0.98 * math.sin(t), 0.98 * math.cos(t), 0.05)]
+    r = optimize(engine, start)
+    o, h1, h2 = (np.array(a[1:4]) for a in r["atoms"])
+    d1, d2 = np.linalg.norm(h1 - o), np.linalg.norm(h2 - o)
+    ang = math.degrees(math.acos((h1 - o) @ (h2 - o) / (d1 * d2)))
+    dhf = api.heat_of_formation(r["atoms"], r["e_tot"])
+    print("water opt", r["evaluations"], d1, d2, ang, dhf, np.abs(r["gradient"]).max(), flush=True)
+    assert np.abs(r["gradient"]).max() <= 2e-3
+    assert abs(d1 - 0.943) <= 0.002 and abs(d2 - 0.943) <= 0.002
+    assert abs(ang - 106.8) <= 0.3
+    assert abs(dhf + 60.9) <= 0.1
+    s = 1.15 / math.sqrt(3)
+    start = [(6, 0.0, 0.0, 0.0), (1, s, s, s), (1, -s, -s, s), (1, -s, s, -s), (1, s, -s, -s * 1.02)]
+    r = optimize(engine, start)
+    c = np.array(r["atoms"][0][1:4])
+    d = [np.linalg.norm(np.array(a[1:4]) - c) for a in r["atoms"][1:]]
+    dhf = api.heat_of_formation(r["atoms"], r["e_tot"])
+    print("methane opt", r["evaluations"], d, dhf, flush=True)
+    assert all(abs(x - 1.104) <= 0.002 for x in d)
+    assert abs(dhf + 11.9) <= 0.1
