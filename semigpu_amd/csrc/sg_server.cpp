@@ -624,11 +624,13 @@ int sg_server_set_diis(sg_server* S, int depth) {
   if (!S) return fail(SG_ERR_INVALID, "null server");
   if (depth != 0 && (depth < 2 || depth > kDiisMax))
     return fail(SG_ERR_INVALID, "DIIS depth must be 0 or lie in [2, " + std::to_string(kDiisMax) + "]");
-  for (Lane* L : S->lanes)
-    for (const Slot& sl : L->slots)
-      if (sl.used || L->busy || !L->queue.empty())
-        return fail(SG_ERR_INVALID, "sg_server_set_diis: requests in flight (set the DIIS depth "
-                                    "before submitting or after draining)");
+  for (Lane* L : S->lanes) {
+    bool idle = !L->busy && L->queue.empty();
+    for (const Slot& sl : L->slots) idle = idle && !sl.used;
+    if (!idle)
+      return fail(SG_ERR_INVALID, "sg_server_set_diis: requests in flight (set the DIIS depth "
+                                  "before submitting or after draining)");
+  }
   HIP_TRY(hipSetDevice(S->ctx->device));
   for (Lane* L : S->lanes) HIP_TRY(hipStreamSynchronize(L->stream));
   auto clear = [](BatchDev& d) {
@@ -637,14 +639,18 @@ int sg_server_set_diis(sg_server* S, int depth) {
     d.diis_F = d.diis_E = d.diis_B = d.diis_part = d.diis_c = nullptr;
     d.diis_k = d.diis_p = nullptr;
   };
-  if (S->diis_pool) (void)hipFree(S->diis_pool);
-  S->diis_pool = nullptr;
-  clear(S->shared);
-  for (Lane* L : S->lanes) {
-    if (L->diis_dev) (void)hipFree(L->diis_dev);
-    L->diis_dev = nullptr;
-    clear(L->d);
-  }
+  // back to no DIIS anywhere (also after a failed allocation below)
+  auto release = [&]() {
+    if (S->diis_pool) (void)hipFree(S->diis_pool);
+    S->diis_pool = nullptr;
+    clear(S->shared);
+    for (Lane* L : S->lanes) {
+      if (L->diis_dev) (void)hipFree(L->diis_dev);
+      L->diis_dev = nullptr;
+      clear(L->d);
+    }
+  };
+  release();
   if (depth == 0) return SG_OK;
   const size_t hist = sizeof(double) * (size_t)depth * (size_t)S->mat_cap;
   hipError_t e = hipMalloc(&S->diis_pool, 2 * hist);
@@ -664,6 +670,7 @@ int sg_server_set_diis(sg_server* S, int depth) {
     e = hipMalloc(&L->diis_dev, end);
     if (e != hipSuccess) {
       L->diis_dev = nullptr;
+      release();
       return fail(SG_ERR_HIP, std::string("sg_server_set_diis: hipMalloc(") +
                                   std::to_string(end) + "): " + hipGetErrorString(e));
     }
