@@ -249,9 +249,10 @@ def main():
     ap.add_argument("--water-box", action="store_true",
                     help="instead: the 1000-water box (N = 6000), FP64 and mixed SCF, one JSON line")
     ap.add_argument("--box-max-steps", type=int, default=60)
-    ap.add_argument("--diis", type=int, default=0,
-                    help="also time the batch with Pulay DIIS of this depth (opt-in SCF aid, "
-                         "not in the reference; reported beside value, never as it)")
+    ap.add_argument("--diis", type=int, default=6,
+                    help="also time the batch and the serving stream with Pulay DIIS of this "
+                         "depth (an SCF aid the reference does not have: reported beside value, "
+                         "never as it; 0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
