@@ -451,6 +451,18 @@ def main():
         out["plain_fp64"] = {"value": round(plain_value, 3), "unit": "molecules/s",
                              "sample": len(idx), "max_abs_dE_vs_batched_eV": max_dE_plain,
                              "batched_mixed_speedup": round(value / plain_value, 3)}
+    # central-difference gradients (sg_gradient_fd): one batched SCF of the
+    # molecule and its 6 n_atoms displaced copies, DIIS, threshold 1e-9 eV
+    grad = []
+    for name, mol in (("pyridine", geometry.pyridine()),
+                      ("(H2O)20", geometry.water_cluster(20, geometry.SplitMix64(20)))):
+        eng.gradient_fd(mol)
+        t = time.perf_counter()
+        g, e, gconv = eng.gradient_fd(mol)
+        grad.append({"molecule": name, "atoms": len(mol), "scf_batch": 1 + 6 * len(mol),
+                     "ms": round((time.perf_counter() - t) * 1e3, 2), "converged": gconv,
+                     "max_abs_sum_eV_per_A": float(np.abs(g.sum(axis=0)).max())})
+    out["gradient"] = grad
     if not args.no_serve:
         t = time.time()
         out["serving"] = serve(eng, args.serve_rate, args.serve_seconds, args.threshold,
