@@ -8,7 +8,8 @@
 //                   has S = 1), written with F into ring slot k mod m;
 //                   64 x 64 blocks of the lower triangle on the FP64 matrix
 //                   cores, e_ji = -e_ij;
-//   2. k_diis_dot   <e_k, e_j> for the stored slots, 16 chunks per molecule,
+//   2. k_diis_dot   <e_k, e_j> for the stored slots over the strict lower
+//                   triangles (twice their sum), 16 row chunks per molecule,
 //                   one partial sum per chunk and slot (no atomics: the sums
 //                   run in a fixed order, so a molecule's result does not
 //                   depend on its batch);
@@ -137,21 +138,23 @@ __global__ void __launch_bounds__(256) k_diis_dot(const MolDesc* __restrict__ mo
   const int m = blockIdx.y;
   if (!mol_live(active, status, m)) return;
   const MolDesc M = mols[m];
-  const int64_t nn = (int64_t)M.n_ao * M.n_ao;
+  const int n = M.n_ao;
   const int s = diis_slot(kcount, m, depth);
   const int nv = min(kcount[m] + 1, depth);
   const double* es = histE + (int64_t)s * stride + M.mat0;
   double acc[kDiisMax];
 #pragma unroll
   for (int j = 0; j < kDiisMax; ++j) acc[j] = 0.0;
-  const int64_t per = (nn + kChunks - 1) / kChunks;
-  const int64_t lo = blockIdx.x * per, hi = min(nn, lo + per);
-  for (int64_t t = lo + threadIdx.x; t < hi; t += blockDim.x) {
-    const double a = es[t];
+  // the error matrices are antisymmetric with a zero diagonal: <e_s, e_j> is
+  // twice the sum over the strict lower triangle, rows chunk, chunk + 16, ...
+  for (int i = blockIdx.x; i < n; i += kChunks)
+    for (int c = threadIdx.x; c < i; c += blockDim.x) {
+      const int64_t t = (int64_t)i * n + c;
+      const double a = es[t];
 #pragma unroll
-    for (int j = 0; j < kDiisMax; ++j)
-      if (j < nv) acc[j] = fma(a, histE[(int64_t)j * stride + M.mat0 + t], acc[j]);
-  }
+      for (int j = 0; j < kDiisMax; ++j)
+        if (j < nv) acc[j] = fma(a, histE[(int64_t)j * stride + M.mat0 + t], acc[j]);
+    }
   __shared__ double red[kDiisMax][256];
 #pragma unroll
   for (int j = 0; j < kDiisMax; ++j) red[j][threadIdx.x] = acc[j];
@@ -162,7 +165,7 @@ __global__ void __launch_bounds__(256) k_diis_dot(const MolDesc* __restrict__ mo
     __syncthreads();
   }
   if (threadIdx.x < kDiisMax)
-    part[((int64_t)m * kChunks + blockIdx.x) * kDiisMax + threadIdx.x] = red[threadIdx.x][0];
+    part[((int64_t)m * kChunks + blockIdx.x) * kDiisMax + threadIdx.x] = 2.0 * red[threadIdx.x][0];
 }
 
 // One thread per molecule: B update, the bordered system, the coefficients.
