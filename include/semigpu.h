@@ -56,6 +56,12 @@ int sg_ctx_destroy(sg_context* ctx);
  * basis a batch accepts is 8192.  No counterpart in the reference, whose
  * dsyev_ call (scfcalculation.cpp:169-203) takes any size on the host. */
 int sg_ctx_set_large_basis(sg_context* ctx, int n_min);
+/* Whether sg_scf_run / sg_scf_run_policy capture one SCF iteration of a batch
+ * as a HIP graph on first use and replay it (on, the default), or launch each
+ * iteration's kernels directly (off: no capture and instantiation per batch,
+ * the cheaper choice for batches that run once, e.g. a single molecule).  The
+ * results are the same either way.  No counterpart in the reference. */
+int sg_ctx_set_graphs(sg_context* ctx, int on);
 
 /* Molecules: atoms of molecule m are atom_offsets[m] .. atom_offsets[m+1]-1 of
  * Z[] / xyz_ang[3*i..3*i+2].  Replaces ReadXYZFile + ListAtomicOrbitals::SetOrbitals
@@ -98,21 +104,38 @@ int sg_energy(sg_batch* b, double* e_el);
 int sg_scf_run(sg_batch* b, double threshold_ev, int max_steps, sg_scf_result* results);
 
 /* Same SCF with a precision policy (no counterpart in the reference, which is
- * FP64 throughout).  SG_POLICY_FP64 is sg_scf_run.  SG_POLICY_MIXED runs the
- * Fock contractions, the Householder reduction and back-transformation of the
- * one-workgroup eigensolver (N <= 1280) and the density product in FP32 (the
- * products on the matrix cores), with bisection to 1e-10 ||T|| and one
- * inverse-iteration solve (factors stored in FP32), until a molecule's |dE| < switch_ev (1e-3 eV
- * suggested), or |dE| < 2^-20 |E_el| (8 FP32 ulps of that energy), or |dE|
- * stops falling by half per step, or 40 FP32 steps; then in FP64 to
- * convergence.  Integrals,
- * energies and the multi-workgroup eigensolver stay FP64, and convergence is
- * only tested between two FP64 steps.  Batches above 512 molecules reduce
- * their FP32 molecules in a launch of their own, whose FP32 sums run in
- * another order: such a batch agrees with its molecules alone to the SCF
- * threshold, not bit for bit (SG_POLICY_FP64 is bit-identical). */
+ * FP64 throughout).  SG_POLICY_FP64 is sg_scf_run.  The mixed policies run
+ * the early steps of each molecule at reduced precision, until its
+ * |dE| < switch_ev (1e-3 eV suggested), or |dE| falls below what the reduced
+ * arithmetic resolves on its energy (2^-20 |E_el|, 8 FP32 ulps; 2^-14 |E_el|
+ * for split BF16), or |dE| stops falling by half per step, or 40 reduced
+ * steps; then in FP64 to convergence, which is only tested between two FP64
+ * steps.  Integrals, Hcore, the energy sums and the multi-workgroup
+ * eigensolver are always FP64.
+ *   SG_POLICY_MIXED        Fock contractions in FP32 (over an FP32 copy of the
+ *                          integrals) and the density product in FP32 on the
+ *                          matrix cores; the eigensolver stays FP64.
+ *   SG_POLICY_MIXED_BF16   Fock contractions in FP32; the density product as
+ *                          split BF16 on the matrix cores (each FP64 vector
+ *                          entry y = hi + lo with hi, lo BF16; hi hi + hi lo +
+ *                          lo hi accumulated in FP32); the eigensolver stays
+ *                          FP64.
+ *   SG_POLICY_MIXED_EIG32  as SG_POLICY_MIXED, and in addition the Householder
+ *                          reduction and back-transformation of the
+ *                          one-workgroup eigensolver (N <= 1280) run in FP32,
+ *                          with bisection to 1e-10 ||T|| and one
+ *                          inverse-iteration solve (factors stored in FP32).
+ *                          Opt-in: it departs from an FP64 eigensolver.
+ *                          Batches above 512 molecules reduce their FP32
+ *                          molecules in a launch of their own, whose sums run
+ *                          in another order: such a batch agrees with its
+ *                          molecules alone to the SCF threshold, not bit for
+ *                          bit.  The other policies are bit-identical in any
+ *                          batch. */
 #define SG_POLICY_FP64 0
 #define SG_POLICY_MIXED 1
+#define SG_POLICY_MIXED_BF16 2
+#define SG_POLICY_MIXED_EIG32 3
 int sg_scf_run_policy(sg_batch* b, double threshold_ev, int max_steps, int policy,
                       double switch_ev, sg_scf_result* results);
 /* Density damping for the batch's later sg_scf_run / sg_scf_run_policy calls

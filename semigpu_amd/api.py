@@ -38,7 +38,7 @@ EXPORTS = (
     "sg_get_status", "sg_calibration_copy", "sg_timing_enable", "sg_timing_read", "sg_timing_reset",
     "sg_server_create", "sg_server_destroy", "sg_server_submit", "sg_server_step",
     "sg_server_poll", "sg_server_clock", "sg_server_set_diis", "sg_ctx_set_large_basis", "sg_server_create_classes",
-    "sg_heat_of_formation",
+    "sg_heat_of_formation", "sg_ctx_set_graphs",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -61,6 +61,10 @@ class Served(ctypes.Structure):
 
 class SemiGpuError(RuntimeError):
     pass
+
+
+# precision policies of sg_scf_run_policy / sg_server_create (include/semigpu.h)
+POLICIES = {"fp64": 0, "mixed": 1, "mixed_bf16": 2, "mixed_eig32": 3}
 
 
 _lib = None
@@ -88,6 +92,7 @@ def load_library(path: str = LIB_PATH):
     L.sg_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
     L.sg_ctx_destroy.argtypes = [_vp]
     L.sg_ctx_set_large_basis.argtypes = [_vp, ctypes.c_int]
+    L.sg_ctx_set_graphs.argtypes = [_vp, ctypes.c_int]
     L.sg_heat_of_formation.argtypes = [ctypes.c_int, _ip, ctypes.c_double, _dp]
     L.sg_batch_create.argtypes = [_vp, ctypes.c_int, _ip, _ip, _dp, ctypes.POINTER(_vp)]
     L.sg_batch_destroy.argtypes = [_vp]
@@ -188,8 +193,8 @@ class Engine:
         except Exception:
             pass
 
-    def batch(self, molecules: Sequence[Sequence[Sequence[float]]]) -> "Batch":
-        return Batch(self, molecules)
+    def batch(self, molecules: Sequence[Sequence[Sequence[float]]] = None, packed=None) -> "Batch":
+        return Batch(self, molecules, packed)
 
     def server(self, **kw) -> "Server":
         return Server(self, **kw)
@@ -213,6 +218,10 @@ class Engine:
         eigensolver (default: only those above the one-workgroup limit, 1280)."""
         _check(self.L.sg_ctx_set_large_basis(self.h, int(n_min)))
 
+    def graphs(self, on: bool = True):
+        """Capture SCF iterations as HIP graphs (default) or launch them directly."""
+        _check(self.L.sg_ctx_set_graphs(self.h, 1 if on else 0))
+
     def timing(self, on: bool = True):
         _check(self.L.sg_timing_enable(self.h, 1 if on else 0))
 
@@ -230,35 +239,49 @@ class Engine:
         _check(self.L.sg_calibration_copy(self.h, int(nbytes)))
 
 
+def pack(molecules):
+    """Molecules [(Z, x, y, z), ...] (Angstrom) -> (atom offsets, Z, xyz), the
+    arrays sg_batch_create takes."""
+    offs = [0]
+    Z: List[int] = []
+    xyz: List[float] = []
+    for mol in molecules:
+        for a in mol:
+            Z.append(int(a[0]))
+            xyz.extend(float(c) for c in a[1:4])
+        offs.append(len(Z))
+    return (np.asarray(offs, dtype=np.int32), np.asarray(Z, dtype=np.int32),
+            np.asarray(xyz, dtype=np.float64))
+
+
 class Batch:
     """A batch of molecules resident on the device (sg_batch)."""
 
-    def __init__(self, engine: Engine, molecules):
+    def __init__(self, engine: Engine, molecules=None, packed=None):
+        """molecules [(Z, x, y, z), ...] each, or packed = pack(molecules)."""
         self.engine = engine
         self.L = engine.L
-        offs = [0]
-        Z: List[int] = []
-        xyz: List[float] = []
-        for mol in molecules:
-            for a in mol:
-                Z.append(int(a[0]))
-                xyz.extend(float(c) for c in a[1:4])
-            offs.append(len(Z))
-        self.n_mol = len(molecules)
-        self.offsets = np.asarray(offs, dtype=np.int32)
-        self.Z = np.asarray(Z, dtype=np.int32)
-        self.xyz = np.asarray(xyz, dtype=np.float64)
+        self.offsets, self.Z, self.xyz = packed if packed is not None else pack(molecules)
+        self.n_mol = len(self.offsets) - 1
         h = _vp()
         _check(self.L.sg_batch_create(engine.h, self.n_mol, _i(self.offsets), _i(self.Z),
                                       _d(self.xyz), ctypes.byref(h)))
         self.h = h
         engine._batches.add(self)
-        self.info = []
-        for m in range(self.n_mol):
-            na, nao, nocc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-            _check(self.L.sg_batch_molecule_info(h, m, ctypes.byref(na), ctypes.byref(nao),
-                                                 ctypes.byref(nocc)))
-            self.info.append((na.value, nao.value, nocc.value))
+        self._info = None
+
+    @property
+    def info(self):
+        """(atoms, basis functions, occupied orbitals) per molecule."""
+        if self._info is None:
+            info = []
+            for m in range(self.n_mol):
+                na, nao, nocc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+                _check(self.L.sg_batch_molecule_info(self.h, m, ctypes.byref(na),
+                                                     ctypes.byref(nao), ctypes.byref(nocc)))
+                info.append((na.value, nao.value, nocc.value))
+            self._info = info
+        return self._info
 
     def close(self):
         if self.h:
@@ -305,9 +328,12 @@ class Batch:
 
     def scf(self, threshold: float = 5e-3, max_steps: int = 100, policy: str = "fp64",
             switch_ev: float = 1e-3):
-        """SCF for every molecule; policy "fp64" (the reference) or "mixed"."""
+        """SCF for every molecule; policy "fp64" (the reference), "mixed" (FP32
+        Fock and density, FP64 eigensolver), "mixed_bf16" (FP32 Fock, split-BF16
+        density) or "mixed_eig32" (also the eigensolver's reduction and
+        back-transform in FP32; opt-in)."""
         res = (ScfResult * self.n_mol)()
-        pol = {"fp64": 0, "mixed": 1}[policy]
+        pol = POLICIES[policy]
         _check(self.L.sg_scf_run_policy(self.h, threshold, max_steps, pol, switch_ev, res))
         return [{"e_el": r.e_el, "e_cc": r.e_cc, "e_tot": r.e_tot, "steps": r.steps,
                  "converged": bool(r.converged), "status": r.status} for r in res]
@@ -363,7 +389,7 @@ class Server:
         self.engine = engine
         self.L = engine.L
         h = _vp()
-        pol = {"fp64": 0, "mixed": 1}[policy]
+        pol = POLICIES[policy]
         bounds = np.asarray(list(classes), dtype=np.int32)
         _check(self.L.sg_server_create_classes(engine.h, float(hbm_bytes), max_slots, max_atoms,
                                                threshold, max_steps, pol, switch_ev,

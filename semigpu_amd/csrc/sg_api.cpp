@@ -47,12 +47,56 @@ struct sg_batch {
   std::vector<double> e_cc;
   std::vector<int> large_idx;       // molecules on the multi-workgroup eigensolver
   std::vector<MolDesc> large_mols;  // their descriptors
+  size_t dev_bytes = 0;             // size of dev_mem (returned to the context's cache)
   hipGraphExec_t iter_exec = nullptr;  // captured SCF iteration
   double g_threshold = -1.0, g_switch = -1.0;
   int g_max_steps = -1, g_mixed = -1;
   double g_damping = -1.0;
   void* diis_mem = nullptr;  // DIIS histories (sg_scf_set_diis)
 };
+
+// Where atom i's row starts in each list of a batch: the all-pairs list, the
+// type-sorted lists (-1 where the atom's kind has no row there), the
+// one-center tasks, and the atom's slot in its molecule's partner order
+// (heavy atoms, then hydrogens).
+struct ListOffsets {
+  int64_t all, xx, hx, hh, diag;
+  int64_t partner;
+};
+
+// One wavefront per atom (row a of its molecule): the pairs (a, j < a) into
+// the all-pairs list and the list of their kind, in column order (a heavy
+// column j is the hv_before(j)-th heavy atom before a, a hydrogen column the
+// (j - hv_before(j))-th hydrogen), the atom's one-center tasks, and its copy
+// in the partner order of the Fock build.
+__global__ void __launch_bounds__(256) k_build_lists(int n_atoms, const MolDesc* __restrict__ mols,
+                                                     const AtomDesc* __restrict__ atoms,
+                                                     const ListOffsets* __restrict__ offs,
+                                                     int2* __restrict__ pall, int2* __restrict__ pxx,
+                                                     int2* __restrict__ phx, int2* __restrict__ phh,
+                                                     int2* __restrict__ diag,
+                                                     AtomDesc* __restrict__ partners) {
+  const int gi = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (gi >= n_atoms) return;
+  const AtomDesc A = atoms[gi];
+  const ListOffsets o = offs[gi];
+  const int atom0 = mols[A.mol].atom0;
+  const bool heavy = A.zn > 1;
+  for (int j = lane; j < A.local; j += 64) {
+    const int gj = atom0 + j;
+    const AtomDesc B = atoms[gj];
+    const int2 pr = make_int2(gi, gj);
+    pall[o.all + j] = pr;
+    const bool hj = B.zn > 1;
+    const int hb = j - B.hv_before;
+    if (heavy && hj) pxx[o.xx + B.hv_before] = pr;
+    else if (heavy) phx[o.hx + hb] = pr;
+    else if (hj) phx[o.hx + B.hv_before] = pr;
+    else phh[o.hh + hb] = pr;
+  }
+  if (lane < (heavy ? 10 : 1)) diag[o.diag + lane] = make_int2(gi, lane);
+  if (lane == 0) partners[o.partner] = A;
+}
 
 // Plain copy, 16 bytes per lane per step: reads and writes exactly n16 * 16 bytes.
 __global__ void k_calibration_copy(const double2* __restrict__ src, double2* __restrict__ dst,
@@ -61,6 +105,61 @@ __global__ void k_calibration_copy(const double2* __restrict__ src, double2* __r
        i += (int64_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
 }
+
+namespace sg {
+
+hipError_t ctx_alloc(sg_context* c, size_t bytes, void** out, size_t* got) {
+  int best = -1;
+  for (int i = 0; i < (int)c->cache.size(); ++i)
+    if (c->cache[i].second >= bytes && (best < 0 || c->cache[i].second < c->cache[best].second))
+      best = i;
+  if (best >= 0) {
+    *out = c->cache[best].first;
+    *got = c->cache[best].second;
+    c->cache.erase(c->cache.begin() + best);
+    return hipSuccess;
+  }
+  hipError_t e = hipMalloc(out, bytes);
+  if (e != hipSuccess && !c->cache.empty()) {  // give the cached blocks back and try once more
+    (void)hipGetLastError();
+    for (auto& blk : c->cache) (void)hipFree(blk.first);
+    c->cache.clear();
+    e = hipMalloc(out, bytes);
+  }
+  *got = e == hipSuccess ? bytes : 0;
+  return e;
+}
+
+void ctx_release(sg_context* c, void* p, size_t bytes) {
+  if (!p) return;
+  c->cache.push_back({p, bytes});
+  if ((int)c->cache.size() > kCacheBlocks) {  // drop the smallest
+    int small = 0;
+    for (int i = 1; i < (int)c->cache.size(); ++i)
+      if (c->cache[i].second < c->cache[small].second) small = i;
+    (void)hipFree(c->cache[small].first);
+    c->cache.erase(c->cache.begin() + small);
+  }
+}
+
+hipError_t ctx_staging(sg_context* c, size_t bytes, void** out) {
+  if (bytes > c->staging_bytes) {
+    const size_t want = std::max(bytes, c->staging_bytes * 2);
+    if (c->staging) (void)hipHostFree(c->staging);
+    c->staging = nullptr;
+    c->staging_bytes = 0;
+    hipError_t e = hipHostMalloc(&c->staging, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      c->staging = nullptr;
+      return e;
+    }
+    c->staging_bytes = want;
+  }
+  *out = c->staging;
+  return hipSuccess;
+}
+
+}  // namespace sg
 
 extern "C" {
 
@@ -91,6 +190,9 @@ int sg_ctx_destroy(sg_context* c) {
     (void)hipEventDestroy(p.second.first);
     (void)hipEventDestroy(p.second.second);
   }
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& blk : c->cache) (void)hipFree(blk.first);
+  if (c->staging) (void)hipHostFree(c->staging);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return SG_OK;
@@ -101,6 +203,12 @@ int sg_ctx_set_large_basis(sg_context* c, int n_min) {
   if (n_min < kMinLarge)
     return fail(SG_ERR_INVALID, "large-basis threshold below " + std::to_string(kMinLarge));
   c->large_min = n_min;
+  return SG_OK;
+}
+
+int sg_ctx_set_graphs(sg_context* c, int on) {
+  if (!c) return fail(SG_ERR_INVALID, "null context");
+  c->graphs = on != 0;
   return SG_OK;
 }
 
@@ -195,17 +303,25 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   b->mols.resize(n_mol);
   b->atoms.resize(n_atoms);
   b->blk_count.resize(n_mol);
-  std::vector<int2> pxx, phx, phh, pall, diag;
+  // Host work is O(atoms): descriptors and, per atom, where its row of each
+  // pair list starts; the lists themselves are written on the device
+  // (k_build_lists), in the order of a host loop over rows a and columns
+  // j < a of each molecule.
+  std::vector<ListOffsets> offs(n_atoms);
   int64_t mat0 = 0, vec0 = 0, blk0 = 0;
+  int64_t n_all = 0, n_xx = 0, n_hx = 0, n_hh = 0, n_diag = 0;
   int max_n = 0;
   for (int m = 0; m < n_mol; ++m) {
     MolDesc& M = b->mols[m];
     M.atom0 = atom_offsets[m];
     M.n_atoms = atom_offsets[m + 1] - atom_offsets[m];
-    int ao = 0, ne = 0, hv = 0;
-    int64_t row = 0;
+    int ao = 0, ne = 0, hv = 0, n_heavy = 0;
+    for (int a = 0; a < M.n_atoms; ++a) n_heavy += Z[M.atom0 + a] > 1;
+    int64_t row = 0, hx_run = 0;
     for (int a = 0; a < M.n_atoms; ++a) {
       const int gi = M.atom0 + a;
+      const bool heavy = Z[gi] > 1;
+      const int hb = a - hv;  // hydrogens before a
       AtomDesc& A = b->atoms[gi];
       A.zn = Z[gi];
       A.x = xyz_ang[3 * gi + 0] * to_bohr;
@@ -217,19 +333,26 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
       A.hv_before = hv;
       A.pad = 0;
       A.row0 = row;
-      for (int j = 0; j <= a; ++j) row += block_size(A.zn, Z[M.atom0 + j]);
+      // blocks (a, j <= a): 100 per heavy pair, 10 per mixed, 1 per H-H
+      row += heavy ? 100 * (int64_t)(hv + 1) + 10 * (int64_t)hb : 10 * (int64_t)hv + hb + 1;
+      ListOffsets& o = offs[gi];
+      o.all = n_all + (int64_t)a * (a - 1) / 2;
+      o.xx = heavy ? n_xx + (int64_t)hv * (hv - 1) / 2 : -1;
+      o.hh = heavy ? -1 : n_hh + (int64_t)hb * (hb - 1) / 2;
+      o.hx = n_hx + hx_run;
+      hx_run += heavy ? hb : hv;
+      o.diag = n_diag;
+      o.partner = M.atom0 + (heavy ? hv : n_heavy + hb);
+      n_diag += heavy ? 10 : 1;
       ao += n_ao_of(A.zn);
       ne += core_charge(A.zn);
-      if (A.zn > 1) ++hv;
-      for (int j = 0; j < a; ++j) {
-        const int gj = M.atom0 + j;
-        int2 pr = make_int2(gi, gj);
-        pall.push_back(pr);
-        const int bs = block_size(A.zn, Z[gj]);
-        (bs == 100 ? pxx : bs == 10 ? phx : phh).push_back(pr);
-      }
-      for (int s = 0; s < (A.zn > 1 ? 10 : 1); ++s) diag.push_back(make_int2(gi, s));
+      if (heavy) ++hv;
     }
+    const int64_t nh = M.n_atoms - n_heavy;
+    n_all += (int64_t)M.n_atoms * (M.n_atoms - 1) / 2;
+    n_xx += (int64_t)n_heavy * (n_heavy - 1) / 2;
+    n_hx += (int64_t)n_heavy * nh;
+    n_hh += nh * (nh - 1) / 2;
     M.n_ao = ao;
     M.n_occ = ne / 2;
     M.mat0 = mat0;
@@ -246,6 +369,10 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
     return fail(SG_ERR_INVALID, "molecules above " + std::to_string(kMaxLarge) +
                                     " basis functions are not supported");
   }
+  if (n_all > INT32_MAX) {
+    delete b;
+    return fail(SG_ERR_INVALID, "more than 2^31 - 1 atom pairs in one batch");
+  }
   BatchDev& d = b->d;
   d.n_mol = n_mol;
   d.n_atoms = n_atoms;
@@ -253,11 +380,11 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   d.n_vec = vec0;
   d.n_blocks = blk0;
   d.max_n = max_n;
-  d.n_xx = (int)pxx.size();
-  d.n_hx = (int)phx.size();
-  d.n_hh = (int)phh.size();
-  d.n_pairs_all = (int)pall.size();
-  d.n_diag_tasks = (int)diag.size();
+  d.n_xx = (int)n_xx;
+  d.n_hx = (int)n_hx;
+  d.n_hh = (int)n_hh;
+  d.n_pairs_all = (int)n_all;
+  d.n_diag_tasks = (int)n_diag;
   d.large_min = std::min(ctx->large_min, kMaxBasis + 1);
 
   // one allocation, 256-byte aligned sub-buffers
@@ -269,9 +396,9 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   };
   const size_t o_mols = take(sizeof(MolDesc) * n_mol), o_atoms = take(sizeof(AtomDesc) * n_atoms),
                o_partners = take(sizeof(AtomDesc) * n_atoms);
-  const size_t o_xx = take(sizeof(int2) * pxx.size()), o_hx = take(sizeof(int2) * phx.size()),
-               o_hh = take(sizeof(int2) * phh.size()), o_all = take(sizeof(int2) * pall.size()),
-               o_diag = take(sizeof(int2) * diag.size());
+  const size_t o_xx = take(sizeof(int2) * n_xx), o_hx = take(sizeof(int2) * n_hx),
+               o_hh = take(sizeof(int2) * n_hh), o_all = take(sizeof(int2) * n_all),
+               o_diag = take(sizeof(int2) * n_diag), o_offs = take(sizeof(ListOffsets) * n_atoms);
   const size_t o_blk = take(sizeof(double) * blk0), o_blk32 = take(sizeof(float) * blk0);
   const size_t o_H = take(sizeof(double) * mat0), o_F = take(sizeof(double) * mat0),
                o_P = take(sizeof(double) * mat0), o_C = take(sizeof(double) * mat0),
@@ -288,7 +415,7 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
                o_s32 = take(sizeof(int) * n_mol), o_dep = take(sizeof(double) * n_mol),
                o_wdiag = take(sizeof(double) * 10 * (size_t)n_atoms);
   (void)hipSetDevice(ctx->device);
-  hipError_t e = hipMalloc(&b->dev_mem, off);
+  hipError_t e = ctx_alloc(ctx, off, &b->dev_mem, &b->dev_bytes);
   if (e != hipSuccess) {
     delete b;
     return fail(SG_ERR_HIP, std::string("hipMalloc(") + std::to_string(off) + "): " +
@@ -332,32 +459,37 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   int* order = (int*)(base + o_order);
   d.order = order;
   hipStream_t s = ctx->stream;
-  auto up = [&](void* dst, const void* src, size_t bytes) {
-    return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
-  };
-  e = up(d.mols, b->mols.data(), sizeof(MolDesc) * n_mol);
-  if (e == hipSuccess) e = up(d.atoms, b->atoms.data(), sizeof(AtomDesc) * n_atoms);
-  // partner order of the Fock build: each molecule's heavy atoms, then its hydrogens
-  std::vector<AtomDesc> partners;
-  partners.reserve(n_atoms);
-  for (int m = 0; m < n_mol; ++m) {
-    const MolDesc& M = b->mols[m];
-    for (int pass = 0; pass < 2; ++pass)
-      for (int a = 0; a < M.n_atoms; ++a)
-        if ((b->atoms[M.atom0 + a].zn > 1) == (pass == 0)) partners.push_back(b->atoms[M.atom0 + a]);
-  }
-  if (e == hipSuccess) e = up(d.partners, partners.data(), sizeof(AtomDesc) * n_atoms);
-  if (e == hipSuccess) e = up(d.pairs_xx, pxx.data(), sizeof(int2) * pxx.size());
-  if (e == hipSuccess) e = up(d.pairs_hx, phx.data(), sizeof(int2) * phx.size());
-  if (e == hipSuccess) e = up(d.pairs_hh, phh.data(), sizeof(int2) * phh.size());
-  if (e == hipSuccess) e = up(d.pairs_all, pall.data(), sizeof(int2) * pall.size());
-  if (e == hipSuccess) e = up(d.diag_tasks, diag.data(), sizeof(int2) * diag.size());
   // largest molecules first in the per-molecule launches
   std::vector<int> ord(n_mol);
   for (int m = 0; m < n_mol; ++m) ord[m] = m;
   std::stable_sort(ord.begin(), ord.end(),
                    [&](int x, int y) { return b->mols[x].n_ao > b->mols[y].n_ao; });
-  if (e == hipSuccess) e = up(order, ord.data(), sizeof(int) * n_mol);
+  // descriptors, list offsets and order through the pinned staging buffer:
+  // one host copy, then one DMA per array
+  const size_t sz_mols = sizeof(MolDesc) * n_mol, sz_atoms = sizeof(AtomDesc) * n_atoms,
+               sz_offs = sizeof(ListOffsets) * n_atoms, sz_ord = sizeof(int) * n_mol;
+  void* stage = nullptr;
+  e = ctx_staging(ctx, sz_mols + sz_atoms + sz_offs + sz_ord, &stage);
+  if (e == hipSuccess) {
+    char* h = (char*)stage;
+    memcpy(h, b->mols.data(), sz_mols);
+    memcpy(h + sz_mols, b->atoms.data(), sz_atoms);
+    memcpy(h + sz_mols + sz_atoms, offs.data(), sz_offs);
+    memcpy(h + sz_mols + sz_atoms + sz_offs, ord.data(), sz_ord);
+    e = hipMemcpyAsync(d.mols, h, sz_mols, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d.atoms, h + sz_mols, sz_atoms, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(base + o_offs, h + sz_mols + sz_atoms, sz_offs, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(order, h + sz_mols + sz_atoms + sz_offs, sz_ord, hipMemcpyHostToDevice, s);
+  }
+  if (e == hipSuccess) {
+    k_build_lists<<<(n_atoms + 3) / 4, 256, 0, s>>>(n_atoms, d.mols, d.atoms,
+                                                    (const ListOffsets*)(base + o_offs), d.pairs_all,
+                                                    d.pairs_xx, d.pairs_hx, d.pairs_hh,
+                                                    d.diag_tasks, d.partners);
+    e = hipGetLastError();
+  }
   // the molecules at or above large_min lead the order
   for (int i = 0; i < n_mol; ++i) {
     const MolDesc& M = b->mols[ord[i]];
@@ -380,7 +512,7 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   if (e == hipSuccess) e = hipMemsetAsync(d.prev_prec, 0, sizeof(int) * n_mol, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
-    (void)hipFree(b->dev_mem);
+    ctx_release(ctx, b->dev_mem, b->dev_bytes);
     delete b;
     return fail(SG_ERR_HIP, std::string("batch upload: ") + hipGetErrorString(e));
   }
@@ -394,7 +526,7 @@ int sg_batch_destroy(sg_batch* b) {
   (void)hipSetDevice(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
   if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
-  if (b->dev_mem) (void)hipFree(b->dev_mem);
+  if (b->dev_mem) ctx_release(b->ctx, b->dev_mem, b->dev_bytes);
   if (b->diis_mem) (void)hipFree(b->diis_mem);
   delete b;
   return SG_OK;
@@ -577,9 +709,7 @@ int sg_scf_fp32_steps(const sg_batch* b, int* steps32) {
 
 int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, double switch_ev,
                       sg_scf_result* res) {
-  if (!b || max_steps < 0 || !(threshold >= 0.0) || (policy != SG_POLICY_FP64 &&
-                                                     policy != SG_POLICY_MIXED) ||
-      !(switch_ev >= 0.0))
+  if (!b || max_steps < 0 || !(threshold >= 0.0) || !valid_policy(policy) || !(switch_ev >= 0.0))
     return fail(SG_ERR_INVALID, "sg_scf_run: bad arguments");
   int r;
   if (!b->have_ints && (r = sg_two_center(b))) return r;
@@ -588,7 +718,7 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
   sg_context* c = b->ctx;
   hipStream_t s = c->stream;
   HIP_TRY(hipSetDevice(c->device));
-  if (policy == SG_POLICY_MIXED && !b->have_fp32) {  // integrals set by the stage API only
+  if (policy != SG_POLICY_FP64 && !b->have_fp32) {  // integrals set by the stage API only
     StageTimer t(c, ST_TWO_CENTER);
     launch_blocks_to_fp32(b->d, s);
     b->have_fp32 = true;
@@ -608,7 +738,7 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
   }
   {
     StageTimer t(c, ST_CONTROL);
-    launch_set_prec(b->d, s, policy == SG_POLICY_MIXED ? 1 : 0);
+    launch_set_prec(b->d, s, reduced_prec(policy));
     launch_scf_init(b->d, s, max_steps, policy);
     launch_diis_reset(b->d, s);
   }
@@ -620,10 +750,11 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
   // replayed; its kernel arguments depend on the batch and on these settings.
   // Batches with molecules on the multi-workgroup eigensolver (thousands of
   // launches per iteration) launch directly.
-  b->d.mixed = policy == SG_POLICY_MIXED ? 1 : 0;
-  const bool use_graph = !c->timing && b->d.n_large == 0;
+  b->d.mixed = policy == SG_POLICY_FP64 ? 0 : 1;
+  b->d.eig32 = policy == SG_POLICY_MIXED_EIG32 ? 1 : 0;
+  const bool use_graph = c->graphs && !c->timing && b->d.n_large == 0;
   if (use_graph && (!b->iter_exec || b->g_threshold != threshold || b->g_max_steps != max_steps ||
-                    b->g_switch != switch_ev || b->g_mixed != b->d.mixed ||
+                    b->g_switch != switch_ev || b->g_mixed != policy ||
                     b->g_damping != b->d.damping)) {
     if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
     b->iter_exec = nullptr;
@@ -644,7 +775,7 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
     b->g_threshold = threshold;
     b->g_max_steps = max_steps;
     b->g_switch = switch_ev;
-    b->g_mixed = b->d.mixed;
+    b->g_mixed = policy;
     b->g_damping = b->d.damping;
   }
   for (int it = 0; it < max_steps && n_active > 0; ++it) {

@@ -25,6 +25,8 @@ namespace sg {
 
 constexpr int kEnergyChunk = 16384;  // entries of a large molecule's E_el per workgroup
 constexpr double kFp32Floor = 9.5367431640625e-07;  // 2^-20 = 8 ulps of FP32 at 1
+// The same for the split-BF16 density (about 16 bits per product): 2^-14
+constexpr double kBf16Floor = 6.103515625e-05;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -386,13 +388,15 @@ __global__ void k_scf_update(int n_mol, double threshold, int max_steps, double 
   if (steps32 && step_prec) steps32[m] += 1;
   const int conv = step_prec == 0 && prev_prec[m] == 0 && de < threshold;
   prev_prec[m] = step_prec;
-  // FP32 steps end at |dE| < switch_ev, or once |dE| is below what FP32
-  // arithmetic resolves on this energy (kFp32Floor |E_el|, 8 FP32 ulps), or
+  // Reduced steps end at |dE| < switch_ev, or once |dE| is below what their
+  // arithmetic resolves on this energy (kFp32Floor |E_el|, 8 FP32 ulps;
+  // kBf16Floor |E_el| for the split-BF16 density), or
   // stops falling by half per step (FP32 rounding, or a slow SCF, sets the
   // pace), or after fp32_cap steps
   const double de_last = de_prev[m];
   de_prev[m] = de;
-  if (step_prec && (de < switch_ev || de < kFp32Floor * fabs(e) || de > 0.5 * de_last ||
+  const double floor = step_prec == 2 ? kBf16Floor : kFp32Floor;
+  if (step_prec && (de < switch_ev || de < floor * fabs(e) || de > 0.5 * de_last ||
                     st >= fp32_cap))
     prec[m] = 0;
   converged[m] = conv;

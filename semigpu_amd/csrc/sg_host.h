@@ -29,6 +29,16 @@ static_assert(ST_TRIDIAG32 + 1 == SG_N_STAGES, "stage list and SG_N_STAGES disag
 int check_launch(const char* what);
 bool supported_element(int z);
 
+inline bool valid_policy(int p) {
+  return p == SG_POLICY_FP64 || p == SG_POLICY_MIXED || p == SG_POLICY_MIXED_BF16 ||
+         p == SG_POLICY_MIXED_EIG32;
+}
+// Precision of a molecule's first SCF steps under a policy (the prec flags):
+// 0 FP64, 1 FP32 Fock and density, 2 FP32 Fock and split-BF16 density.
+inline int reduced_prec(int policy) {
+  return policy == SG_POLICY_FP64 ? 0 : policy == SG_POLICY_MIXED_BF16 ? 2 : 1;
+}
+
 }  // namespace sg
 
 struct sg_context {
@@ -36,10 +46,28 @@ struct sg_context {
   hipStream_t stream = nullptr;
   bool timing = false;
   int large_min = sg::kMaxBasis + 1;  // batches created from now on
+  bool graphs = true;                 // capture SCF iterations as HIP graphs
   double ms[SG_N_STAGES] = {0};
   int counts[SG_N_STAGES] = {0};
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  // Device blocks of destroyed batches, kept for the next batches (a batch
+  // takes the smallest one that fits; at most kCacheBlocks are kept), and a
+  // pinned host buffer through which batch descriptors are uploaded.
+  std::vector<std::pair<void*, size_t>> cache;
+  void* staging = nullptr;
+  size_t staging_bytes = 0;
 };
+
+namespace sg {
+constexpr int kCacheBlocks = 4;
+// A device block of at least `bytes` for a batch: from the context's cache
+// when one fits, else hipMalloc.  *got is the block's size.
+hipError_t ctx_alloc(sg_context* c, size_t bytes, void** out, size_t* got);
+// Returns a batch's block to the context's cache.
+void ctx_release(sg_context* c, void* p, size_t bytes);
+// The context's pinned staging buffer, grown to at least `bytes`.
+hipError_t ctx_staging(sg_context* c, size_t bytes, void** out);
+}  // namespace sg
 
 namespace sg {
 

@@ -68,9 +68,14 @@ struct BatchDev {
   // per atom, 10 entries: the Coulomb weights w(l) of its diagonal density
   // block (P_l, doubled off the diagonal), gathered before each Fock build
   double* wdiag = nullptr;
-  // The SCF runs the mixed-precision policy: only then can a step's molecules
-  // be in FP32, and the eigensolver launches its FP32 kernels.
+  // The SCF runs a mixed-precision policy: only then can a step's molecules
+  // be at reduced precision (prec 1: FP32 Fock and density; 2: FP32 Fock and
+  // split-BF16 density).
   int mixed = 0;
+  // The reduced steps also run the one-workgroup eigensolver's reduction and
+  // back-transform in FP32 (SG_POLICY_MIXED_EIG32): only then do the
+  // eigensolver launches take the step precision.
+  int eig32 = 0;
   // Density damping of the SCF (opt-in, 0 = off): each step's density is
   // beta P_old + (1 - beta) P_new, with P_old kept in C meanwhile.
   double damping = 0.0;

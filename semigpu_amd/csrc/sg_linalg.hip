@@ -1273,6 +1273,42 @@ __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Stores a wave's 32 x 32 block of P from its 2 x 2 accumulator tiles (result
+// layout of the 16 x 16 instruction of type T): rows of the block straight
+// out; for a block below the diagonal also its transpose, through LDS so that
+// both stores run along rows.
+template <typename T>
+__device__ __forceinline__ void density_store(const typename Mfma16<T>::v4 (&acc)[2][2], int n,
+                                              int lo, int i0, int j0, int lane,
+                                              double* __restrict__ p, double* __restrict__ tr) {
+  const int lr = lane & 15;
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ii = 16 * u + Mfma16<T>::row(lane, r), jj = 16 * v + lr;
+        const int i = i0 + ii, j = j0 + jj;
+        const double s = (double)acc[u][v][r];
+        const double val = lo == 0 ? 2.0 * s : 2.0 * ((i == j ? 1.0 : 0.0) - s);
+        if (i < n && j < n) p[(int64_t)i * n + j] = val;
+        if (i0 != j0) tr[ii * 33 + jj] = val;
+      }
+  if (i0 != j0) {
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    for (int t = lane; t < 1024; t += 64) {
+      const int jj = t >> 5, ii = t & 31;
+      const int i = i0 + ii, j = j0 + jj;
+      if (i < n && j < n) p[(int64_t)j * n + i] = tr[ii * 33 + jj];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+
 // P from the selected vectors Y (N x k, row-major): 2 Y Y^T (occupied side)
 // or 2 (I - Y Y^T) (virtual side), on the matrix cores with
 // v_mfma_f64_16x16x4f64 (FP64 steps) or v_mfma_f32_16x16x4f32 (FP32 steps of
@@ -1317,32 +1353,77 @@ __device__ __forceinline__ void density_block(const YT* __restrict__ Y, int n, i
 #pragma unroll
         for (int v = 0; v < 2; ++v) acc[u][v] = Mfma16<T>::op(a[st][u], b[st][v], acc[u][v]);
   }
-  // rows of the block straight out; for a block below the diagonal also its
-  // transpose, through LDS so that both stores run along rows
+  density_store<T>(acc, n, lo, i0, j0, lane, p, tr);
+}
+
+// Split BF16 (the density product of SG_POLICY_MIXED_BF16): each FP64 entry y
+// of Y becomes y ~ hi + lo with hi = bf16(y) and lo = bf16(y - hi), both
+// rounded to nearest even from the FP32 value of y, and a tile of Y Y^T
+// accumulates hi hi^T + hi lo^T + lo hi^T in FP32 with
+// v_mfma_f32_16x16x16bf16_1k (operand lane l: row l % 16, k = 4 (l / 16) + j
+// for its four entries j; results in the layout of the f32 16x16x4 form).
+// The dropped lo lo^T term and the BF16 rounding of lo leave a relative
+// error of about 2^-16 per product.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
+  unsigned int u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_value(unsigned short h) {
+  return __uint_as_float((unsigned int)h << 16);
+}
+__device__ __forceinline__ void split_bf16(double y, short& hi, short& lo) {
+  const float f = (float)y;
+  const unsigned short h = bf16_rne(f);
+  hi = (short)h;
+  lo = (short)bf16_rne(f - bf16_value(h));
+}
+
+__device__ __forceinline__ void density_block_bf16(const double* __restrict__ Y, int n, int k,
+                                                   int lo, int i0, int j0, int lane,
+                                                   double* __restrict__ p,
+                                                   double* __restrict__ tr) {
+  typedef Mfma16<float>::v4 v4;
+  v4 acc[2][2] = {{{0, 0, 0, 0}, {0, 0, 0, 0}}, {{0, 0, 0, 0}, {0, 0, 0, 0}}};
+  const int lr = lane & 15, kk = lane >> 4;
+  const double* ya[2];
+  const double* yb[2];
+  bool oka[2], okb[2];
 #pragma unroll
-  for (int u = 0; u < 2; ++u)
-#pragma unroll
-    for (int v = 0; v < 2; ++v)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int ii = 16 * u + Mfma16<T>::row(lane, r), jj = 16 * v + lr;
-        const int i = i0 + ii, j = j0 + jj;
-        const double s = (double)acc[u][v][r];
-        const double val = lo == 0 ? 2.0 * s : 2.0 * ((i == j ? 1.0 : 0.0) - s);
-        if (i < n && j < n) p[(int64_t)i * n + j] = val;
-        if (i0 != j0) tr[ii * 33 + jj] = val;
-      }
-  if (i0 != j0) {
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    for (int t = lane; t < 1024; t += 64) {
-      const int jj = t >> 5, ii = t & 31;
-      const int i = i0 + ii, j = j0 + jj;
-      if (i < n && j < n) p[(int64_t)j * n + i] = tr[ii * 33 + jj];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+  for (int h = 0; h < 2; ++h) {
+    const int ra = i0 + 16 * h + lr, rb = j0 + 16 * h + lr;
+    oka[h] = ra < n;
+    okb[h] = rb < n;
+    ya[h] = Y + (int64_t)(oka[h] ? ra : 0) * k;
+    yb[h] = Y + (int64_t)(okb[h] ? rb : 0) * k;
   }
+  for (int c0 = 0; c0 < k; c0 += 16) {
+    bf16x4 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int cc = c0 + 4 * kk + j;
+        const bool okc = cc < k;
+        short x, y;
+        split_bf16((oka[h] && okc) ? ya[h][cc] : 0.0, x, y);
+        ah[h][j] = x;
+        al[h][j] = y;
+        split_bf16((okb[h] && okc) ? yb[h][cc] : 0.0, x, y);
+        bh[h][j] = x;
+        bl[h][j] = y;
+      }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int v = 0; v < 2; ++v) {
+        acc[u][v] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[u], bl[v], acc[u][v], 0, 0, 0);
+        acc[u][v] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[u], bh[v], acc[u][v], 0, 0, 0);
+        acc[u][v] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[u], bh[v], acc[u][v], 0, 0, 0);
+      }
+  }
+  density_store<float>(acc, n, lo, i0, j0, lane, p, tr);
 }
 
 __global__ void __launch_bounds__(256) k_density(const MolDesc* __restrict__ mols,
@@ -1363,14 +1444,17 @@ __global__ void __launch_bounds__(256) k_density(const MolDesc* __restrict__ mol
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int t32 = (n + 31) >> 5;
   const int nblk = t32 * (t32 + 1) / 2;  // lower-triangle 32 x 32 blocks, row order
-  const bool fp32 = prec && prec[m];
+  const int pm = prec ? prec[m] : 0;  // 0 FP64, 1 FP32, 2 split BF16
+  const bool fp32 = pm == 1;
   // FP32 vectors of the one-workgroup back-transform (float_y), at 2 N^2
   const float* Yf = (const float*)(work + 5 * M.mat0 + 2 * (int64_t)n * n);
   __shared__ double trans[4][32 * 33];  // per-wave transpose buffer
   for (int blk = blockIdx.x * 4 + wave; blk < nblk; blk += gridDim.x * 4) {
     int bi = 0, bj = blk;
     while (bj > bi) bj -= ++bi;
-    if (fp32 && float_y)
+    if (pm == 2)
+      density_block_bf16(Y, n, k, lo, 32 * bi, 32 * bj, lane, p, trans[wave]);
+    else if (fp32 && float_y)
       density_block<float>(Yf, n, k, lo, 32 * bi, 32 * bj, lane, p, trans[wave]);
     else if (fp32)
       density_block<float>(Y, n, k, lo, 32 * bi, 32 * bj, lane, p, trans[wave]);
@@ -1455,7 +1539,7 @@ struct TridiagPlan {
 TridiagPlan tridiag_plan(const BatchDev& b, bool all) {
   TridiagPlan t;
   t.p = small_part(b);
-  t.prec = (all || !b.mixed) ? nullptr : b.prec;
+  t.prec = (all || !b.eig32) ? nullptr : b.prec;
   t.small = t.p.max_n <= 384;
   t.both = t.prec && t.p.count <= (t.small ? 2 : 1) * kDeviceCUs;
   return t;
@@ -1507,7 +1591,7 @@ void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all) {
   const Part p = small_part(b);
   // FP32 steps of the one-workgroup path bisect to the accuracy of their
   // FP32-reduced T; the multi-workgroup path reduces in FP64 throughout
-  const int* prec = (all || !b.mixed) ? nullptr : b.prec;
+  const int* prec = (all || !b.eig32) ? nullptr : b.prec;
   if (p.count > 0) bisect_part(b, s, all, p.order, p.count, p.max_n, prec);
   if (b.n_large > 0) bisect_part(b, s, all, b.order, b.n_large, b.max_n, nullptr);
 }
@@ -1523,7 +1607,7 @@ void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all) {
     const int threads = min(kT, max(64, (kmax + 63) / 64 * 64));
     k_invit<<<p.count, threads, inv_lds, s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm, b.W, all ? 1 : 0,
                                               act, b.status, p.order,
-                                              (all || !b.mixed) ? nullptr : b.prec);
+                                              (all || !b.eig32) ? nullptr : b.prec);
   }
   for (int i = 0; i < b.n_large; ++i) {
     const int m = b.large_idx[i];
@@ -1554,7 +1638,7 @@ void launch_eigh_back(const BatchDev& b, hipStream_t s, bool all) {
       bytes = std::max(bytes, std::max(b64, b32));
     }
     k_backtransform<<<p.count, kT, bytes, s>>>(b.mols, b.W, b.tau, all ? 1 : 0, act, b.status,
-                                               p.order, all ? nullptr : b.prec);
+                                               p.order, (all || !b.eig32) ? nullptr : b.prec);
   }
   for (int i = 0; i < b.n_large; ++i) launch_back_large(b, s, b.large_idx[i], b.large_mols[i], act, all);
   if (all) {
@@ -1584,7 +1668,7 @@ void launch_density(const BatchDev& b, hipStream_t s, bool all) {
     k_density_from_c<<<grid, 256, 0, s>>>(b.mols, b.C, b.P, 64);
   } else {
     const Part p = small_part(b);
-    if (p.count > 0) density_part(b, s, p.order, p.count, p.max_n, 1);
+    if (p.count > 0) density_part(b, s, p.order, p.count, p.max_n, b.eig32);
     if (b.n_large > 0) density_part(b, s, b.order, b.n_large, b.max_n, 0);
   }
 }

@@ -493,7 +493,7 @@ int lane_launch(sg_server* S, Lane* L) {
     }
     {
       StageTimer t(c, ST_CONTROL, s);
-      launch_set_prec(nb, s, S->policy == SG_POLICY_MIXED ? 1 : 0, L->mask_new);
+      launch_set_prec(nb, s, reduced_prec(S->policy), L->mask_new);
       launch_scf_init(nb, s, S->max_steps, S->policy, L->mask_new);
       launch_diis_reset(nb, s, L->mask_new);
     }
@@ -518,7 +518,7 @@ int server_create(sg_context* ctx, double hbm_bytes, int max_slots, int max_atom
                   double threshold_ev, int max_steps, int policy, double switch_ev,
                   int n_classes, const int* bounds, sg_server** out) {
   if (!ctx || !out || max_slots <= 0 || max_atoms <= 0 || !(hbm_bytes > 0) || max_steps < 0 ||
-      !(threshold_ev >= 0.0) || (policy != SG_POLICY_FP64 && policy != SG_POLICY_MIXED) ||
+      !(threshold_ev >= 0.0) || !valid_policy(policy) ||
       n_classes < 1 || n_classes > 16 || (n_classes > 1 && !bounds) || max_slots < n_classes ||
       max_atoms < n_classes)
     return fail(SG_ERR_INVALID, "sg_server_create: bad arguments");
@@ -569,7 +569,8 @@ int server_create(sg_context* ctx, double hbm_bytes, int max_slots, int max_atom
   }
   char* base = (char*)S->pool;
   BatchDev& d = S->shared;
-  d.mixed = policy == SG_POLICY_MIXED ? 1 : 0;
+  d.mixed = policy == SG_POLICY_FP64 ? 0 : 1;
+  d.eig32 = policy == SG_POLICY_MIXED_EIG32 ? 1 : 0;
   d.blocks = (double*)(base + o_blk);
   d.blocks32 = (float*)(base + o_blk32);
   d.H = (double*)(base + o_H);

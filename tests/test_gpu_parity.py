@@ -5,7 +5,7 @@ Bars (also stated in DESIGN.md):
   Hcore, Fock            |gpu - ref| <= 1e-10 * max|ref|  (normwise relative)
   energies               |gpu - ref| <= 1e-6 eV at SCF threshold 1e-6 eV, same step count
   batch vs alone         bit-identical (no floating-point atomics, fixed reduction order;
-                         mixed-policy batches above 512 molecules: the SCF threshold)
+                         mixed_eig32 batches above 512 molecules: the SCF threshold)
 Reference values are the committed fixtures written by the compiled reference
 (tests/golden); sizes beyond the fixtures are checked against the oracle.
 """
@@ -149,24 +149,30 @@ def test_step_cap_reports_not_converged(engine):
     assert r0["steps"] == 0 and not r0["converged"]
 
 
-def test_mixed_policy_within_1e4(engine):
-    """FP32 contractions until |dE| < 1e-3 eV, then FP64: |E - E_ref| <= 1e-4 eV."""
+MIXED = ["mixed", "mixed_bf16", "mixed_eig32"]
+
+
+@pytest.mark.parametrize("policy", MIXED)
+def test_mixed_policy_within_1e4(engine, policy):
+    """Reduced-precision steps until |dE| < 1e-3 eV, then FP64:
+    |E - E_ref| <= 1e-4 eV on every fixture."""
     names = FULL + LARGE
     mols = [atoms_of(load_golden(n)) for n in names]
-    res = engine.batch(mols).scf(threshold=1e-6, max_steps=100, policy="mixed")
+    res = engine.batch(mols).scf(threshold=1e-6, max_steps=100, policy=policy)
     for n, r in zip(names, res):
         g = load_golden(n)
-        say("mixed", n, r["steps"], r["e_tot"] - float(g["e_tot"]))
+        say(policy, n, r["steps"], r["e_tot"] - float(g["e_tot"]))
         assert r["converged"], n
         assert abs(r["e_tot"] - float(g["e_tot"])) <= 1e-4, n
 
 
-def test_mixed_policy_water_batch(engine):
+@pytest.mark.parametrize("policy", MIXED)
+def test_mixed_policy_water_batch(engine, policy):
     mols = geometry.water_batch(64, seed=11)
     b = engine.batch(mols)
     ref = b.scf(threshold=1e-6, policy="fp64")
     assert not b.fp32_steps().any()
-    mix = b.scf(threshold=1e-6, policy="mixed")
+    mix = b.scf(threshold=1e-6, policy=policy)
     s32 = b.fp32_steps()
     worst = max(abs(a["e_tot"] - c["e_tot"]) for a, c in zip(ref, mix))
     say("mixed vs fp64 worst |dE|", worst, "FP32 steps", s32.min(), s32.max())
@@ -176,42 +182,48 @@ def test_mixed_policy_water_batch(engine):
     assert all(1 <= n32 <= r["steps"] - 2 for n32, r in zip(s32, mix))
 
 
-def test_mixed_policy_large_batch_vs_alone(engine):
-    """Above 512 molecules each SCF step reduces its FP32 and FP64 molecules
-    in separate launches, the FP32 one with four wavefronts per molecule;
-    alone, one launch takes both types with eight.  The FP32 steps then sum in
-    another order, so the results agree to the SCF threshold, not bit for bit
-    (FP64 steps and the FP64 policy are the same in both)."""
+@pytest.mark.parametrize("policy", MIXED)
+def test_mixed_policy_large_batch_vs_alone(engine, policy):
+    """A 600-molecule batch against its molecules alone.  With the FP64
+    eigensolver (mixed, mixed_bf16) every kernel is per molecule in a fixed
+    order: equal bit for bit.  mixed_eig32 reduces its FP32 and FP64
+    molecules in separate launches above 512 molecules, the FP32 one with four
+    wavefronts per molecule; alone, one launch takes both types with eight.
+    The FP32 steps then sum in another order, so the results agree to the SCF
+    threshold, not bit for bit."""
     mols = geometry.water_batch(600, seed=5, n_min=2, n_max=10)
     b = engine.batch(mols)
-    big = b.scf(threshold=1e-6, policy="mixed")
+    big = b.scf(threshold=1e-6, policy=policy)
     assert all(r["converged"] for r in big)
     ref = b.scf(threshold=1e-6, policy="fp64")
     for i in (0, 1, 299, 599):
-        alone = engine.batch([mols[i]]).scf(threshold=1e-6, policy="mixed")[0]
-        say("mixed alone", i, alone["steps"], alone["e_tot"] - big[i]["e_tot"],
+        alone = engine.batch([mols[i]]).scf(threshold=1e-6, policy=policy)[0]
+        say(policy, "alone", i, alone["steps"], alone["e_tot"] - big[i]["e_tot"],
             alone["e_tot"] - ref[i]["e_tot"])
         assert alone["converged"]
-        assert abs(alone["e_tot"] - big[i]["e_tot"]) <= 1e-6, i
+        if policy == "mixed_eig32":
+            assert abs(alone["e_tot"] - big[i]["e_tot"]) <= 1e-6, i
+        else:
+            assert alone == big[i], i
         assert abs(alone["e_tot"] - ref[i]["e_tot"]) <= 1e-4, i
     b.close()
 
 
 def test_mixed_policy_split_launches_above_384(engine):
-    """The FP32 reduction for N > 384 (eight wavefronts, one workgroup per CU):
+    """mixed_eig32: the FP32 reduction for N > 384 (eight wavefronts, one workgroup per CU):
     300 molecules, one with N = 420, take separate FP32 and FP64 launches;
     alone, the N = 420 molecule takes the combined one.  Both within the SCF
     threshold of each other and within 1e-4 eV of FP64."""
     mols = geometry.water_batch(299, seed=8, n_min=2, n_max=6)
     mols.insert(150, geometry.water_cluster(70, geometry.SplitMix64(70)))  # N = 420
     b = engine.batch(mols)
-    mix = b.scf(threshold=1e-6, policy="mixed")
+    mix = b.scf(threshold=1e-6, policy="mixed_eig32")
     s32 = b.fp32_steps()
     ref = b.scf(threshold=1e-6, policy="fp64")
     assert all(r["converged"] for r in mix)
     assert max(abs(a["e_tot"] - c["e_tot"]) for a, c in zip(ref, mix)) <= 1e-4
     assert s32[150] >= 1
-    alone = engine.batch([mols[150]]).scf(threshold=1e-6, policy="mixed")[0]
+    alone = engine.batch([mols[150]]).scf(threshold=1e-6, policy="mixed_eig32")[0]
     say("N=420 mixed batch", mix[150]["steps"], "alone", alone["steps"],
         alone["e_tot"] - mix[150]["e_tot"], "vs fp64", mix[150]["e_tot"] - ref[150]["e_tot"])
     assert alone["converged"] and abs(alone["e_tot"] - mix[150]["e_tot"]) <= 1e-6
@@ -219,15 +231,15 @@ def test_mixed_policy_split_launches_above_384(engine):
 
 
 def test_fp32_eigensolver_step_density(engine):
-    """The first SCF step of the mixed policy diagonalises the FP64 guess Fock
-    matrix on the FP32 path (reduction, bisection to 1e-10 ||T||, one solve,
+    """The first SCF step of the mixed_eig32 policy diagonalises the FP64 guess
+    Fock matrix on the FP32 path (reduction, bisection to 1e-10 ||T||, one solve,
     back-transform and density in FP32).  Its density is held to the FP64
     step's: |dP| <= 1e-4, trace 2 n_occ to 1e-4, and P P = 2 P to 1e-4."""
     mols = [geometry.water_cluster(64, geometry.SplitMix64(4)),   # N = 384
             geometry.water_cluster(100, geometry.SplitMix64(5)),  # N = 600 (eight wavefronts)
             geometry.alkane(12), geometry.pyridine()]
     b = engine.batch(mols)
-    b.scf(threshold=1e-6, max_steps=1, policy="mixed")
+    b.scf(threshold=1e-6, max_steps=1, policy="mixed_eig32")
     assert all(n32 == 1 for n32 in b.fp32_steps())
     p32 = [b.packed(i, api.SG_DENSITY) for i in range(len(mols))]
     b.scf(threshold=1e-6, max_steps=1, policy="fp64")
@@ -244,6 +256,39 @@ def test_fp32_eigensolver_step_density(engine):
         assert dp <= 1e-4
         assert abs(np.trace(P) - 2 * b.info[i][2]) <= 1e-4
         assert idem <= 1e-4
+    b.close()
+
+
+@pytest.mark.parametrize("policy,bar", [("mixed", 1e-5), ("mixed_bf16", 5e-4)])
+def test_reduced_density_product_first_step(engine, policy, bar):
+    """The first SCF step of the mixed and mixed_bf16 policies diagonalises
+    the FP64 guess Fock matrix with the FP64 eigensolver and forms the density
+    product in FP32, or as split BF16 (hi hi + hi lo + lo hi, FP32
+    accumulation), on the matrix cores.  Its density against the FP64 step's:
+    |dP| within the bar (FP32 rounding: about 1e-7 per product; split BF16
+    about 2^-16), the trace 2 n_occ and P P = 2 P to the same bar."""
+    mols = [geometry.water_cluster(64, geometry.SplitMix64(4)),   # N = 384
+            geometry.water_cluster(100, geometry.SplitMix64(5)),  # N = 600
+            geometry.alkane(12), geometry.pyridine()]
+    b = engine.batch(mols)
+    b.scf(threshold=1e-6, max_steps=1, policy=policy)
+    assert all(n32 == 1 for n32 in b.fp32_steps())
+    pr = [b.packed(i, api.SG_DENSITY) for i in range(len(mols))]
+    b.scf(threshold=1e-6, max_steps=1, policy="fp64")
+    for i, m in enumerate(mols):
+        n = b.n_ao(i)
+        p64 = b.packed(i, api.SG_DENSITY)
+        P = np.zeros((n, n))
+        r, c = np.tril_indices(n)
+        P[r, c] = pr[i]
+        P[c, r] = pr[i]
+        dp = np.abs(pr[i] - p64).max()
+        idem = np.abs(P @ P - 2.0 * P).max()
+        say(policy, "first step N", n, "dP", dp, "idempotency", idem,
+            "trace", np.trace(P) - 2 * b.info[i][2])
+        assert dp <= bar
+        assert abs(np.trace(P) - 2 * b.info[i][2]) <= bar * n
+        assert idem <= 4 * bar
     b.close()
 
 

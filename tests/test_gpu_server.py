@@ -189,3 +189,49 @@ def test_server_diis_equals_alone(engine):
             assert {k: out[i][k] for k in KEYS} == alone, (policy, i)
             assert alone["converged"] and alone["steps"] <= plain["steps"]
             assert abs(alone["e_tot"] - plain["e_tot"]) <= 1e-5
+
+
+def serving_slice():
+    """A seeded slice of the serving stream (200/s, seed 7): its first 12
+    requests (alkanes up to C46H94, water clusters up to (H2O)161, N = 966),
+    the largest water cluster of the first 5 s ((H2O)199, N = 1194), the
+    longest alkane (C60H122) and the alkanes that have reference fixtures."""
+    reqs = geometry.serving_stream(200.0, 5.0, 7)
+    picks = list(range(12)) + [108, 47, 22, 41, 72, 82, 98]
+    return [(i, reqs[i][1], reqs[i][2]) for i in picks]
+
+
+FIXTURE_OF = {"C2H6": "alkane_02", "C3H8": "alkane_03", "C5H12": "alkane_05",
+              "C10H22": "alkane_10", "C40H82": "cfg2_c40h82"}
+
+
+@pytest.mark.parametrize("policy", ["fp64", "mixed"])
+def test_serving_stream_slice_three_classes(engine, policy):
+    """The serving stream's molecules through the server with the bench's
+    three size classes (N < 300, < 700, >= 700): two water clusters above
+    N = 700 in the large class, every result equal to the molecule alone bit
+    for bit, and the alkanes with reference fixtures on the fixtures (FP64:
+    same steps, 1e-6 eV; mixed: 1e-4 eV)."""
+    picks = serving_slice()
+    assert sum(1 for _, _, m in picks if geometry.n_basis(m) >= 700) >= 2
+    srv = engine.server(hbm_bytes=4e9, max_slots=64, max_atoms=1 << 14, threshold=1e-6,
+                        policy=policy, classes=(300, 700))
+    for i, label, m in picks:
+        say("submit", policy, i, label, geometry.n_basis(m))
+        srv.submit(i, m)
+    out = {r["id"]: r for r in srv.drain()}
+    srv.close()
+    assert sorted(out) == sorted(i for i, _, _ in picks)
+    for i, label, m in picks:
+        r = out[i]
+        alone = engine.batch([m]).scf(threshold=1e-6, policy=policy)[0]
+        say("stream", policy, i, label, r["n_ao"], r["steps"], r["e_tot"], alone["e_tot"])
+        assert r["converged"] and r["n_ao"] == geometry.n_basis(m)
+        assert {k: r[k] for k in KEYS} == alone, (policy, i, label)
+        if label in FIXTURE_OF:
+            g = load_golden(FIXTURE_OF[label])
+            if policy == "fp64":
+                assert r["steps"] == int(g["steps"]), label
+                assert abs(r["e_tot"] - float(g["e_tot"])) <= 1e-6, label
+            else:
+                assert abs(r["e_tot"] - float(g["e_tot"])) <= 1e-4, label
