@@ -93,6 +93,22 @@ int sg_fock(sg_batch* b);
 /* SCFCalculation::GetEigenValVecOfFmatrix, i.e. dsyev_('V','L') on F
  * (scfcalculation.cpp:169-203). */
 int sg_eigh(sg_batch* b);
+/* The full eigendecomposition of sg_eigh for one symmetric n x n matrix in
+ * device memory, without allocating: no counterpart in the reference (whose
+ * dsyev_ call, scfcalculation.cpp:169-203, allocates its own workspace); for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).  a_dev: n x n row-major (only the lower triangle is
+ * read); eigval_dev[n]: ascending eigenvalues; eigvec_dev[n*n]: eigenvectors
+ * column-major.  All three are device pointers of the caller's.  The kernels
+ * are sg_eigh's; their workspace is one fixed device array per GPU held by the
+ * library for the life of the process (about 10.5 MB), so n is at most 512
+ * and larger n is rejected with SG_ERR_INVALID.  Every call on a GPU uses that
+ * GPU's array: calls that overlap on one GPU (other streams, other contexts,
+ * other host threads) race, and must be ordered by the caller.  The work is
+ * queued on `stream` (a hipStream_t; NULL = the context's stream) and the call
+ * returns without waiting; it calls no allocation routine. */
+int sg_eigh_noalloc(sg_context* ctx, int n, const double* a_dev, double* eigval_dev,
+                    double* eigvec_dev, void* stream);
 /* DensityMatrix::ComputeMatrix from the eigenvectors (densitymatrix.cpp:38-45). */
 int sg_density(sg_batch* b);
 /* ElectronicEnergy::ComputeEnergy (electronicenergy.cpp:12-36); e_el[n_mol]. */

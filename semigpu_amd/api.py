@@ -38,7 +38,7 @@ EXPORTS = (
     "sg_get_status", "sg_calibration_copy", "sg_timing_enable", "sg_timing_read", "sg_timing_reset",
     "sg_server_create", "sg_server_destroy", "sg_server_submit", "sg_server_step",
     "sg_server_poll", "sg_server_clock", "sg_server_set_diis", "sg_ctx_set_large_basis", "sg_server_create_classes",
-    "sg_heat_of_formation", "sg_ctx_set_graphs",
+    "sg_heat_of_formation", "sg_ctx_set_graphs", "sg_eigh_noalloc",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -93,6 +93,7 @@ def load_library(path: str = LIB_PATH):
     L.sg_ctx_destroy.argtypes = [_vp]
     L.sg_ctx_set_large_basis.argtypes = [_vp, ctypes.c_int]
     L.sg_ctx_set_graphs.argtypes = [_vp, ctypes.c_int]
+    L.sg_eigh_noalloc.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]
     L.sg_heat_of_formation.argtypes = [ctypes.c_int, _ip, ctypes.c_double, _dp]
     L.sg_batch_create.argtypes = [_vp, ctypes.c_int, _ip, _ip, _dp, ctypes.POINTER(_vp)]
     L.sg_batch_destroy.argtypes = [_vp]
@@ -217,6 +218,14 @@ class Engine:
         """Molecules of later batches with N >= n_min take the multi-workgroup
         eigensolver (default: only those above the one-workgroup limit, 1280)."""
         _check(self.L.sg_ctx_set_large_basis(self.h, int(n_min)))
+
+    def eigh_noalloc(self, n: int, a_ptr: int, eigval_ptr: int, eigvec_ptr: int,
+                     stream: int = 0):
+        """sg_eigh_noalloc on device pointers (integers): queues the full
+        eigendecomposition of the n x n matrix at a_ptr on `stream` (0: the
+        context's) without allocating; the caller orders and waits."""
+        _check(self.L.sg_eigh_noalloc(self.h, int(n), _vp(a_ptr), _vp(eigval_ptr),
+                                      _vp(eigvec_ptr), _vp(stream) if stream else None))
 
     def graphs(self, on: bool = True):
         """Capture SCF iterations as HIP graphs (default) or launch them directly."""

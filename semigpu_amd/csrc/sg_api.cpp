@@ -98,6 +98,39 @@ __global__ void __launch_bounds__(256) k_build_lists(int n_atoms, const MolDesc*
   if (lane == 0) partners[o.partner] = A;
 }
 
+// Workspace of sg_eigh_noalloc: one file-scope device array per GPU, sized for
+// the largest matrix that entry point takes (kNoAllocMaxN): the molecule
+// descriptor, order and status words, the Gershgorin bound, d, e, tau and the
+// eigensolver's 5 N^2 work matrix.  Host code never uses the array's name as
+// an address: sg_ctx_create looks up its device address per GPU
+// (hipGetSymbolAddress) and that address is what the kernels get.
+constexpr int kNoAllocMaxN = 512;
+constexpr size_t kNoAllocHead = 1024;  // bytes: descriptor, order, status, bound
+constexpr size_t kNoAllocDoubles =
+    kNoAllocHead / 8 + 3 * (size_t)kNoAllocMaxN + 5 * (size_t)kNoAllocMaxN * kNoAllocMaxN;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+static void* g_scratch_ptr[64];  // device address of `scratch` per device ordinal
+
+// Writes the single-molecule descriptor, order and status of an
+// sg_eigh_noalloc call into its workspace (a kernel rather than copies, so
+// that nothing in the call allocates staging memory or records a memset).
+__global__ void k_noalloc_setup(MolDesc* __restrict__ mol, int* __restrict__ order,
+                                int* __restrict__ status, int n) {
+  if (threadIdx.x == 0) {
+    MolDesc M;
+    M.atom0 = 0;
+    M.n_atoms = 0;
+    M.n_ao = n;
+    M.n_occ = n / 2;
+    M.mat0 = 0;
+    M.blk0 = 0;
+    M.vec0 = 0;
+    *mol = M;
+    *order = 0;
+    *status = 0;
+  }
+}
+
 // Plain copy, 16 bytes per lane per step: reads and writes exactly n16 * 16 bytes.
 __global__ void k_calibration_copy(const double2* __restrict__ src, double2* __restrict__ dst,
                                    int64_t n16) {
@@ -172,6 +205,11 @@ int sg_ctx_create(int device, sg_context** out) {
   if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0)
     return fail(SG_ERR_NO_DEVICE, "no HIP device " + std::to_string(device));
   HIP_TRY(hipSetDevice(device));
+  if (device < 64 && !g_scratch_ptr[device]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = p;
+  }
   auto* c = new sg_context();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -615,6 +653,46 @@ int sg_eigh(sg_batch* b) {
     timed_eigh(b->ctx, b->d, b->ctx->stream, true);
   }
   return check_launch("eigh");
+}
+
+int sg_eigh_noalloc(sg_context* c, int n, const double* a_dev, double* eigval_dev,
+                    double* eigvec_dev, void* stream) {
+  if (!c || !a_dev || !eigval_dev || !eigvec_dev)
+    return fail(SG_ERR_INVALID, "sg_eigh_noalloc: null argument");
+  if (n < 1 || n > kNoAllocMaxN)
+    return fail(SG_ERR_INVALID, "sg_eigh_noalloc: n = " + std::to_string(n) + " outside 1.." +
+                                    std::to_string(kNoAllocMaxN) +
+                                    " (the fixed workspace's limit; sg_eigh takes larger ones)");
+  if (c->device >= 64 || !g_scratch_ptr[c->device])
+    return fail(SG_ERR_INVALID, "sg_eigh_noalloc: no workspace address for this device");
+  HIP_TRY(hipSetDevice(c->device));
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  char* base = (char*)g_scratch_ptr[c->device];
+  BatchDev d;
+  d.n_mol = 1;
+  d.n_mat = (int64_t)n * n;
+  d.n_vec = n;
+  d.max_n = n;
+  d.max_n_small = n;
+  d.mols = (MolDesc*)base;
+  int* order = (int*)(base + 256);
+  d.order = order;
+  d.status = (int*)(base + 512);
+  d.tnorm = (double*)(base + 768);
+  double* v = (double*)(base + kNoAllocHead);
+  d.dT = v;
+  d.eT = v + kNoAllocMaxN;
+  d.tau = v + 2 * kNoAllocMaxN;
+  d.W = v + 3 * kNoAllocMaxN;
+  d.F = const_cast<double*>(a_dev);
+  d.eig = eigval_dev;
+  d.C = eigvec_dev;
+  static_assert(sizeof(MolDesc) <= 256, "descriptor room");
+  k_noalloc_setup<<<1, 64, 0, s>>>(d.mols, order, d.status, n);
+  int r = check_launch("sg_eigh_noalloc setup");
+  if (r) return r;
+  launch_eigh(d, s, true);
+  return check_launch("sg_eigh_noalloc");
 }
 
 int sg_density(sg_batch* b) {
