@@ -202,6 +202,18 @@ int sg_gradient_fd(sg_context* ctx, int n_atoms, const int* Z, const double* xyz
                    double step_ang, double threshold_ev, int diis_depth, double* e_tot,
                    double* grad, int* all_converged);
 
+/* Analytic nuclear gradient of the MNDO total energy (E_el + E_cc) of every
+ * molecule of the batch at its current density, i.e. after sg_scf_run /
+ * sg_scf_run_policy has converged it: the derivatives of every atom pair's
+ * two-center integrals, overlaps and core-core term at fixed density (exact
+ * for a converged NDDO SCF, whose basis is orthonormal by assumption; the
+ * error is first order in the SCF residual, about 1e-6 eV/A after an energy
+ * test of 1e-12 eV, about 1e-8 after further steps with threshold 0).
+ * grad[3 i + k] for atom i of the batch (the atom_offsets order), axis k, in
+ * eV/Angstrom.  No counterpart in the reference, whose README lists gradients
+ * and a geometry optimiser as to do (README.md:109-110). */
+int sg_gradient(sg_batch* b, double* grad);
+
 /* Host copies for the reference's host-side views. */
 long sg_pair_block_count(const sg_batch* b, int mol);
 /* double**** all2CenterIntegral contents in the reference jagged order

@@ -37,6 +37,7 @@ UNITS = {
     "sg_linalg.hip": [],
     "sg_large.hip": [],
     "sg_diis.hip": [],
+    "sg_gradient.hip": ["-ffp-contract=off"],
     "sg_api.cpp": ["-x", "hip"],
     "sg_server.cpp": ["-x", "hip"],
 }

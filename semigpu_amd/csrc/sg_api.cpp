@@ -44,6 +44,7 @@ struct sg_batch {
   std::vector<AtomDesc> atoms;
   std::vector<int64_t> blk_count;  // doubles of pair blocks per molecule
   bool have_ints = false, have_hcore = false, have_ecc = false, have_fp32 = false;
+  bool have_scf = false;  // an SCF has run: P is the converged density
   std::vector<double> e_cc;
   std::vector<int> large_idx;       // molecules on the multi-workgroup eigensolver
   std::vector<MolDesc> large_mols;  // their descriptors
@@ -866,6 +867,7 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
     HIP_TRY(hipStreamSynchronize(s));
     n_active = host_active;
   }
+  b->have_scf = true;
   if (res) {
     const int n = b->d.n_mol;
     std::vector<double> e(n);
@@ -1029,6 +1031,22 @@ int sg_gradient_fd(sg_context* ctx, int n_atoms, const int* Z, const double* xyz
     grad[q] = (res[1 + 2 * q].e_tot - res[2 + 2 * q].e_tot) / (2.0 * step_ang);
   if (e_tot) *e_tot = res[0].e_tot;
   if (all_converged) *all_converged = conv;
+  return SG_OK;
+}
+
+int sg_gradient(sg_batch* b, double* grad) {
+  if (!b || !grad) return fail(SG_ERR_INVALID, "sg_gradient: null batch or output");
+  if (!b->have_scf) return fail(SG_ERR_ORDER, "sg_gradient needs an SCF first (sg_scf_run)");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  hipStream_t s = b->ctx->stream;
+  // the output passes through the per-atom Fock weights (10 doubles per atom,
+  // rebuilt by every Fock build)
+  launch_gradient(b->d, s, b->d.wdiag);
+  int r = check_launch("gradient");
+  if (r) return r;
+  HIP_TRY(hipMemcpyAsync(grad, b->d.wdiag, sizeof(double) * 3 * (size_t)b->d.n_atoms,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
   return SG_OK;
 }
 

@@ -26,314 +26,15 @@
 //        (corecorerepulsion.cpp:12-65).
 #include "sg_common.h"
 #include "sg_kernels.h"
+#include "sg_pair_math.h"
 
 namespace sg {
 
 constexpr int kCoreChunk = 2048;  // core-core pair terms per round of k_core_core
-
-// ---------------------------------------------------------------------------
-// Multipole interactions in Hartree; (R, element of first pair, element of
-// second pair).  Same formulas and evaluation order as multipole.cpp.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double pp_(int z, int l) {
-  return l == 0 ? c_par.pp0[z] : (l == 1 ? c_par.pp1[z] : c_par.pp2[z]);
-}
-__device__ __forceinline__ double addt(int a, int l, int b, int m) {
-  double t = pp_(a, l) + pp_(b, m);
-  return t * t;
-}
-__device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
-
-__device__ double mp_qq(double R, int a, int b) { return 1.0 / sqrt(R * R + addt(a, 0, b, 0)); }
-__device__ double mp_qUz(double R, int a, int b) {
-  double t = addt(a, 0, b, 1), f = 0.0, x;
-  x = R + c_par.dd0[b]; f += rcp(2.0 * sqrt(x * x + t));
-  x = R - c_par.dd0[b]; f -= rcp(2.0 * sqrt(x * x + t));
-  return f;
-}
-__device__ double mp_qQpipi(double R, int a, int b) {
-  double t = addt(a, 0, b, 2), f = 0.0, x;
-  x = (2.0 * c_par.dd1[b]) * (2.0 * c_par.dd1[b]);
-  f += rcp(2.0 * sqrt(R * R + x + t));
-  f -= rcp(2.0 * sqrt(R * R + t));
-  return f;
-}
-__device__ double mp_qQzz(double R, int a, int b) {
-  double t = addt(a, 0, b, 2), f = 0.0, x;
-  x = R + 2.0 * c_par.dd1[b]; f += rcp(4.0 * sqrt(x * x + t));
-  f -= rcp(2.0 * sqrt(R * R + t));
-  x = R - 2.0 * c_par.dd1[b]; f += rcp(4.0 * sqrt(x * x + t));
-  return f;
-}
-__device__ double mp_UpiUpi(double R, int a, int b) {
-  double t = addt(a, 1, b, 1), f = 0.0, x;
-  x = c_par.dd0[a] - c_par.dd0[b]; f += rcp(2.0 * sqrt(R * R + x * x + t));
-  x = c_par.dd0[a] + c_par.dd0[b]; f -= rcp(2.0 * sqrt(R * R + x * x + t));
-  return f;
-}
-__device__ double mp_UzUz(double R, int a, int b) {
-  double t = addt(a, 1, b, 1), f = 0.0, x;
-  const double da = c_par.dd0[a], db = c_par.dd0[b];
-  x = da + db + R;    f += -1.0 * rcp(4.0 * sqrt(x * x + t));
-  x = da + -db + R;   f += 1.0 * rcp(4.0 * sqrt(x * x + t));
-  x = -da + db + R;   f += 1.0 * rcp(4.0 * sqrt(x * x + t));
-  x = -da + -db + R;  f += -1.0 * rcp(4.0 * sqrt(x * x + t));
-  return f;
-}
-__device__ double mp_UpiQpiz(double R, int a, int b) {
-  double t = addt(a, 1, b, 2), f = 0.0, x, y;
-  const double da = c_par.dd0[a], db = c_par.dd1[b];
-  const double sg[2] = {1.0, -1.0}, sn[3] = {-1.0, 1.0, -1.0};
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      x = R + sg[i] * db;
-      y = da + sg[j] * db;
-      f += sn[i + j] * rcp(4.0 * sqrt(x * x + y * y + t));
-    }
-  return f;
-}
-__device__ double mp_UzQpipi(double R, int a, int b) {
-  double t = addt(a, 1, b, 2), f = 0.0, x, y;
-  const double da = c_par.dd0[a], db = c_par.dd1[b];
-  const double sg[2] = {1.0, -1.0}, sn[2] = {-1.0, 1.0};
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    x = R + sg[i] * da;
-    y = 2.0 * db;
-    f += sn[i] * rcp(4.0 * sqrt(x * x + y * y + t));
-  }
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    x = R + sg[i] * da;
-    f += sg[i] * rcp(4.0 * sqrt(x * x + t));
-  }
-  return f;
-}
-__device__ double mp_UzQz(double R, int a, int b) {
-  double t = addt(a, 1, b, 2), f = 0.0, x;
-  const double da = c_par.dd0[a], db = c_par.dd1[b];
-  const double sg[2] = {1.0, -1.0}, sn[4] = {-1.0, -1.0, 1.0, 1.0};
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      x = R + sg[i] * da + sg[j] * 2.0 * db;
-      f += sn[2 * i + j] * rcp(8.0 * sqrt(x * x + t));
-    }
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    x = R + sg[i] * da;
-    f += sg[i] * rcp(4.0 * sqrt(x * x + t));
-  }
-  return f;
-}
-__device__ double mp_QpipiQpipi(double R, int a, int b) {
-  double t = addt(a, 2, b, 2), f = 0.0, x, y;
-  const double da = c_par.dd1[a], db = c_par.dd1[b];
-  y = da + db;  x = 4.0 * (y * y); f += rcp(8.0 * sqrt(R * R + x + t));
-  y = da + -db; x = 4.0 * (y * y); f += rcp(8.0 * sqrt(R * R + x + t));
-  y = 2.0 * da; x = y * y; f -= rcp(4.0 * sqrt(R * R + x + t));
-  y = 2.0 * db; x = y * y; f -= rcp(4.0 * sqrt(R * R + x + t));
-  f += rcp(4.0 * sqrt(R * R + t));
-  return f;
-}
-__device__ double mp_QxxQyy(double R, int a, int b) {
-  double t = addt(a, 2, b, 2), f = 0.0, x;
-  const double ya = 2.0 * c_par.dd1[a], yb = 2.0 * c_par.dd1[b];
-  x = ya * ya + yb * yb; f += rcp(4.0 * sqrt(R * R + x + t));
-  x = ya * ya;           f -= rcp(4.0 * sqrt(R * R + x + t));
-  x = yb * yb;           f -= rcp(4.0 * sqrt(R * R + x + t));
-  f += rcp(4.0 * sqrt(R * R + t));
-  return f;
-}
-__device__ double mp_QpipiQzz(double R, int a, int b) {
-  double t = addt(a, 2, b, 2), f = 0.0, aux, auxx, x;
-  const double ya = 2.0 * c_par.dd1[a], db = c_par.dd1[b];
-  auxx = ya * ya + t;
-  const double sg[2] = {1.0, -1.0};
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    x = R + 2.0 * sg[i] * db;
-    aux = x * x;
-    f += rcp(8.0 * sqrt(aux + auxx));
-    f -= rcp(8.0 * sqrt(aux + t));
-    f -= rcp(8.0 * sqrt(R * R + auxx));
-  }
-  f += rcp(4.0 * sqrt(R * R + t));
-  return f;
-}
-__device__ double mp_QzzQzz(double R, int a, int b) {
-  double t = addt(a, 2, b, 2), f = 0.0, x;
-  const double da = c_par.dd1[a], db = c_par.dd1[b];
-  const double sg[2] = {1.0, -1.0};
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      x = R + sg[i] * 2.0 * da + sg[j] * 2.0 * db;
-      f += rcp(16.0 * sqrt(x * x + t));
-    }
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    x = R + 2.0 * sg[i] * da; f -= rcp(8.0 * sqrt(x * x + t));
-    x = R + 2.0 * sg[i] * db; f -= rcp(8.0 * sqrt(x * x + t));
-  }
-  f += rcp(4.0 * sqrt(R * R + t));
-  return f;
-}
-__device__ double mp_QpizQpiz(double R, int a, int b) {
-  double t = addt(a, 2, b, 2), f = 0.0, x, y;
-  const double da = c_par.dd1[a], db = c_par.dd1[b];
-  const double sg[2] = {1.0, -1.0};
-  const double sn[8] = {1.0, -1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0};
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        x = R + sg[i] * da + sg[j] * db;
-        y = da + sg[k] * db;
-        f += sn[4 * i + 2 * j + k] * rcp(8.0 * sqrt(x * x + y * y + t));
-      }
-  return f;
-}
-
-// Local frame: row 2 = unit vector A->C, rows 0/1 complete it
-// (ComputeRotationMatrix, twocenterintegral.cpp:468-490, including its fixed
-// frame when A->C is within 1e-10 of the z axis).
-__device__ __forceinline__ double dist3(double ax, double ay, double az, double bx, double by,
-                                        double bz) {
-  double r = 0.0;
-  r += (ax - bx) * (ax - bx);
-  r += (ay - by) * (ay - by);
-  r += (az - bz) * (az - bz);
-  return sqrt(r);
-}
-
-struct Frame {
-  double m[3][3];
-};
-
-__device__ __forceinline__ Frame local_frame(const AtomDesc& A, const AtomDesc& C, double R) {
-  Frame f;
-  f.m[2][0] = (C.x - A.x) / R;
-  f.m[2][1] = (C.y - A.y) / R;
-  f.m[2][2] = (C.z - A.z) / R;
-  if (fabs(f.m[2][0] - 0.0) < 1e-10 && fabs(f.m[2][1] - 0.0) < 1e-10) {
-    f.m[1][0] = 0.0; f.m[1][1] = 1.0; f.m[1][2] = 0.0;
-    f.m[0][0] = 1.0; f.m[0][1] = 0.0; f.m[0][2] = 0.0;
-  } else {
-    double s = 1.0 / sqrt(f.m[2][0] * f.m[2][0] + f.m[2][1] * f.m[2][1]);
-    f.m[1][0] = s * -f.m[2][1];
-    f.m[1][1] = s * f.m[2][0];
-    f.m[1][2] = 0.0;
-    f.m[0][0] = f.m[2][1] * f.m[1][2] - f.m[2][2] * f.m[1][1];
-    f.m[0][1] = f.m[2][2] * f.m[1][0] - f.m[2][0] * f.m[1][2];
-    f.m[0][2] = f.m[2][0] * f.m[1][1] - f.m[2][1] * f.m[1][0];
-  }
-  return f;
-}
-
-// The local-frame combinations an X-X pair needs, each formed once with the
-// same additions as the reference's IntegralType* functions.
-struct LocalXX {
-  double qq, qUz, Uzq;
-  double sspp_pi, sspp_z, ppss_pi, ppss_z;   // SS|PP and PP|SS (inverted) sums
-  double sppi, spzz;                          // SP|SP: UpiUpi, UzUz
-  double sppp1, sppp2, sppp3;                 // SP|PP
-  double ppsp1, ppsp2, ppsp3;                 // PP|SP (inverted)
-  double zzzz, pppp, zzpp, ppzz, pzpz, xxyy, xyxy;
-};
-
-__device__ LocalXX local_xx(double R, int a, int c) {
-  LocalXX v;
-  v.qq = mp_qq(R, a, c);
-  v.qUz = mp_qUz(R, a, c);
-  v.Uzq = mp_qUz(R, c, a) * -1.0;
-  const double qQpp = mp_qQpipi(R, a, c), Qppq = mp_qQpipi(R, c, a);
-  const double qQzz = mp_qQzz(R, a, c), Qzzq = mp_qQzz(R, c, a);
-  v.sspp_pi = v.qq + qQpp;
-  v.sspp_z = v.qq + qQzz;
-  v.ppss_pi = v.qq + Qppq;
-  v.ppss_z = v.qq + Qzzq;
-  v.sppi = mp_UpiUpi(R, a, c);
-  v.spzz = mp_UzUz(R, a, c);
-  v.sppp1 = v.Uzq + mp_UzQpipi(R, a, c);
-  v.sppp2 = v.Uzq + mp_UzQz(R, a, c);
-  v.sppp3 = mp_UpiQpiz(R, a, c);
-  v.ppsp1 = v.qUz + mp_UzQpipi(R, c, a) * -1.0;
-  v.ppsp2 = v.qUz + mp_UzQz(R, c, a) * -1.0;
-  v.ppsp3 = mp_UpiQpiz(R, c, a) * -1.0;
-  v.zzzz = v.qq + qQzz + Qzzq + mp_QzzQzz(R, a, c);
-  v.pppp = v.qq + qQpp + Qppq + mp_QpipiQpipi(R, a, c);
-  v.zzpp = v.qq + qQpp + Qzzq + mp_QpipiQzz(R, c, a);
-  v.ppzz = v.qq + Qppq + qQzz + mp_QpipiQzz(R, a, c);
-  v.pzpz = mp_QpizQpiz(R, a, c);
-  v.xxyy = v.qq + qQpp + Qppq + mp_QxxQyy(R, a, c);
-  v.xyxy = (v.pppp - v.xxyy) * 0.5;
-  return v;
-}
-
-// Rotated entry (pair K of A | pair L of C) in eV; K, L in MOPAC order.
-template <int K, int L>
-__device__ __forceinline__ double xx_entry(const LocalXX& v, const Frame& F) {
-  constexpr int s0 = mop_a(K), s1 = mop_b(K), s2 = mop_a(L), s3 = mop_b(L);
-  constexpr int pa0 = (s0 > 0) + (s1 > 0), pb0 = (s2 > 0) + (s3 > 0);
-  constexpr bool inv = pa0 > pb0;  // CorrectOrderIntegral (:785-796)
-  constexpr int pa = inv ? pb0 : pa0, pb = inv ? pa0 : pb0;
-  constexpr int t0 = inv ? s2 : s0, t1 = inv ? s3 : s1, t2 = inv ? s0 : s2, t3 = inv ? s1 : s3;
-  const double(&m)[3][3] = F.m;
-  double r;
-  if constexpr (pa == 0 && pb == 0) {
-    r = v.qq;
-  } else if constexpr (pa == 0 && pb == 1) {
-    r = (inv ? v.Uzq : v.qUz) * m[2][t3 - 1];
-  } else if constexpr (pa == 0 && pb == 2) {
-    double vp = inv ? v.ppss_pi : v.sspp_pi, vz = inv ? v.ppss_z : v.sspp_z;
-    vp = vp * m[0][t2 - 1] * m[0][t3 - 1] + vp * m[1][t2 - 1] * m[1][t3 - 1];
-    vz = vz * m[2][t2 - 1] * m[2][t3 - 1];
-    r = vp + vz;
-  } else if constexpr (pa == 1 && pb == 1) {
-    double v1 = v.sppi * m[0][t1 - 1] * m[0][t3 - 1] + v.sppi * m[1][t1 - 1] * m[1][t3 - 1];
-    double v2 = v.spzz * m[2][t1 - 1] * m[2][t3 - 1];
-    r = v1 + v2;
-  } else if constexpr (pa == 1 && pb == 2) {
-    const double w1 = inv ? v.ppsp1 : v.sppp1, w2 = inv ? v.ppsp2 : v.sppp2,
-                 w3 = inv ? v.ppsp3 : v.sppp3;
-    double v1 = w1 * m[2][t1 - 1] * m[0][t2 - 1] * m[0][t3 - 1] +
-                w1 * m[2][t1 - 1] * m[1][t2 - 1] * m[1][t3 - 1];
-    double v2 = w2 * m[2][t1 - 1] * m[2][t2 - 1] * m[2][t3 - 1];
-    double v3 = w3 * m[0][t1 - 1] * m[2][t2 - 1] * m[0][t3 - 1] +
-                w3 * m[1][t1 - 1] * m[2][t2 - 1] * m[1][t3 - 1] +
-                w3 * m[0][t1 - 1] * m[2][t3 - 1] * m[0][t2 - 1] +
-                w3 * m[1][t1 - 1] * m[2][t3 - 1] * m[1][t2 - 1];
-    r = v1 + v2 + v3;
-  } else {
-    const int a = t0 - 1, b = t1 - 1, c = t2 - 1, d = t3 - 1;
-    double zz = v.zzzz * m[2][a] * m[2][b] * m[2][c] * m[2][d];
-    double pp = v.pppp * m[0][a] * m[0][b] * m[0][c] * m[0][d] +
-                v.pppp * m[1][a] * m[1][b] * m[1][c] * m[1][d];
-    double zp = v.zzpp * m[2][a] * m[2][b] * m[0][c] * m[0][d] +
-                v.zzpp * m[2][a] * m[2][b] * m[1][c] * m[1][d];
-    double pz = v.ppzz * m[0][a] * m[0][b] * m[2][c] * m[2][d] +
-                v.ppzz * m[1][a] * m[1][b] * m[2][c] * m[2][d];
-    double q = v.pzpz;
-    double pzpz = q * m[0][a] * m[2][b] * m[0][c] * m[2][d] + q * m[0][a] * m[2][b] * m[0][d] * m[2][c] +
-                  q * m[0][b] * m[2][a] * m[0][c] * m[2][d] + q * m[0][b] * m[2][a] * m[0][d] * m[2][c] +
-                  q * m[1][a] * m[2][b] * m[1][c] * m[2][d] + q * m[1][a] * m[2][b] * m[1][d] * m[2][c] +
-                  q * m[1][b] * m[2][a] * m[1][c] * m[2][d] + q * m[1][b] * m[2][a] * m[1][d] * m[2][c];
-    double xxyy = v.xxyy * m[0][a] * m[0][b] * m[1][c] * m[1][d] +
-                  v.xxyy * m[1][a] * m[1][b] * m[0][c] * m[0][d];
-    double w = v.xyxy;
-    double xyxy = w * m[0][a] * m[1][b] * m[0][c] * m[1][d] + w * m[0][a] * m[1][b] * m[0][d] * m[1][c] +
-                  w * m[0][b] * m[1][a] * m[0][c] * m[1][d] + w * m[0][b] * m[1][a] * m[0][d] * m[1][c];
-    r = zz + pp + zp + pz + pzpz + xxyy + xyxy;
-  }
-  return r * kHartreeToEv;
-}
+// Molecules with more atom pairs than this sum their core-core terms in a fixed
+// tree instead of the reference's serial order (one lane adding 4.5 M terms
+// of the 1000-water box took 117 ms).
+constexpr int64_t kCoreSerialMax = 1 << 17;
 
 template <int E>
 struct XXFill {
@@ -460,61 +161,6 @@ __global__ void k_one_center(int n_atoms, const AtomDesc* __restrict__ atoms,
     out[t] = v;
     if (blocks32) blocks32[o + t] = (float)v;
   }
-}
-
-// ---------------------------------------------------------------------------
-// Off-diagonal Hcore blocks beta * S / 2 for one atom pair per lane, launched
-// per pair type so the lanes of a wave do the same work.  The STO-6G overlap
-// is Overlap::ComputeOverlap (overlap.cpp:68-134): a sum over the 6 x 6
-// primitive pairs of exp(-mu R^2) times one Gaussian moment per axis.  The AO
-// pairs of one shell pair (s-s, s-p, p-s, p-p) share their primitive
-// exponents, so exp(-mu R^2), the Gaussian centre and the axis moments are
-// formed once per primitive pair and shell pair.
-template <int NA, int NB>
-__device__ __forceinline__ void hcore_shell(const AtomDesc& A, int sa, const AtomDesc& B, int sb,
-                                            double ab, double* tot) {
-  const double ac[3] = {A.x, A.y, A.z}, bc[3] = {B.x, B.y, B.z};
-  for (int i = 0; i < 6; ++i)
-    for (int j = 0; j < 6; ++j) {
-      const double al = sa == 0 ? c_sto.expS[A.zn][i] : c_sto.expP[A.zn][i];
-      const double be = sb == 0 ? c_sto.expS[B.zn][j] : c_sto.expP[B.zn][j];
-      const double p = al + be;
-      const double mu = al * be / p;
-      double o = exp(-mu * ab);
-      o *= sa == 0 ? c_sto.cS[A.zn][i] : c_sto.cP[A.zn][i];
-      o *= sb == 0 ? c_sto.cS[B.zn][j] : c_sto.cP[B.zn][j];
-      // Per axis, the reference's 12-point Gauss-Hermite sums of 1, (x+pa),
-      // (x+pb) and (x+pa)(x+pb) against exp(-p x^2) (overlap.cpp:114-124,
-      // 177-192) are exact for these degrees, so their closed forms are used:
-      // sqrt(pi/p) {1, pa, pb, pa pb + 1/(2p)}
-      const double i00 = sqrt(M_PI / p);
-      const double h2 = 0.5 / p;
-      double f10[3], f01[3], f11[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        double P = al * ac[k];
-        P += be * bc[k];
-        P /= p;
-        const double pa = P - ac[k], pb = P - bc[k];
-        f10[k] = pa * i00;
-        f01[k] = pb * i00;
-        f11[k] = (pa * pb + h2) * i00;
-      }
-      // AO u of shell sa (0 = s; 1..3 = px, py, pz when sa = 1), likewise v
-#pragma unroll
-      for (int u = 0; u < NA; ++u)
-#pragma unroll
-        for (int v = 0; v < NB; ++v) {
-          const int la = sa ? u + 1 : 0, lb = sb ? v + 1 : 0;
-          double t = o;
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            const bool pa = la == k + 1, pb = lb == k + 1;
-            t *= pa ? (pb ? f11[k] : f10[k]) : (pb ? f01[k] : i00);
-          }
-          tot[u * NB + v] += t;
-        }
-    }
 }
 
 __device__ __forceinline__ void hcore_store(const AtomDesc& A, int u, const AtomDesc& B, int v,
@@ -665,7 +311,11 @@ __device__ __forceinline__ void pair_of(int64_t p, int na, int& i, int& j) {
 // One workgroup per molecule.  The pairs are taken in the reference order in
 // rounds of kCoreChunk terms: waves 1-3 evaluate round r + 1 into one LDS
 // buffer while lane 0 of wave 0 adds round r from the other to the running
-// sum, in order, so the sum is the reference's serial sum bit for bit.
+// sum, in order, so the sum is the reference's serial sum bit for bit.  Above
+// kCoreSerialMax pairs every thread evaluates and sums its share of a round
+// (terms t, t + 256, ... in order), the 256 shares are added by a fixed tree
+// and the rounds in order: the same sum for the molecule wherever it is run,
+// with a smaller rounding error than the serial one.
 __global__ void __launch_bounds__(256) k_core_core(const AtomDesc* __restrict__ atoms,
                                                     const MolDesc* __restrict__ mols,
                                                     const double* __restrict__ blocks,
@@ -680,6 +330,28 @@ __global__ void __launch_bounds__(256) k_core_core(const AtomDesc* __restrict__ 
   const AtomDesc* at = atoms + M.atom0;
   __shared__ double term[2][kCoreChunk];
   const int nr = (int)((npair + kCoreChunk - 1) / kCoreChunk);
+  if (npair > kCoreSerialMax) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double e = 0.0;
+    for (int r = 0; r < nr; ++r) {
+      const int64_t p0 = (int64_t)r * kCoreChunk;
+      const int cnt = (int)(npair - p0 < kCoreChunk ? npair - p0 : kCoreChunk);
+      double part = 0.0;
+      for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+        int i, j;
+        pair_of(p0 + q, na, i, j);
+        part += core_pair(at[i], at[j], blk);
+      }
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o, 64);
+      if (lane == 0) term[0][wave] = part;
+      __syncthreads();
+      if (threadIdx.x == 0) e += (term[0][0] + term[0][1]) + (term[0][2] + term[0][3]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) e_cc[m] = e;
+    return;
+  }
   auto fill = [&](int r, int t0, int nt) {
     const int64_t p0 = (int64_t)r * kCoreChunk;
     const int cnt = (int)(npair - p0 < kCoreChunk ? npair - p0 : kCoreChunk);

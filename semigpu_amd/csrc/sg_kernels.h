@@ -220,5 +220,9 @@ void launch_tridiag_large(const BatchDev& b, hipStream_t s, int m, const MolDesc
 void launch_back_large(const BatchDev& b, hipStream_t s, int m, const MolDesc& M,
                        const int* active, bool all);
 void launch_density(const BatchDev& b, hipStream_t s, bool all);
+// Analytic nuclear gradient at the current density: grad[3 atom + axis] in
+// eV / Angstrom (the pair derivatives pass through each molecule's
+// eigensolver workspace).
+void launch_gradient(const BatchDev& b, hipStream_t s, double* grad);
 
 }  // namespace sg
