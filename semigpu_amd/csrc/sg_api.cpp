@@ -542,6 +542,12 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   d.n_large = (int)b->large_idx.size();
   d.large_idx = b->large_idx.data();
   d.large_mols = b->large_mols.data();
+  {
+    std::vector<int> nsmall;
+    nsmall.reserve(n_mol - d.n_large);
+    for (int i = d.n_large; i < n_mol; ++i) nsmall.push_back(b->mols[ord[i]].n_ao);
+    set_tri_buckets(d, nsmall.data(), (int)nsmall.size());
+  }
   for (double* z : {d.H, d.F, d.P, d.C})
     if (e == hipSuccess) e = hipMemsetAsync(z, 0, sizeof(double) * mat0, s);
   if (e == hipSuccess) e = hipMemsetAsync(d.status, 0, sizeof(int) * n_mol, s);
@@ -675,6 +681,7 @@ int sg_eigh_noalloc(sg_context* c, int n, const double* a_dev, double* eigval_de
   d.n_vec = n;
   d.max_n = n;
   d.max_n_small = n;
+  set_tri_buckets(d, &n, 1);
   d.mols = (MolDesc*)base;
   int* order = (int*)(base + 256);
   d.order = order;

@@ -23,6 +23,14 @@ constexpr int kDiisMax = 8;   // largest DIIS depth
 // molecule, row-major (symmetric ones are stored in full); eigenvectors are
 // column-major (C[k*N + i] = component i of eigenvector k), which is the
 // layout the reference hands to dsyev (scfcalculation.cpp:175-179).
+// Size buckets of the FP64 reduction of the one-workgroup eigensolver
+// (k_tridiag_reg): largest first, bucket s takes the molecules with
+// kTriBucketMin[s] < N (<= the previous bucket's minimum); the first bucket
+// also takes those above kRegRows after k_tridiag's first N - kRegRows columns.
+constexpr int kTriBuckets = 8;
+constexpr int kRegRows = 256;
+constexpr int kTriBucketMin[kTriBuckets] = {224, 192, 160, 128, 96, 64, 32, 0};
+
 struct BatchDev {
   int n_mol = 0, n_atoms = 0;
   int64_t n_mat = 0;     // sum N^2
@@ -107,7 +115,16 @@ struct BatchDev {
   const int* large_idx = nullptr;
   const MolDesc* large_mols = nullptr;
   int max_n_small = 0;
+  // The molecules of the one-workgroup part (the tail of `order` after the
+  // n_large ones) by reduction bucket: positions [tri_start[s], tri_start[s + 1])
+  // of that part are bucket s; its first tri_head have N > kRegRows.
+  int tri_start[kTriBuckets + 1] = {};
+  int tri_head = 0;
 };
+
+// tri_start / tri_head from the N of the one-workgroup part's molecules in
+// launch order (non-increasing; free server slots count as N = 0).
+void set_tri_buckets(BatchDev& d, const int* n_in_order, int count);
 
 // Molecule m takes part in a launch: every molecule (active null), or one
 // still iterating without an eigensolver failure.
@@ -204,9 +221,9 @@ int64_t diis_part_doubles(int n_mol);
 // Eigensolver of F (all = every eigenpair, else the selected subspace of the
 // active molecules): tridiagonalisation, bisection, inverse iteration,
 // back-transformation; launch_eigh runs the four in order.  The reduction's
-// launches: which = 0 the FP64 one (both types when all workgroups are
-// resident at once) and the multi-workgroup molecules, 1 the separate FP32
-// one (tridiag_fp32_launch: whether there is one), -1 all.
+// launches: which = 0 the FP64 ones and the multi-workgroup molecules, 1 the
+// FP32 one of the mixed_eig32 policy (tridiag_fp32_launch: whether there is
+// one), -1 all.
 bool tridiag_fp32_launch(const BatchDev& b, bool all);
 void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all, int which);
 void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all);

@@ -12,8 +12,12 @@
 //                   wavefront), then the trailing block gets one rank-2nb update
 //                   on the matrix cores.  Reflector c is stored in the upper part
 //                   of row c of the work matrix.  In the FP32 steps of the
-//                   mixed-precision policy the same reduction runs in FP32
-//                   (k_tridiag<., float>, or per molecule in k_tridiag_both).
+//                   mixed_eig32 policy the same reduction runs in FP32
+//                   (k_tridiag<., float>).  In FP64 the trailing block (up
+//                   to 256 x 256) is reduced on chip (k_tridiag_reg, in
+//                   launches bucketed by N; unblocked, no memory traffic per
+//                   column), after k_tridiag's first N - 256 columns for the
+//                   larger molecules.
 //  k_bisect         eigenvalues of the tridiagonal matrix by bisection on the
 //                   Sturm count, two eigenvalues per lane (only the indices the
 //                   caller selected).
@@ -231,7 +235,7 @@ template <int TMAX, typename R>
 __device__ __forceinline__ void tridiag_mol(const MolDesc& M, const double* __restrict__ f,
                                             R* __restrict__ A, double* __restrict__ d,
                                             double* __restrict__ e, double* __restrict__ tau,
-                                            unsigned char* L) {
+                                            unsigned char* L, int stop) {
   typedef Mfma16<R> Mf;
   const int n = M.n_ao, tid = threadIdx.x, nt = blockDim.x;
   const int wave = tid >> 6, lane = tid & 63, nwave = nt >> 6;
@@ -254,7 +258,8 @@ __device__ __forceinline__ void tridiag_mol(const MolDesc& M, const double* __re
   for (int i = wave; i < n; i += nwave)
     for (int l = lane; l <= i; l += 64) A[(int64_t)i * n + l] = (R)f[(int64_t)i * n + l];
   __syncthreads();
-  const int nref = n > 2 ? n - 2 : 0;
+  // the first `stop` columns only, when k_tridiag_reg takes the rest
+  const int nref = min(n > 2 ? n - 2 : 0, stop);
   for (int k0 = 0; k0 < nref; k0 += nb) {
     const int b = min(nb, nref - k0);
     for (int t = tid; t < nb * n; t += nt) {
@@ -536,7 +541,7 @@ __device__ __forceinline__ void tridiag_mol(const MolDesc& M, const double* __re
     }
     __syncthreads();
   }
-  if (tid == 0) {
+  if (tid == 0 && nref + 2 >= n) {
     if (n >= 2) {
       d[n - 2] = A[(int64_t)(n - 2) * n + (n - 2)];
       e[n - 2] = A[(int64_t)(n - 1) * n + (n - 2)];
@@ -565,7 +570,9 @@ constexpr int tridiag_threads() {
 // One workgroup per molecule, one launch per working type: prec (null for
 // the full decomposition) marks the molecules whose SCF step runs in FP32,
 // which the float instantiation reduces (in the first half of the
-// work-matrix room) and the double one skips.
+// work-matrix room) and the double one skips.  reg_rows > 0: only the
+// molecules with N > reg_rows, and only their first N - reg_rows columns
+// (k_tridiag_reg reduces the trailing reg_rows x reg_rows block).
 template <int TMAX, typename R>
 __global__ void __launch_bounds__((tridiag_threads<TMAX, R>()))
 __attribute__((amdgpu_waves_per_eu(tridiag_waves<TMAX, R>())))
@@ -573,40 +580,24 @@ k_tridiag(const MolDesc* __restrict__ mols, const double* __restrict__ F,
           double* __restrict__ work, double* __restrict__ dA, double* __restrict__ eA,
           double* __restrict__ tauA, const int* __restrict__ active,
           const int* __restrict__ status, const int* __restrict__ order,
-          const int* __restrict__ prec) {
+          const int* __restrict__ prec, int reg_rows) {
   extern __shared__ double lds[];
   const int m = order ? order[blockIdx.x] : blockIdx.x;
   if (!mol_live(active, status, m)) return;
   const bool fp32 = prec && prec[m];
   if (fp32 != (sizeof(R) == 4)) return;
   const MolDesc M = mols[m];
+  if (reg_rows > 0 && M.n_ao <= reg_rows) return;
+  const int stop = reg_rows > 0 ? M.n_ao - reg_rows : M.n_ao;
   tridiag_mol<TMAX, R>(M, F + M.mat0, (R*)(work + 5 * M.mat0), dA + M.vec0, eA + M.vec0,
-                       tauA + M.vec0, (unsigned char*)lds);
+                       tauA + M.vec0, (unsigned char*)lds, stop);
 }
 
-// Both working types in one launch, chosen per molecule: for launches whose
-// workgroups are all resident at once (a server lane's few molecules), where
-// two launches would run the FP32 and FP64 molecules one after the other.
-template <int TMAX>
-__global__ void __launch_bounds__((tridiag_threads<TMAX, double>()))
-__attribute__((amdgpu_waves_per_eu(tridiag_waves<TMAX, double>())))
-k_tridiag_both(const MolDesc* __restrict__ mols, const double* __restrict__ F,
-               double* __restrict__ work, double* __restrict__ dA, double* __restrict__ eA,
-               double* __restrict__ tauA, const int* __restrict__ active,
-               const int* __restrict__ status, const int* __restrict__ order,
-               const int* __restrict__ prec) {
-  extern __shared__ double lds[];
-  const int m = order ? order[blockIdx.x] : blockIdx.x;
-  if (!mol_live(active, status, m)) return;
-  const MolDesc M = mols[m];
-  double* base = work + 5 * M.mat0;
-  unsigned char* L = (unsigned char*)lds;
-  if (prec[m])
-    tridiag_mol<TMAX, float>(M, F + M.mat0, (float*)base, dA + M.vec0, eA + M.vec0,
-                             tauA + M.vec0, L);
-  else
-    tridiag_mol<TMAX, double>(M, F + M.mat0, base, dA + M.vec0, eA + M.vec0, tauA + M.vec0, L);
-}
+}  // namespace sg
+
+#include "sg_tridiag_reg.h"
+
+namespace sg {
 
 // ---------------------------------------------------------------------------
 // 1/q from the hardware reciprocal estimate and two Newton-Raphson steps
@@ -1525,30 +1516,50 @@ Part small_part(const BatchDev& b) {
 }
 }  // namespace
 
-// SCF steps: while every workgroup fits on the device at once (two per CU up
-// to N = 384, one above), one launch takes both working types; larger batches
-// of the mixed policy get one launch per type (the FP64 molecules' workgroups
-// exit at once in the FP32 launch and the other way round), each with its own
-// register budget.  The full decomposition is FP64.
+// FP64 reductions: every molecule's trailing block on chip in the launch of
+// its size bucket (k_tridiag_reg), after k_tridiag's first N - kRegRows
+// columns for the molecules above kRegRows.  FP32 steps of the mixed_eig32
+// policy: k_tridiag<., float>, its own launch.
+void set_tri_buckets(BatchDev& d, const int* n_in_order, int count) {
+  int s = 0;
+  d.tri_start[0] = 0;
+  d.tri_head = 0;
+  for (int i = 0; i < count; ++i) {
+    const int n = n_in_order[i];
+    if (n > kRegRows) d.tri_head = i + 1;
+    while (s + 1 < kTriBuckets && n <= kTriBucketMin[s]) d.tri_start[++s] = i;
+  }
+  while (s < kTriBuckets) d.tri_start[++s] = count;
+}
+
+// One bucket's launch: tiles per dimension, wavefronts, register tiles per
+// wavefront (k_tridiag_reg<NT, NW, NR>, MR = 32 NT >= the bucket's largest N).
+template <int NT, int NW, int NR>
+static void tridiag_reg_launch(const BatchDev& b, hipStream_t s, const int* order, int count,
+                               const int* act, const int* prec) {
+  static_assert(32 * NT <= kRegRows, "bucket larger than the register block");
+  k_tridiag_reg<NT, NW, NR><<<count, 64 * NW, tridiag_reg_lds<NT, NW, NR>(), s>>>(
+      b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status, order, prec);
+}
+
 namespace {
 struct TridiagPlan {
   Part p;
   const int* prec;  // null: FP64 only
-  bool small, both;
+  bool small;
 };
 TridiagPlan tridiag_plan(const BatchDev& b, bool all) {
   TridiagPlan t;
   t.p = small_part(b);
   t.prec = (all || !b.eig32) ? nullptr : b.prec;
   t.small = t.p.max_n <= 384;
-  t.both = t.prec && t.p.count <= (t.small ? 2 : 1) * kDeviceCUs;
   return t;
 }
 }  // namespace
 
 bool tridiag_fp32_launch(const BatchDev& b, bool all) {
   const TridiagPlan t = tridiag_plan(b, all);
-  return t.p.count > 0 && t.prec && !t.both;
+  return t.p.count > 0 && t.prec;
 }
 
 void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all, int which) {
@@ -1556,18 +1567,35 @@ void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all, int which) 
   const TridiagPlan t = tridiag_plan(b, all);
   const Part& p = t.p;
   const int* prec = t.prec;
-#define SG_TRIDIAG(KERNEL, TM, R)                                                      \
+#define SG_TRIDIAG(KERNEL, TM, R, ROWS)                                                \
   KERNEL<<<p.count, tridiag_threads<TM, R>(), tridiag_lds<TM, R>(p.max_n), s>>>(       \
-      b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status, p.order, prec)
+      b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status, p.order, prec, ROWS)
   if (which != 1 && p.count > 0) {
-    if (t.both && t.small) SG_TRIDIAG(k_tridiag_both<6>, 6, double);
-    else if (t.both) SG_TRIDIAG(k_tridiag_both<20>, 20, double);
-    else if (t.small) SG_TRIDIAG((k_tridiag<6, double>), 6, double);
-    else SG_TRIDIAG((k_tridiag<20, double>), 20, double);
+    if (b.tri_head > 0) {
+      // only the first tri_head molecules of the part have N > kRegRows
+      Part h = p;
+      h.count = b.tri_head;
+      const Part& p = h;
+      if (t.small) SG_TRIDIAG((k_tridiag<6, double>), 6, double, kRegRows);
+      else SG_TRIDIAG((k_tridiag<20, double>), 20, double, kRegRows);
+    }
+    const int* st = b.tri_start;
+#define SG_REG(S, NT, NW, NR)                                                             \
+    if (st[S + 1] > st[S])                                                                 \
+      tridiag_reg_launch<NT, NW, NR>(b, s, p.order + st[S], st[S + 1] - st[S], act, prec)
+    SG_REG(0, 8, 12, 3);
+    SG_REG(1, 7, 10, 3);
+    SG_REG(2, 6, 7, 3);
+    SG_REG(3, 5, 5, 3);
+    SG_REG(4, 4, 4, 3);
+    SG_REG(5, 3, 2, 3);
+    SG_REG(6, 2, 1, 3);
+    SG_REG(7, 1, 1, 1);
+#undef SG_REG
   }
-  if (which != 0 && p.count > 0 && prec && !t.both) {
-    if (t.small) SG_TRIDIAG((k_tridiag<6, float>), 6, float);
-    else SG_TRIDIAG((k_tridiag<20, float>), 20, float);
+  if (which != 0 && p.count > 0 && prec) {
+    if (t.small) SG_TRIDIAG((k_tridiag<6, float>), 6, float, 0);
+    else SG_TRIDIAG((k_tridiag<20, float>), 20, float, 0);
   }
 #undef SG_TRIDIAG
   if (which != 1)

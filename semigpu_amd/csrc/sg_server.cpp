@@ -149,6 +149,7 @@ struct Lane {
   Pinned<AtomDesc> atoms;
   Pinned<AtomDesc> partners;  // the Fock build's partner order per molecule (descriptor copies)
   Pinned<int> mask, order;
+  std::vector<int> tri_n;  // N of the slots in launch order (reduction buckets)
   Pinned<int2> pairs;
   Pinned<int> rb_act, rb_steps, rb_conv, rb_stat;
   Pinned<double> rb_e, rb_ecc;
@@ -260,6 +261,7 @@ int lane_create(const BatchDev& shared, Lane* L, int max_slots, int max_atoms, i
   HIP_TRY(L->partners.alloc(max_atoms));
   HIP_TRY(L->mask.alloc(ns));
   HIP_TRY(L->order.alloc(ns));
+  L->tri_n.assign(ns, 0);
   HIP_TRY(L->pairs.alloc(L->pair_cap));
   for (Pinned<int>* p : {&L->rb_act, &L->rb_steps, &L->rb_conv, &L->rb_stat}) HIP_TRY(p->alloc(ns));
   HIP_TRY(L->rb_e.alloc(ns));
@@ -439,6 +441,15 @@ int lane_launch(sg_server* S, Lane* L) {
     L->order_dirty = false;
   }
   it.order = L->order_dev;
+  {
+    // reduction buckets from the slots' N in launch order (free slots: 0)
+    int* nb = L->tri_n.data();
+    for (int i = 0; i < hw; ++i) {
+      const int sl = L->order[i];
+      nb[i] = L->slots[sl].used ? L->mols[sl].n_ao : 0;
+    }
+    set_tri_buckets(it, nb, hw);
+  }
   if (!L->fresh.empty()) {
     for (int sl : L->fresh) L->mask[sl] = 1;
     HIP_TRY(hipMemcpyAsync(L->d.mols, L->mols.p, sizeof(MolDesc) * hw, hipMemcpyHostToDevice, s));
