@@ -12,10 +12,11 @@ the batched FP64 result / step time, under the mixed policy (FP32 Fock and
 density product until |dE| < 1e-3 eV, then FP64; the eigensolver FP64).
 
 Printed JSON (one line): the driver's contract fields plus
-  roofline          the dominant kernel (Householder tridiagonalisation) rate:
-                    algorithmic flops per launch / average launch time, the
-                    time from HIP events around each launch on the library's
-                    stream during the timed steps
+  roofline          the dominant stage, the FP64 Householder reduction of each
+                    SCF iteration (head launch + on-chip bucket launches):
+                    algorithmic flops per pass / average pass time, the time
+                    from HIP events around its launches on the library's
+                    stream during the timed steps; parts: head and on chip
   fock_roofline     the Fock build's HBM rate (algorithmic bytes / kernel time)
   density_roofline  the density product's flop rate (algorithmic flops / time)
   resident_batch    the same SCF with the batch already resident (no setup)
@@ -65,7 +66,8 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 matrix
 BF16_PEAK_TFLOPS = 2500.0  # MI355X BF16 matrix (dense)
 METRIC = "SCF molecules/s batched mixed vs plain FP64 at |dE|<=1e-4 eV; p50/p99 latency"
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_tridiag_water4096_pmc.md")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_tridiag_reg_water4096_pmc.md")
+REG_ROWS = 256            # trailing block the on-chip reduction holds (kRegRows)
 # Compiled reference CPU path (g++ -O2, MKL dsyev, one core) over the oracle's
 # reference-algorithm mode on the same geometries and machine (full SCFs to
 # 1e-6 eV of pyridine, (H2O)8, (H2O)16, C10H22 and (H2O)32: ratios 1.85, 1.73,
@@ -107,10 +109,10 @@ def full_step(eng, packed, threshold: float, policy: str):
     return res, s32
 
 
-def pmc_traffic(key: str = "traffic_bytes_per_launch"):
-    """HBM bytes per tridiagonalisation launch (FP64 launch, or key
-    traffic_bytes_per_launch_fp32 for the FP32 one) from the committed PMC
-    summary."""
+def pmc_traffic(key: str = "traffic_bytes_per_pass"):
+    """HBM bytes from the committed PMC summary: per FP64 reduction pass
+    (traffic_bytes_per_pass; traffic_bytes_head / traffic_bytes_onchip for its
+    parts) or per Fock build (fock_traffic_bytes_per_launch)."""
     try:
         with open(PMC_SUMMARY) as f:
             m = re.search(key + r":\s*([0-9.eE+]+)", f.read())
@@ -364,26 +366,62 @@ def water_box(args):
     print(json.dumps(out), flush=True)
 
 
-def tridiag_roofline(stage, sizes, steps, s32, K, label):
-    """k_tridiag's rate over the timed steps: the (4/3) N^3 flops of each
-    molecule in each SCF step whose reduction ran in the FP64 launch, per
-    launch, over the average launch time (HIP events on the library stream)."""
+def tridiag_roofline(stage, sizes, steps, s32, K):
+    """The FP64 Householder reduction of each SCF iteration, the step's
+    dominant stage: k_tridiag on the first N - 256 columns of the molecules
+    above N = 256 (stage tridiag_head), then the on-chip k_tridiag_reg launches
+    (stage tridiag).  Work: (4/3) N^3 flops per molecule per FP64 step, split
+    as (4/3)(N^3 - 256^3) for the head and (4/3) min(N, 256)^3 on chip; time:
+    HIP events around each part on the library stream, per pass."""
     tri_ms, tri_cnt = stage["tridiag"]
+    head_ms, head_cnt = stage["tridiag_head"]
     t32_ms, t32_cnt = stage["tridiag_fp32"]
+    passes = max(1, tri_cnt)
     N3 = sizes.astype(np.float64) ** 3
+    R3 = np.minimum(sizes, REG_ROWS).astype(np.float64) ** 3
     steps64 = steps - s32 if t32_cnt else steps
-    flops_launch = float(np.sum(steps64 * (4.0 / 3.0) * N3)) * K / max(1, tri_cnt)
-    avg_ms = tri_ms / max(1, tri_cnt)
-    tflops = flops_launch / (avg_ms * 1e-3) / 1e12
+    head_flops = float(np.sum(steps64 * (4.0 / 3.0) * (N3 - R3))) * K / passes
+    chip_flops = float(np.sum(steps64 * (4.0 / 3.0) * R3)) * K / passes
+    head_avg = head_ms / passes
+    chip_avg = tri_ms / passes
+    pass_ms = head_avg + chip_avg
+    tflops = (head_flops + chip_flops) / (pass_ms * 1e-3) / 1e12
     traffic = pmc_traffic()
-    return {"kernel": label, "bound": "mfma", "achieved": round(tflops, 4),
-            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(tflops / FP64_PEAK_TFLOPS, 6), "traffic": traffic,
-            "avg_launch_ms": round(avg_ms, 4), "flops_per_launch": flops_launch,
-            "work": "(4/3) N^3 flops per molecule per FP64 reduction, per launch",
-            "traffic_rate_GBs": (round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None),
-            "traffic_frac_of_hbm": (round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
-                                    if traffic else None)}
+    traffic_head = pmc_traffic("traffic_bytes_head")
+    traffic_chip = pmc_traffic("traffic_bytes_onchip")
+
+    def rate(b, ms):
+        return round(b / (ms * 1e-3) / 1e9, 1) if b and ms > 0 else None
+
+    return {"kernel": "FP64 Householder reduction pass: k_tridiag<6, double> (first N-256 "
+                      "columns, N > 256) + k_tridiag_reg (8 size buckets, trailing block on chip)",
+            "bound": "mfma", "achieved": round(tflops, 4), "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(tflops / FP64_PEAK_TFLOPS, 6), "traffic": traffic,
+            "avg_launch_ms": round(pass_ms, 4), "flops_per_launch": head_flops + chip_flops,
+            "work": "(4/3) N^3 flops per molecule per FP64 reduction, per pass (one pass per "
+                    "SCF iteration: the head launch and the bucket launches)",
+            "traffic_rate_GBs": rate(traffic, pass_ms),
+            "traffic_frac_of_hbm": (round(rate(traffic, pass_ms) / HBM_PEAK_GBS, 4)
+                                    if traffic else None),
+            "parts": {
+                "head": {"kernel": "k_tridiag<6, double>", "avg_ms": round(head_avg, 4),
+                         "flops": head_flops,
+                         "tflops": round(head_flops / (head_avg * 1e-3) / 1e12, 4)
+                         if head_avg > 0 else None,
+                         "traffic": traffic_head, "traffic_rate_GBs": rate(traffic_head, head_avg),
+                         "bound": "hbm (the matrix-vector product re-reads the trailing block "
+                                  "once per column)"},
+                "on_chip": {"kernel": "k_tridiag_reg<NT, NW, NR> (8 launches)",
+                            "avg_ms": round(chip_avg, 4), "flops": chip_flops,
+                            "tflops": round(chip_flops / (chip_avg * 1e-3) / 1e12, 4)
+                            if chip_avg > 0 else None,
+                            "frac_of_fp64_peak": round(chip_flops / (chip_avg * 1e-3) / 1e12
+                                                       / FP64_PEAK_TFLOPS, 5)
+                            if chip_avg > 0 else None,
+                            "traffic": traffic_chip,
+                            "bound": "latency: three workgroup barriers and their dependent "
+                                     "LDS and cross-lane reductions per column, one molecule "
+                                     "per CU above N = 160"}}}
 
 
 def single_molecule(eng, repeats: int, threshold: float):
@@ -505,7 +543,7 @@ def main():
     conv = np.array([r["converged"] for r in res])
     s32 = s32.astype(np.float64)
     K = max(1, args.steps)
-    roofline = tridiag_roofline(stage, sizes, steps, s32, K, "k_tridiag<6, double>")
+    roofline = tridiag_roofline(stage, sizes, steps, s32, K)
 
     T = sizes * (sizes + 1) / 2
     I = np.array([pair_block_doubles(m) for m in mols], dtype=np.float64)

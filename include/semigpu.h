@@ -142,12 +142,8 @@ int sg_scf_run(sg_batch* b, double threshold_ev, int max_steps, sg_scf_result* r
  *                          with bisection to 1e-10 ||T|| and one
  *                          inverse-iteration solve (factors stored in FP32).
  *                          Opt-in: it departs from an FP64 eigensolver.
- *                          Batches above 512 molecules reduce their FP32
- *                          molecules in a launch of their own, whose sums run
- *                          in another order: such a batch agrees with its
- *                          molecules alone to the SCF threshold, not bit for
- *                          bit.  The other policies are bit-identical in any
- *                          batch. */
+ *                          Every policy gives a molecule the same result alone
+ *                          and in any batch, bit for bit. */
 #define SG_POLICY_FP64 0
 #define SG_POLICY_MIXED 1
 #define SG_POLICY_MIXED_BF16 2
@@ -292,12 +288,13 @@ double sg_server_clock(const sg_server* srv);
 
 /* Per-stage GPU time (HIP events on the context stream), milliseconds summed
  * since the last reset; stage order: two_center, hcore, guess, fock, tridiag,
- * bisect, invit, backtransform (the eigensolver's four kernels), density,
- * energy, control, tridiag_fp32.  tridiag is the FP64 reduction launch of each
- * eigensolve (with the FP32 molecules too when every workgroup is resident at
- * once); tridiag_fp32 the separate FP32 launch of the mixed policy's larger
- * batches.  counts[] gets the number of launches timed. */
-#define SG_N_STAGES 12
+ * bisect, invit, backtransform (the eigensolver's kernels), density, energy,
+ * control, tridiag_fp32, tridiag_head.  tridiag is the on-chip FP64 reduction
+ * of each eigensolve (its size-bucketed launches, plus the multi-workgroup
+ * molecules); tridiag_head the first N - 256 columns of the molecules above
+ * N = 256, which precede it; tridiag_fp32 the FP32 reduction of the
+ * mixed_eig32 policy.  counts[] gets the number of passes timed. */
+#define SG_N_STAGES 13
 /* One plain device-to-device copy kernel (k_calibration_copy) over freshly
  * allocated buffers: reads and writes exactly `bytes` (a multiple of 16).  No
  * counterpart in the reference: it is the known-size kernel that hardware byte

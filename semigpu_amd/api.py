@@ -24,8 +24,8 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include", "semigpu.h")
 
 SG_HCORE, SG_FOCK, SG_DENSITY = 0, 1, 2
 STAGES = ("two_center", "hcore", "guess", "fock", "tridiag", "bisect", "invit", "backtransform",
-          "density", "energy", "control", "tridiag_fp32")
-EIGH_STAGES = ("tridiag", "tridiag_fp32", "bisect", "invit", "backtransform")
+          "density", "energy", "control", "tridiag_fp32", "tridiag_head")
+EIGH_STAGES = ("tridiag_head", "tridiag", "tridiag_fp32", "bisect", "invit", "backtransform")
 
 # Every function declared in include/semigpu.h (checked by tests/test_abi.py).
 EXPORTS = (

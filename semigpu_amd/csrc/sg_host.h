@@ -22,9 +22,9 @@ int fail(int code, const std::string& msg);
 
 enum {
   ST_TWO_CENTER = 0, ST_HCORE, ST_GUESS, ST_FOCK, ST_TRIDIAG, ST_BISECT, ST_INVIT, ST_BACK,
-  ST_DENSITY, ST_ENERGY, ST_CONTROL, ST_TRIDIAG32
+  ST_DENSITY, ST_ENERGY, ST_CONTROL, ST_TRIDIAG32, ST_TRIDIAG_HEAD
 };
-static_assert(ST_TRIDIAG32 + 1 == SG_N_STAGES, "stage list and SG_N_STAGES disagree");
+static_assert(ST_TRIDIAG_HEAD + 1 == SG_N_STAGES, "stage list and SG_N_STAGES disagree");
 
 int check_launch(const char* what);
 bool supported_element(int z);
@@ -99,8 +99,13 @@ struct StageTimer {
 inline void scf_iteration(sg_context* c, const BatchDev& d, hipStream_t s, double threshold,
                           int max_steps, double switch_ev, int fp32_cap);
 
-// The eigensolver's four kernels, each timed as its own stage.
+// The eigensolver's kernels, each timed as its own stage (the reduction as
+// its head, on-chip and FP32 launches).
 inline void timed_eigh(sg_context* c, const BatchDev& d, hipStream_t s, bool all) {
+  if (d.tri_head > 0) {
+    StageTimer t(c, ST_TRIDIAG_HEAD, s);
+    launch_eigh_tridiag(d, s, all, 2);
+  }
   {
     StageTimer t(c, ST_TRIDIAG, s);
     launch_eigh_tridiag(d, s, all, 0);

@@ -221,9 +221,11 @@ int64_t diis_part_doubles(int n_mol);
 // Eigensolver of F (all = every eigenpair, else the selected subspace of the
 // active molecules): tridiagonalisation, bisection, inverse iteration,
 // back-transformation; launch_eigh runs the four in order.  The reduction's
-// launches: which = 0 the FP64 ones and the multi-workgroup molecules, 1 the
-// FP32 one of the mixed_eig32 policy (tridiag_fp32_launch: whether there is
-// one), -1 all.
+// launches: which = 0 the on-chip FP64 ones (k_tridiag_reg) and the
+// multi-workgroup molecules, 2 the head of the molecules above kRegRows
+// (k_tridiag, their first N - kRegRows columns; it must precede 0), 1 the FP32
+// one of the mixed_eig32 policy (tridiag_fp32_launch: whether there is one),
+// -1 all.
 bool tridiag_fp32_launch(const BatchDev& b, bool all);
 void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all, int which);
 void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all);

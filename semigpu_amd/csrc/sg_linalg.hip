@@ -1570,15 +1570,15 @@ void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all, int which) 
 #define SG_TRIDIAG(KERNEL, TM, R, ROWS)                                                \
   KERNEL<<<p.count, tridiag_threads<TM, R>(), tridiag_lds<TM, R>(p.max_n), s>>>(       \
       b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status, p.order, prec, ROWS)
-  if (which != 1 && p.count > 0) {
-    if (b.tri_head > 0) {
-      // only the first tri_head molecules of the part have N > kRegRows
-      Part h = p;
-      h.count = b.tri_head;
-      const Part& p = h;
-      if (t.small) SG_TRIDIAG((k_tridiag<6, double>), 6, double, kRegRows);
-      else SG_TRIDIAG((k_tridiag<20, double>), 20, double, kRegRows);
-    }
+  if ((which == -1 || which == 2) && p.count > 0 && b.tri_head > 0) {
+    // only the first tri_head molecules of the part have N > kRegRows
+    Part h = p;
+    h.count = b.tri_head;
+    const Part& p = h;
+    if (t.small) SG_TRIDIAG((k_tridiag<6, double>), 6, double, kRegRows);
+    else SG_TRIDIAG((k_tridiag<20, double>), 20, double, kRegRows);
+  }
+  if ((which == -1 || which == 0) && p.count > 0) {
     const int* st = b.tri_start;
 #define SG_REG(S, NT, NW, NR)                                                             \
     if (st[S + 1] > st[S])                                                                 \
@@ -1593,12 +1593,12 @@ void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all, int which) 
     SG_REG(7, 1, 1, 1);
 #undef SG_REG
   }
-  if (which != 0 && p.count > 0 && prec) {
+  if ((which == -1 || which == 1) && p.count > 0 && prec) {
     if (t.small) SG_TRIDIAG((k_tridiag<6, float>), 6, float, 0);
     else SG_TRIDIAG((k_tridiag<20, float>), 20, float, 0);
   }
 #undef SG_TRIDIAG
-  if (which != 1)
+  if (which == -1 || which == 0)
     for (int i = 0; i < b.n_large; ++i)
       launch_tridiag_large(b, s, b.large_idx[i], b.large_mols[i], act);
 }
