@@ -335,18 +335,44 @@ def water_box(args):
     out = {"workload": "water box, 1000 H2O on a 10x10x10 lattice (3.1 A, +-0.2 A jitter), "
                        "N = %d, SCF to %g eV" % (b.n_ao(0), args.threshold)}
     eng.timing(True)
-    for pol in ("fp64", "mixed"):
+    b.record_energies(args.box_max_steps + 1)
+    hist = {}
+    for pol in ("fp64", "mixed", "mixed_bf16"):
         eng.timing_reset()
         t = time.perf_counter()
         r = b.scf(threshold=args.threshold, max_steps=args.box_max_steps, policy=pol)[0]
         dt = time.perf_counter() - t
         st = eng.timing_read()
+        hist[pol] = b.energy_history(0)
         out[pol] = {"seconds": round(dt, 3), "steps": r["steps"], "converged": r["converged"],
-                    "e_tot_eV": r["e_tot"],
+                    "reduced_steps": int(b.fp32_steps()[0]), "e_tot_eV": r["e_tot"],
                     "stage_ms": {k: round(v[0], 2) for k, v in st.items()},
                     "tridiag_ms_per_iteration": round(st["tridiag"][0] / max(1, st["tridiag"][1]), 2)}
         print("water box %s: %d steps, %.2f s" % (pol, r["steps"], dt), file=sys.stderr, flush=True)
     out["mixed_minus_fp64_eV"] = out["mixed"]["e_tot_eV"] - out["fp64"]["e_tot_eV"]
+    out["mixed_bf16_minus_fp64_eV"] = out["mixed_bf16"]["e_tot_eV"] - out["fp64"]["e_tot_eV"]
+    # per-iteration |E_el(k) - E_el(k) of the FP64 oracle| (committed fixture,
+    # the oracle's SCF of the same box), for the FP64 and mixed SCFs
+    ref = None
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "water_box_1000_oracle.json")) as f:
+            ref = json.load(f)
+    except OSError:
+        pass
+    per_it = {}
+    for pol, h in hist.items():
+        if ref is not None:
+            eo = np.asarray(ref["energies_el"])
+            k = min(len(eo), len(h))
+            per_it[pol] = [float(abs(h[i] - eo[i])) for i in range(k)]
+        else:
+            k = min(len(hist["fp64"]), len(h))
+            per_it[pol] = [float(abs(h[i] - hist["fp64"][i])) for i in range(k)]
+    out["per_iteration_abs_dE_el_eV"] = per_it
+    out["per_iteration_reference"] = ("FP64 oracle SCF of the same box "
+                                      "(tests/golden/water_box_1000_oracle.json)" if ref is not None
+                                      else "the device FP64 SCF (oracle fixture absent)")
+    b.record_energies(0)
     if args.diis:
         b.set_diis(args.diis)
         for pol in ("fp64", "mixed"):

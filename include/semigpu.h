@@ -169,6 +169,15 @@ int sg_scf_set_damping(sg_batch* b, double beta);
  * converged molecule keeps the Fock matrix of its final density; for the
  * server, sg_server_set_diis. */
 int sg_scf_set_diis(sg_batch* b, int depth);
+/* Per-iteration energies (opt-in; no counterpart in the reference, whose SCF
+ * keeps only the last two energies, scfcalculation.cpp:82-146).  With cap > 0,
+ * the batch's later sg_scf_run / sg_scf_run_policy calls record every
+ * molecule's E_el of the guess (entry 0) and of each step k < cap (entry k);
+ * cap = 0 frees the record.  It takes cap doubles per molecule. */
+int sg_scf_record_energies(sg_batch* b, int cap);
+/* The record of molecule mol into out (cap doubles): E_el(0) .. E_el(steps),
+ * *count = min(steps + 1, cap) entries; SG_ERR_ORDER without a record. */
+int sg_scf_energy_history(const sg_batch* b, int mol, double* out, int* count);
 /* FP32 steps of each molecule in the last sg_scf_run_policy (0 under
  * SG_POLICY_FP64); steps32 gets n_mol entries.  No counterpart in the
  * reference: it splits the steps for per-precision rates. */

@@ -39,6 +39,7 @@ EXPORTS = (
     "sg_server_create", "sg_server_destroy", "sg_server_submit", "sg_server_step",
     "sg_server_poll", "sg_server_clock", "sg_server_set_diis", "sg_ctx_set_large_basis", "sg_server_create_classes",
     "sg_heat_of_formation", "sg_ctx_set_graphs", "sg_eigh_noalloc", "sg_gradient",
+    "sg_scf_record_energies", "sg_scf_energy_history",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -94,6 +95,8 @@ def load_library(path: str = LIB_PATH):
     L.sg_ctx_set_large_basis.argtypes = [_vp, ctypes.c_int]
     L.sg_ctx_set_graphs.argtypes = [_vp, ctypes.c_int]
     L.sg_gradient.argtypes = [_vp, _dp]
+    L.sg_scf_record_energies.argtypes = [_vp, ctypes.c_int]
+    L.sg_scf_energy_history.argtypes = [_vp, ctypes.c_int, _dp, _ip]
     L.sg_eigh_noalloc.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]
     L.sg_heat_of_formation.argtypes = [ctypes.c_int, _ip, ctypes.c_double, _dp]
     L.sg_batch_create.argtypes = [_vp, ctypes.c_int, _ip, _ip, _dp, ctypes.POINTER(_vp)]
@@ -385,6 +388,20 @@ class Batch:
         out = np.zeros(3 * int(self.offsets[-1]))
         _check(self.L.sg_gradient(self.h, _d(out)))
         return out.reshape(-1, 3)
+
+    def record_energies(self, cap: int = 101):
+        """Record every molecule's E_el per SCF iteration in later scf() calls
+        (sg_scf_record_energies): the guess and steps 1 .. cap - 1; 0 stops."""
+        self.cap = int(cap)
+        _check(self.L.sg_scf_record_energies(self.h, self.cap))
+
+    def energy_history(self, mol: int) -> np.ndarray:
+        """E_el (eV) of molecule mol after the guess and each step of the last
+        scf(): an array of min(steps + 1, cap) entries."""
+        out = np.zeros(max(1, getattr(self, "cap", 0)))
+        n = ctypes.c_int(0)
+        _check(self.L.sg_scf_energy_history(self.h, int(mol), _d(out), ctypes.byref(n)))
+        return out[: n.value].copy()
 
     def fp32_steps(self) -> np.ndarray:
         """FP32 steps of each molecule in the last scf() (0 for policy fp64)."""

@@ -50,6 +50,7 @@ struct sg_batch {
   std::vector<MolDesc> large_mols;  // their descriptors
   size_t dev_bytes = 0;             // size of dev_mem (returned to the context's cache)
   hipGraphExec_t iter_exec = nullptr;  // captured SCF iteration
+  void* hist_mem = nullptr;             // per-iteration energies (sg_scf_record_energies)
   double g_threshold = -1.0, g_switch = -1.0;
   int g_max_steps = -1, g_mixed = -1;
   double g_damping = -1.0;
@@ -573,6 +574,7 @@ int sg_batch_destroy(sg_batch* b) {
   if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
   if (b->dev_mem) ctx_release(b->ctx, b->dev_mem, b->dev_bytes);
   if (b->diis_mem) (void)hipFree(b->diis_mem);
+  if (b->hist_mem) (void)hipFree(b->hist_mem);
   delete b;
   return SG_OK;
 }
@@ -738,6 +740,47 @@ int sg_scf_set_damping(sg_batch* b, double beta) {
   if (!b) return fail(SG_ERR_INVALID, "null batch");
   if (!(beta >= 0.0 && beta < 1.0)) return fail(SG_ERR_INVALID, "damping must lie in [0, 1)");
   b->d.damping = beta;
+  return SG_OK;
+}
+
+int sg_scf_record_energies(sg_batch* b, int cap) {
+  if (!b) return fail(SG_ERR_INVALID, "null batch");
+  if (cap < 0 || cap > 100000) return fail(SG_ERR_INVALID, "sg_scf_record_energies: cap out of range");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  // the captured iteration holds the old record pointer
+  if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
+  b->iter_exec = nullptr;
+  if (b->hist_mem) (void)hipFree(b->hist_mem);
+  b->hist_mem = nullptr;
+  b->d.e_hist = nullptr;
+  b->d.e_hist_cap = 0;
+  if (cap == 0) return SG_OK;
+  const size_t bytes = sizeof(double) * (size_t)cap * (size_t)b->d.n_mol;
+  hipError_t e = hipMalloc(&b->hist_mem, bytes);
+  if (e != hipSuccess) {
+    b->hist_mem = nullptr;
+    return fail(SG_ERR_HIP, std::string("hipMalloc for the energy record: ") + hipGetErrorString(e));
+  }
+  HIP_TRY(hipMemset(b->hist_mem, 0, bytes));
+  b->d.e_hist = (double*)b->hist_mem;
+  b->d.e_hist_cap = cap;
+  return SG_OK;
+}
+
+int sg_scf_energy_history(const sg_batch* b, int mol, double* out, int* count) {
+  if (!b || !out || !count) return fail(SG_ERR_INVALID, "null argument");
+  if (mol < 0 || mol >= b->d.n_mol) return fail(SG_ERR_INVALID, "bad molecule index");
+  if (!b->d.e_hist) return fail(SG_ERR_ORDER, "sg_scf_energy_history: no record (sg_scf_record_energies)");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  hipStream_t s = b->ctx->stream;
+  const int cap = b->d.e_hist_cap;
+  int steps = 0;
+  HIP_TRY(hipMemcpyAsync(out, b->d.e_hist + (size_t)mol * cap, sizeof(double) * cap,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&steps, b->d.steps + mol, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  *count = std::min(steps + 1, cap);
   return SG_OK;
 }
 

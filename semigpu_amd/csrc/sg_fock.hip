@@ -339,12 +339,18 @@ __global__ void k_scf_init(int n_mol, int max_steps, int policy, const double* _
                            int* __restrict__ status, int* __restrict__ n_active,
                            int* __restrict__ prec, int* __restrict__ prev_prec,
                            int* __restrict__ steps32, double* __restrict__ de_prev,
-                           const int* __restrict__ mask) {
+                           const int* __restrict__ mask, double* __restrict__ e_hist,
+                           int e_hist_cap) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m == 0 && !mask) *n_active = 0;
   if (m >= n_mol) return;
   if (mask && !mask[m]) return;
   e_el[m] = e0[m];
+  if (e_hist) {
+    double* h = e_hist + (int64_t)m * e_hist_cap;
+    h[0] = e0[m];
+    for (int k = 1; k < e_hist_cap; ++k) h[k] = 0.0;
+  }
   steps[m] = 0;
   if (steps32) steps32[m] = 0;
   de_prev[m] = INFINITY;
@@ -372,7 +378,8 @@ __global__ void k_scf_update(int n_mol, double threshold, int max_steps, double 
                              int* __restrict__ active, int* __restrict__ converged,
                              const int* __restrict__ status, int* __restrict__ n_active,
                              int* __restrict__ prec, int* __restrict__ prev_prec,
-                             int* __restrict__ steps32, double* __restrict__ de_prev) {
+                             int* __restrict__ steps32, double* __restrict__ de_prev,
+                             double* __restrict__ e_hist, int e_hist_cap) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n_mol || !active[m]) return;
   if (status[m]) {
@@ -383,6 +390,7 @@ __global__ void k_scf_update(int n_mol, double threshold, int max_steps, double 
   e_el[m] = e;
   const int st = steps[m] + 1;
   steps[m] = st;
+  if (e_hist && st < e_hist_cap) e_hist[(int64_t)m * e_hist_cap + st] = e;
   const double de = fabs(old - e);
   const int step_prec = prec[m];
   if (steps32 && step_prec) steps32[m] += 1;
@@ -504,7 +512,7 @@ void launch_scf_init(const BatchDev& b, hipStream_t s, int max_steps, int policy
   k_scf_init<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(b.n_mol, max_steps, policy, b.e_new, b.e_el,
                                                       b.steps, b.active, b.converged, b.status,
                                                       b.n_active, b.prec, b.prev_prec, b.steps32,
-                                                      b.de_prev, mask);
+                                                      b.de_prev, mask, b.e_hist, b.e_hist_cap);
 }
 
 void launch_set_prec(const BatchDev& b, hipStream_t s, int v, const int* mask) {
@@ -517,7 +525,7 @@ void launch_scf_update(const BatchDev& b, hipStream_t s, double threshold, int m
                                                         fp32_cap, b.e_new, b.e_el, b.steps,
                                                         b.active, b.converged, b.status, b.n_active,
                                                         b.prec, b.prev_prec, b.steps32,
-                                                        b.de_prev);
+                                                        b.de_prev, b.e_hist, b.e_hist_cap);
 }
 
 }  // namespace sg

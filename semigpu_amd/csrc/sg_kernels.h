@@ -63,6 +63,10 @@ struct BatchDev {
   double* e_cc = nullptr;    // per molecule
   double* e_el = nullptr;    // current electronic energy
   double* e_new = nullptr;   // energy of the step just computed
+  // opt-in per-iteration record (sg_scf_record_energies): E_el of the guess
+  // and of every step k < e_hist_cap at e_hist[m * e_hist_cap + k]
+  double* e_hist = nullptr;
+  int e_hist_cap = 0;
   double* e_atom = nullptr;  // per atom: its share of sum P (H + F), written by the Fock build
   int* steps = nullptr;
   int* active = nullptr;     // 1 while the SCF of the molecule iterates

@@ -136,6 +136,35 @@ def test_water_clusters_against_oracle(engine):
         assert abs(r["e_tot"] - o["e_tot"]) <= 1e-6
 
 
+def test_energy_history_against_oracle(engine):
+    """The per-iteration record (sg_scf_record_energies) against the oracle's
+    SCF energies step by step (scfcalculation.cpp:66-146: E_0 from the guess
+    Fock matrix, then one energy per step): same number of steps and every
+    E_el(k) within 1e-6 eV, in one batch; under the mixed policy the record
+    has one entry per step and its last FP64 steps meet the oracle's
+    converged energy within 1e-4 eV."""
+    import pyoracle
+    mols = [geometry.pyridine(), geometry.water_cluster(8, geometry.SplitMix64(8)),
+            geometry.alkane(10)]
+    b = engine.batch(mols)
+    b.record_energies(101)
+    res = b.scf(threshold=1e-6)
+    hist = [b.energy_history(i) for i in range(len(mols))]
+    resm = b.scf(threshold=1e-6, policy="mixed")
+    histm = [b.energy_history(i) for i in range(len(mols))]
+    b.close()
+    for i, mol in enumerate(mols):
+        o = pyoracle.Molecule(mol).scf(threshold=1e-6, mode="nddo")
+        eo = np.asarray(o["energies"])
+        d = np.abs(hist[i] - eo).max() if len(hist[i]) == len(eo) else np.inf
+        say("history", i, len(hist[i]), len(eo), d)
+        assert res[i]["steps"] == o["steps"] and len(hist[i]) == o["steps"] + 1
+        assert d <= 1e-6
+        assert hist[i][-1] == res[i]["e_el"]
+        assert len(histm[i]) == resm[i]["steps"] + 1
+        assert abs(histm[i][-1] - o["e_el"]) <= 1e-4
+
+
 def test_unsupported_element_rejected(engine):
     with pytest.raises(api.SemiGpuError, match="Z=9"):
         engine.batch([[(9, 0.0, 0.0, 0.0), (1, 0.0, 0.0, 0.9)]])
