@@ -67,6 +67,7 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 matrix
 BF16_PEAK_TFLOPS = 2500.0  # MI355X BF16 matrix (dense)
 METRIC = "SCF molecules/s batched mixed vs plain FP64 at |dE|<=1e-4 eV; p50/p99 latency"
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_tridiag_reg_water4096_pmc.md")
+MFMA_SUMMARY = os.path.join(ROOT, "profiles", "k_density_water4096_mfma.md")
 REG_ROWS = 256            # trailing block the on-chip reduction holds (kRegRows)
 # Compiled reference CPU path (g++ -O2, MKL dsyev, one core) over the oracle's
 # reference-algorithm mode on the same geometries and machine (full SCFs to
@@ -109,12 +110,13 @@ def full_step(eng, packed, threshold: float, policy: str):
     return res, s32
 
 
-def pmc_traffic(key: str = "traffic_bytes_per_pass"):
-    """HBM bytes from the committed PMC summary: per FP64 reduction pass
+def pmc_traffic(key: str = "traffic_bytes_per_pass", path: str = None):
+    """A figure from a committed PMC summary: HBM bytes per FP64 reduction pass
     (traffic_bytes_per_pass; traffic_bytes_head / traffic_bytes_onchip for its
-    parts) or per Fock build (fock_traffic_bytes_per_launch)."""
+    parts) or per Fock build (fock_traffic_bytes_per_launch), or the density
+    product's matrix-core utilisation (density_mfma_busy_frac, MFMA_SUMMARY)."""
     try:
-        with open(PMC_SUMMARY) as f:
+        with open(path or PMC_SUMMARY) as f:
             m = re.search(key + r":\s*([0-9.eE+]+)", f.read())
         return float(m.group(1)) if m else None
     except OSError:
@@ -605,6 +607,8 @@ def main():
     den_roof["unit"] = "TFLOP/s"
     den_roof["peak"] = round((f64_flops + red_flops) / busy / 1e12, 3) if busy else None
     den_roof["frac"] = round(busy / (den_avg_ms * 1e-3), 6)
+    # the matrix cores' busy cycles over the dispatch, from the committed counters
+    den_roof["mfma_busy_frac_pmc"] = pmc_traffic("density_mfma_busy_frac", MFMA_SUMMARY)
     stage_ms = {k: round(v[0] / K, 3) for k, v in stage.items()}
     stage_ms["eigh"] = round(sum(stage_ms[k] for k in api.EIGH_STAGES), 3)
 
