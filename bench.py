@@ -505,6 +505,8 @@ def main():
                     help="more arrival rates, RATE:SECONDS[,RATE:SECONDS...], reported under "
                          "serving_rates (e.g. 1000:1,5000:1)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shapes of the reduction buckets")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 1 GiB device copy (PMC byte-counter calibration)")
     ap.add_argument("--policy", default="mixed", choices=sorted(api.POLICIES))
@@ -549,6 +551,11 @@ def main():
     eng = api.Engine(0)
     if args.calib_copy:
         calib_copy(eng)
+    # launch shapes of the on-chip reduction's size buckets, picked once per
+    # device before the clock (the shapes give the same results bit for bit)
+    t = time.perf_counter()
+    tune = eng.autotune(512) if not args.no_tune else eng.tri_shapes()
+    tune["seconds"] = round(time.perf_counter() - t, 3)
 
     for w in range(args.warmup):
         t = time.time()
@@ -707,6 +714,7 @@ def main():
                    "step": "batch submit (arrays) to last result: create, integrals, Hcore, "
                            "core-core, SCF, results, destroy",
                    "scf_threshold_eV": args.threshold},
+        "autotune": tune,
         "roofline": roofline,
         "fock_roofline": fock_roof,
         "density_roofline": den_roof,

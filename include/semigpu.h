@@ -56,6 +56,28 @@ int sg_ctx_destroy(sg_context* ctx);
  * basis a batch accepts is 8192.  No counterpart in the reference, whose
  * dsyev_ call (scfcalculation.cpp:169-203) takes any size on the host. */
 int sg_ctx_set_large_basis(sg_context* ctx, int n_min);
+/* Autotuner of the one-workgroup eigensolver's FP64 reduction (no
+ * counterpart in the reference).  The reduction holds each molecule's
+ * trailing block (up to 256 x 256) in the registers of one workgroup, launched
+ * in SG_TUNE_BUCKETS size buckets (N above 224, (192, 224], (160, 192],
+ * (128, 160], (96, 128], (64, 96], (32, 64], up to 32); a bucket can run in up
+ * to SG_TUNE_CANDIDATES launch shapes (wavefronts per molecule x register
+ * tiles per wavefront), which give the same results bit for bit.
+ * sg_ctx_autotune times every shape of every bucket on `molecules` random
+ * symmetric matrices of the bucket's largest N (best of two launches after a
+ * warmup) and makes the fastest the context's choice for batches and servers
+ * created afterwards (and the running servers' next iterations).
+ * ms[SG_TUNE_BUCKETS * SG_TUNE_CANDIDATES] gets the times (-1: no such shape)
+ * and choice[SG_TUNE_BUCKETS] the choices (either may be null).  Without a
+ * call the first shape of each bucket is used.  sg_ctx_set_tri_variant sets one
+ * bucket's shape; sg_ctx_tri_shapes reports the choices (variant[]), the
+ * number of shapes per bucket (count[]) and their wavefronts and register
+ * tiles (nw[], nr[], SG_TUNE_CANDIDATES per bucket). */
+#define SG_TUNE_BUCKETS 8
+#define SG_TUNE_CANDIDATES 3
+int sg_ctx_autotune(sg_context* ctx, int molecules, double* ms, int* choice);
+int sg_ctx_set_tri_variant(sg_context* ctx, int bucket, int variant);
+int sg_ctx_tri_shapes(const sg_context* ctx, int* variant, int* count, int* nw, int* nr);
 /* Whether sg_scf_run / sg_scf_run_policy capture one SCF iteration of a batch
  * as a HIP graph on first use and replay it (on, the default), or launch each
  * iteration's kernels directly (off: no capture and instantiation per batch,

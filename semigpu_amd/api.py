@@ -26,6 +26,8 @@ SG_HCORE, SG_FOCK, SG_DENSITY = 0, 1, 2
 STAGES = ("two_center", "hcore", "guess", "fock", "tridiag", "bisect", "invit", "backtransform",
           "density", "energy", "control", "tridiag_fp32", "tridiag_head")
 EIGH_STAGES = ("tridiag_head", "tridiag", "tridiag_fp32", "bisect", "invit", "backtransform")
+TUNE_BUCKETS = 8       # SG_TUNE_BUCKETS
+TUNE_CANDIDATES = 3    # SG_TUNE_CANDIDATES
 
 # Every function declared in include/semigpu.h (checked by tests/test_abi.py).
 EXPORTS = (
@@ -39,7 +41,8 @@ EXPORTS = (
     "sg_server_create", "sg_server_destroy", "sg_server_submit", "sg_server_step",
     "sg_server_poll", "sg_server_clock", "sg_server_set_diis", "sg_ctx_set_large_basis", "sg_server_create_classes",
     "sg_heat_of_formation", "sg_ctx_set_graphs", "sg_eigh_noalloc", "sg_gradient",
-    "sg_scf_record_energies", "sg_scf_energy_history",
+    "sg_scf_record_energies", "sg_scf_energy_history", "sg_ctx_autotune",
+    "sg_ctx_set_tri_variant", "sg_ctx_tri_shapes",
 )
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -96,6 +99,9 @@ def load_library(path: str = LIB_PATH):
     L.sg_ctx_set_graphs.argtypes = [_vp, ctypes.c_int]
     L.sg_gradient.argtypes = [_vp, _dp]
     L.sg_scf_record_energies.argtypes = [_vp, ctypes.c_int]
+    L.sg_ctx_autotune.argtypes = [_vp, ctypes.c_int, _dp, _ip]
+    L.sg_ctx_set_tri_variant.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+    L.sg_ctx_tri_shapes.argtypes = [_vp, _ip, _ip, _ip, _ip]
     L.sg_scf_energy_history.argtypes = [_vp, ctypes.c_int, _dp, _ip]
     L.sg_eigh_noalloc.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]
     L.sg_heat_of_formation.argtypes = [ctypes.c_int, _ip, ctypes.c_double, _dp]
@@ -237,6 +243,36 @@ class Engine:
         g = b.gradient()
         b.close()
         return g, r["e_tot"], bool(r["converged"] and r["status"] == 0)
+
+    def autotune(self, molecules: int = 512) -> dict:
+        """Times every launch shape of every size bucket of the on-chip FP64
+        reduction (sg_ctx_autotune) and keeps the fastest for later batches
+        and servers.  Returns {"ms": buckets x candidates (-1: none),
+        "choice": per bucket, "shapes": per bucket [(wavefronts, tiles per
+        wavefront), ...]}."""
+        nb, nc = TUNE_BUCKETS, TUNE_CANDIDATES
+        ms = np.zeros(nb * nc)
+        ch = np.zeros(nb, dtype=np.int32)
+        _check(self.L.sg_ctx_autotune(self.h, int(molecules), _d(ms), _i(ch)))
+        out = self.tri_shapes()
+        out["ms"] = ms.reshape(nb, nc).round(4).tolist()
+        return out
+
+    def tri_shapes(self) -> dict:
+        """The reduction buckets' launch shapes and the current choice."""
+        nb, nc = TUNE_BUCKETS, TUNE_CANDIDATES
+        var = np.zeros(nb, dtype=np.int32)
+        cnt = np.zeros(nb, dtype=np.int32)
+        nw = np.zeros(nb * nc, dtype=np.int32)
+        nr = np.zeros(nb * nc, dtype=np.int32)
+        _check(self.L.sg_ctx_tri_shapes(self.h, _i(var), _i(cnt), _i(nw), _i(nr)))
+        shapes = [[(int(nw[b * nc + c]), int(nr[b * nc + c])) for c in range(int(cnt[b]))]
+                  for b in range(nb)]
+        return {"choice": var.tolist(), "shapes": shapes}
+
+    def set_tri_variant(self, bucket: int, variant: int):
+        """Forces one reduction bucket's launch shape (sg_ctx_set_tri_variant)."""
+        _check(self.L.sg_ctx_set_tri_variant(self.h, int(bucket), int(variant)))
 
     def set_large_basis(self, n_min: int):
         """Molecules of later batches with N >= n_min take the multi-workgroup

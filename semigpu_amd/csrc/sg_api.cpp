@@ -246,6 +246,44 @@ int sg_ctx_set_large_basis(sg_context* c, int n_min) {
   return SG_OK;
 }
 
+int sg_ctx_autotune(sg_context* c, int molecules, double* ms, int* choice) {
+  if (!c) return fail(SG_ERR_INVALID, "null context");
+  if (molecules < 1 || molecules > 4096)
+    return fail(SG_ERR_INVALID, "sg_ctx_autotune: molecules must lie in [1, 4096]");
+  HIP_TRY(hipSetDevice(c->device));
+  double t[kTriBuckets * kTriCandidates];
+  int ch[kTriBuckets];
+  HIP_TRY(tri_autotune(c->stream, molecules, 2, t, ch));
+  for (int k = 0; k < kTriBuckets; ++k) c->tri_variant[k] = ch[k];
+  if (ms) std::copy(t, t + kTriBuckets * kTriCandidates, ms);
+  if (choice) std::copy(ch, ch + kTriBuckets, choice);
+  return SG_OK;
+}
+
+int sg_ctx_set_tri_variant(sg_context* c, int bucket, int variant) {
+  if (!c || bucket < 0 || bucket >= kTriBuckets) return fail(SG_ERR_INVALID, "bad bucket");
+  int count, nw[kTriCandidates], nr[kTriCandidates];
+  tri_candidates(bucket, &count, nw, nr);
+  if (variant < 0 || variant >= count) return fail(SG_ERR_INVALID, "bad launch shape for the bucket");
+  c->tri_variant[bucket] = variant;
+  return SG_OK;
+}
+
+int sg_ctx_tri_shapes(const sg_context* c, int* variant, int* count, int* nw, int* nr) {
+  if (!c) return fail(SG_ERR_INVALID, "null context");
+  for (int k = 0; k < kTriBuckets; ++k) {
+    if (variant) variant[k] = c->tri_variant[k];
+    int cnt, w[kTriCandidates], r[kTriCandidates];
+    tri_candidates(k, &cnt, w, r);
+    if (count) count[k] = cnt;
+    for (int j = 0; j < kTriCandidates; ++j) {
+      if (nw) nw[k * kTriCandidates + j] = w[j];
+      if (nr) nr[k * kTriCandidates + j] = r[j];
+    }
+  }
+  return SG_OK;
+}
+
 int sg_ctx_set_graphs(sg_context* c, int on) {
   if (!c) return fail(SG_ERR_INVALID, "null context");
   c->graphs = on != 0;
@@ -426,6 +464,7 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   d.n_pairs_all = (int)n_all;
   d.n_diag_tasks = (int)n_diag;
   d.large_min = std::min(ctx->large_min, kMaxBasis + 1);
+  for (int k = 0; k < kTriBuckets; ++k) d.tri_variant[k] = ctx->tri_variant[k];
 
   // one allocation, 256-byte aligned sub-buffers
   size_t off = 0;
@@ -684,6 +723,7 @@ int sg_eigh_noalloc(sg_context* c, int n, const double* a_dev, double* eigval_de
   d.max_n = n;
   d.max_n_small = n;
   set_tri_buckets(d, &n, 1);
+  for (int k = 0; k < kTriBuckets; ++k) d.tri_variant[k] = c->tri_variant[k];
   d.mols = (MolDesc*)base;
   int* order = (int*)(base + 256);
   d.order = order;

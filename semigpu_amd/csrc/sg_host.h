@@ -47,6 +47,7 @@ struct sg_context {
   bool timing = false;
   int large_min = sg::kMaxBasis + 1;  // batches created from now on
   bool graphs = true;                 // capture SCF iterations as HIP graphs
+  int tri_variant[sg::kTriBuckets] = {};  // launch shape per reduction bucket (autotuner)
   double ms[SG_N_STAGES] = {0};
   int counts[SG_N_STAGES] = {0};
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;

@@ -30,6 +30,10 @@ constexpr int kDiisMax = 8;   // largest DIIS depth
 constexpr int kTriBuckets = 8;
 constexpr int kRegRows = 256;
 constexpr int kTriBucketMin[kTriBuckets] = {224, 192, 160, 128, 96, 64, 32, 0};
+// Launch shapes a bucket can take (wavefronts per molecule, register tiles per
+// wavefront), all with the same arithmetic: the autotuner picks one per
+// bucket and device (sg_ctx_autotune), index 0 by default.
+constexpr int kTriCandidates = 3;
 
 struct BatchDev {
   int n_mol = 0, n_atoms = 0;
@@ -124,7 +128,18 @@ struct BatchDev {
   // of that part are bucket s; its first tri_head have N > kRegRows.
   int tri_start[kTriBuckets + 1] = {};
   int tri_head = 0;
+  // launch shape of each bucket (an index into its candidates)
+  int tri_variant[kTriBuckets] = {};
 };
+
+// Candidates of bucket s: *count of them, their (wavefronts, register tiles
+// per wavefront) into nw[] / nr[] (kTriCandidates entries).
+void tri_candidates(int s, int* count, int* nw, int* nr);
+// Times every candidate of every bucket on a synthetic batch of random
+// symmetric matrices of the bucket's largest N on stream s (best of `reps`
+// launches, milliseconds into ms[s * kTriCandidates + c], -1 where a bucket
+// has fewer candidates) and writes the fastest into choice[s].
+hipError_t tri_autotune(hipStream_t s, int molecules, int reps, double* ms, int* choice);
 
 // tri_start / tri_head from the N of the one-workgroup part's molecules in
 // launch order (non-increasing; free server slots count as N = 0).

@@ -43,6 +43,9 @@
 // factors (3 N x min(k,512), floats in FP32 steps) at 2 N^2, where FP32 steps
 // also put the FP32 copy of Y after the inverse iteration.
 // Golub & Van Loan, Matrix Computations (4th ed.) 5.1.7, 8.3, 8.4.
+#include <algorithm>
+#include <vector>
+
 #include "sg_common.h"
 #include "sg_kernels.h"
 
@@ -1520,6 +1523,24 @@ Part small_part(const BatchDev& b) {
 // its size bucket (k_tridiag_reg), after k_tridiag's first N - kRegRows
 // columns for the molecules above kRegRows.  FP32 steps of the mixed_eig32
 // policy: k_tridiag<., float>, its own launch.
+// Symmetric test matrices for the autotuner: entry (i, j) from a hash of
+// (molecule, max(i, j), min(i, j)) in [-0.5, 0.5), plus 0.1 i on the diagonal.
+__global__ void k_tune_fill(const MolDesc* __restrict__ mols, double* __restrict__ F) {
+  const MolDesc M = mols[blockIdx.x];
+  const int n = M.n_ao;
+  for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+    const int i = t / n, j = t % n, hi = i > j ? i : j, lo = i > j ? j : i;
+    uint64_t z = ((uint64_t)blockIdx.x << 40) ^ ((uint64_t)hi << 20) ^ (uint64_t)lo;
+    z += 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    z ^= z >> 31;
+    double v = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    if (i == j) v += 0.1 * i;
+    F[M.mat0 + t] = v;
+  }
+}
+
 void set_tri_buckets(BatchDev& d, const int* n_in_order, int count) {
   int s = 0;
   d.tri_start[0] = 0;
@@ -1534,12 +1555,139 @@ void set_tri_buckets(BatchDev& d, const int* n_in_order, int count) {
 
 // One bucket's launch: tiles per dimension, wavefronts, register tiles per
 // wavefront (k_tridiag_reg<NT, NW, NR>, MR = 32 NT >= the bucket's largest N).
+// Every shape has at least as many threads as rows (the product's row totals
+// and p.v then sum in the same order), so a bucket's candidates give the same
+// results bit for bit.
 template <int NT, int NW, int NR>
 static void tridiag_reg_launch(const BatchDev& b, hipStream_t s, const int* order, int count,
                                const int* act, const int* prec) {
   static_assert(32 * NT <= kRegRows, "bucket larger than the register block");
+  static_assert(64 * NW >= 32 * NT, "fewer threads than rows");
+  static_assert(NW * NR >= NT * (NT + 1) / 2, "tiles in LDS: the register budget spills");
   k_tridiag_reg<NT, NW, NR><<<count, 64 * NW, tridiag_reg_lds<NT, NW, NR>(), s>>>(
       b.mols, b.F, b.W, b.dT, b.eT, b.tau, act, b.status, order, prec);
+}
+
+// The candidates of each bucket (wavefronts, register tiles per wavefront);
+// the first is the default.
+namespace {
+struct TriShape {
+  int nw, nr;
+};
+constexpr TriShape kTriShapes[kTriBuckets][kTriCandidates] = {
+    {{12, 3}, {0, 0}, {0, 0}},   // N in (224, 256], and the tail of larger ones: NT = 8
+    {{10, 3}, {0, 0}, {0, 0}},   // (192, 224]: NT = 7
+    {{7, 3}, {11, 2}, {0, 0}},   // (160, 192]: NT = 6
+    {{5, 3}, {8, 2}, {0, 0}},    // (128, 160]: NT = 5
+    {{4, 3}, {5, 2}, {10, 1}},   // (96, 128]: NT = 4
+    {{2, 3}, {3, 2}, {6, 1}},    // (64, 96]: NT = 3
+    {{1, 3}, {2, 2}, {3, 1}},    // (32, 64]: NT = 2
+    {{1, 1}, {0, 0}, {0, 0}},    // up to 32: NT = 1
+};
+}  // namespace
+
+void tri_candidates(int s, int* count, int* nw, int* nr) {
+  int c = 0;
+  for (int k = 0; k < kTriCandidates; ++k) {
+    nw[k] = kTriShapes[s][k].nw;
+    nr[k] = kTriShapes[s][k].nr;
+    if (nw[k] > 0) c = k + 1;
+  }
+  *count = c;
+}
+
+static void tridiag_bucket(const BatchDev& b, hipStream_t s, int bucket, int variant,
+                           const int* order, int count, const int* act, const int* prec) {
+#define SG_SHAPE(NT, V, NW, NR)                                                  \
+  if (variant == V) {                                                            \
+    tridiag_reg_launch<NT, NW, NR>(b, s, order, count, act, prec);               \
+    return;                                                                      \
+  }
+  switch (bucket) {
+    case 0: SG_SHAPE(8, 0, 12, 3) break;
+    case 1: SG_SHAPE(7, 0, 10, 3) break;
+    case 2: SG_SHAPE(6, 0, 7, 3) SG_SHAPE(6, 1, 11, 2) break;
+    case 3: SG_SHAPE(5, 0, 5, 3) SG_SHAPE(5, 1, 8, 2) break;
+    case 4: SG_SHAPE(4, 0, 4, 3) SG_SHAPE(4, 1, 5, 2) SG_SHAPE(4, 2, 10, 1) break;
+    case 5: SG_SHAPE(3, 0, 2, 3) SG_SHAPE(3, 1, 3, 2) SG_SHAPE(3, 2, 6, 1) break;
+    case 6: SG_SHAPE(2, 0, 1, 3) SG_SHAPE(2, 1, 2, 2) SG_SHAPE(2, 2, 3, 1) break;
+    default: SG_SHAPE(1, 0, 1, 1) break;
+  }
+#undef SG_SHAPE
+  // a variant this bucket does not have: its default
+  if (variant != 0) tridiag_bucket(b, s, bucket, 0, order, count, act, prec);
+}
+
+hipError_t tri_autotune(hipStream_t s, int molecules, int reps, double* ms, int* choice) {
+  const int maxn = kRegRows;
+  const int64_t nn = (int64_t)maxn * maxn;
+  // one descriptor block per bucket size, every molecule of a bucket the
+  // bucket's largest N; matrices filled on the device from a hash of the index
+  double* F = nullptr;
+  double* W = nullptr;
+  double* vec = nullptr;
+  MolDesc* mols = nullptr;
+  hipError_t e = hipMalloc(&F, sizeof(double) * nn * molecules);
+  if (e == hipSuccess) e = hipMalloc(&W, sizeof(double) * 5 * nn * molecules);
+  if (e == hipSuccess) e = hipMalloc(&vec, sizeof(double) * 3 * (int64_t)maxn * molecules);
+  if (e == hipSuccess) e = hipMalloc(&mols, sizeof(MolDesc) * molecules);
+  hipEvent_t a = nullptr, z = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&z);
+  std::vector<MolDesc> h(molecules);
+  for (int bucket = 0; bucket < kTriBuckets && e == hipSuccess; ++bucket) {
+    const int n = bucket == 0 ? kRegRows : kTriBucketMin[bucket - 1];
+    for (int m = 0; m < molecules; ++m) {
+      h[m] = MolDesc{};
+      h[m].n_ao = n;
+      h[m].n_occ = n / 2;
+      h[m].mat0 = (int64_t)m * n * n;
+      h[m].vec0 = (int64_t)m * n;
+    }
+    e = hipMemcpyAsync(mols, h.data(), sizeof(MolDesc) * molecules, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) break;
+    k_tune_fill<<<molecules, 256, 0, s>>>(mols, F);
+    BatchDev b;
+    b.n_mol = molecules;
+    b.mols = mols;
+    b.F = F;
+    b.W = W;
+    b.dT = vec;
+    b.eT = vec + (int64_t)maxn * molecules;
+    b.tau = vec + 2 * (int64_t)maxn * molecules;
+    int count, nw[kTriCandidates], nr[kTriCandidates];
+    tri_candidates(bucket, &count, nw, nr);
+    double best = 1e300;
+    choice[bucket] = 0;
+    for (int c = 0; c < kTriCandidates; ++c) {
+      ms[bucket * kTriCandidates + c] = -1.0;
+      if (c >= count) continue;
+      double t_best = 1e300;
+      for (int r = 0; r < reps + 1 && e == hipSuccess; ++r) {
+        e = hipEventRecord(a, s);
+        tridiag_bucket(b, s, bucket, c, nullptr, molecules, nullptr, nullptr);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventRecord(z, s);
+        if (e == hipSuccess) e = hipEventSynchronize(z);
+        float t = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, a, z);
+        if (r > 0) t_best = std::min(t_best, (double)t);  // the first run warms up
+      }
+      ms[bucket * kTriCandidates + c] = t_best;
+      if (t_best < best) {
+        best = t_best;
+        choice[bucket] = c;
+      }
+    }
+  }
+  if (a) (void)hipEventDestroy(a);
+  if (z) (void)hipEventDestroy(z);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(F);
+  (void)hipFree(W);
+  (void)hipFree(vec);
+  (void)hipFree(mols);
+  return e;
 }
 
 namespace {
@@ -1580,18 +1728,9 @@ void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all, int which) 
   }
   if ((which == -1 || which == 0) && p.count > 0) {
     const int* st = b.tri_start;
-#define SG_REG(S, NT, NW, NR)                                                             \
-    if (st[S + 1] > st[S])                                                                 \
-      tridiag_reg_launch<NT, NW, NR>(b, s, p.order + st[S], st[S + 1] - st[S], act, prec)
-    SG_REG(0, 8, 12, 3);
-    SG_REG(1, 7, 10, 3);
-    SG_REG(2, 6, 7, 3);
-    SG_REG(3, 5, 5, 3);
-    SG_REG(4, 4, 4, 3);
-    SG_REG(5, 3, 2, 3);
-    SG_REG(6, 2, 1, 3);
-    SG_REG(7, 1, 1, 1);
-#undef SG_REG
+    for (int k = 0; k < kTriBuckets; ++k)
+      if (st[k + 1] > st[k])
+        tridiag_bucket(b, s, k, b.tri_variant[k], p.order + st[k], st[k + 1] - st[k], act, prec);
   }
   if ((which == -1 || which == 1) && p.count > 0 && prec) {
     if (t.small) SG_TRIDIAG((k_tridiag<6, float>), 6, float, 0);

@@ -425,6 +425,7 @@ int lane_launch(sg_server* S, Lane* L) {
   L->hw = hw;
   if (hw == 0) return SG_OK;
   BatchDev it = L->d;
+  for (int k = 0; k < kTriBuckets; ++k) it.tri_variant[k] = S->ctx->tri_variant[k];
   it.n_mol = hw;
   it.n_atoms = hw_atoms;
   it.max_n = max_n;
