@@ -288,13 +288,23 @@ k_tridiag_reg(const MolDesc* __restrict__ mols, const double* __restrict__ F,
     // v (0 at or before k, 1 at k + 1): this wavefront's copy, and the
     // reflector into row c of the work matrix
     // (entries before k were cleared at their own column)
+    {
+      // all loads first: the stores could alias them for the compiler
+      double xv[NJ];
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) {
-      const int i = k + lane + 64 * j;
-      if (i < MR) {
-        const double v = i > kn ? x[i] * sc : (i == kn ? 1.0 : 0.0);
-        vw[i] = v;
-        if (i > k && i % NW == wave) A[(int64_t)c * n + gbase + i] = v;
+      for (int j = 0; j < NJ; ++j) {
+        const int i = k + lane + 64 * j;
+        xv[j] = i < MR ? x[i] : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int i = k + lane + 64 * j;
+        if (i < MR) {
+          const double v = i > kn ? xv[j] * sc : (i == kn ? 1.0 : 0.0);
+          vw[i] = v;
+          // the reflector's 64-entry chunk j is stored by wavefront j % NW
+          if (i > k && j % NW == wave) A[(int64_t)c * n + gbase + i] = v;
+        }
       }
     }
     // p: row and column sums per tile
@@ -360,10 +370,19 @@ k_tridiag_reg(const MolDesc* __restrict__ mols, const double* __restrict__ F,
     for (int w = 0; w < NW; ++w) pv += pvw[w];
     const double al = -0.5 * tk * pv;
     // w = p + al v (0 at or before k): this wavefront's copy
+    {
+      double pv_[NJ], vv_[NJ];
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) {
-      const int i = k + lane + 64 * j;
-      if (i < MR) vw[MR + i] = i > k ? pbuf[i] + al * vw[i] : 0.0;
+      for (int j = 0; j < NJ; ++j) {
+        const int i = k + lane + 64 * j;
+        pv_[j] = i < MR ? pbuf[i] : 0.0;
+        vv_[j] = i < MR ? vw[i] : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int i = k + lane + 64 * j;
+        if (i < MR) vw[MR + i] = i > k ? pv_[j] + al * vv_[j] : 0.0;
+      }
     }
     // A22 -= v w^T + w v^T, and column k + 1 for the next step
 #pragma unroll
