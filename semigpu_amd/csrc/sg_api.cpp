@@ -458,6 +458,7 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   d.n_vec = vec0;
   d.n_blocks = blk0;
   d.max_n = max_n;
+  set_fock_lds(d, b->mols.data(), n_mol);
   d.n_xx = (int)n_xx;
   d.n_hx = (int)n_hx;
   d.n_hh = (int)n_hh;

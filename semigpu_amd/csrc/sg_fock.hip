@@ -18,6 +18,9 @@
 // F_BA.  J_A is reduced across the wave with a fixed butterfly, so every run
 // (alone or inside any batch) adds the same numbers in the same order: no
 // floating-point atomics anywhere.
+#include <algorithm>
+#include <type_traits>
+
 #include "sg_common.h"
 #include "sg_kernels.h"
 
@@ -189,6 +192,7 @@ __global__ void k_wdiag(int n_atoms, const AtomDesc* __restrict__ atoms,
   if (active && (!active[A.mol] || status[A.mol])) return;
   const MolDesc M = mols[A.mol];
   const int N = M.n_ao;
+  if (N <= kFockMolMaxN) return;  // k_fock_mol's molecules
   const double* p = P + M.mat0;
   const int sb = A.zn > 1 ? 10 : 1;
   for (int l = 0; l < sb; ++l) {
@@ -217,6 +221,7 @@ __global__ void __launch_bounds__(256) k_fock(int n_atoms, const AtomDesc* __res
   const AtomDesc A = atoms[ia];
   if (active && (!active[A.mol] || status[A.mol])) return;
   const MolDesc M = mols[A.mol];
+  if (M.n_ao <= kFockMolMaxN) return;  // k_fock_mol's molecules
   const double* h = H + M.mat0;
   const double* p = P + M.mat0;
   double* f = F + M.mat0;
@@ -234,7 +239,313 @@ __global__ void __launch_bounds__(256) k_fock(int n_atoms, const AtomDesc* __res
   if (lane == 0) e_atom[ia] = e;
 }
 
-// E_el = 1/2 sum of the atoms' shares written by k_fock, in atom order
+// ---- Fock build with one workgroup per molecule (N <= kFockMolMaxN) ----
+//
+// Every stored block (A, B), B.local <= A.local, is read once: by the lane
+// that takes partner B in the pass of row atom A.  That lane
+//   * adds sum_l (k|l) w_B(l) to its Coulomb accumulators of A (summed over
+//     the wavefront by a fixed butterfly once the row is done),
+//   * adds sum_k (k|l) w_A(k) to B's Coulomb sums in its wavefront's own copy
+//     in LDS (one lane per partner: no two lanes of a pass touch one entry),
+//   * forms the exchange block K_AB and writes F_AB and F_BA with their share
+//     of the electronic energy (K_AA of the one-center block goes to LDS).
+// Row atoms go to the wavefronts round robin, heavy partners first, then
+// hydrogens (each pass runs one block shape).  When every row is done the
+// wavefronts' copies are added in wavefront order and the diagonal blocks
+// written, so each sum runs in a fixed order and a molecule's result does not
+// depend on the rest of its launch.  The weights w (P_l, doubled for the
+// off-diagonal AO pairs of an atom) are gathered into LDS first.
+//
+// LDS slots: atom A's Coulomb entries start at 9 hv_before + local (10 for a
+// heavy atom, 1 for a hydrogen), its K_AA at 15 hv_before + local (16 or 1).
+__device__ __forceinline__ int jslot(const AtomDesc& A) { return 9 * A.hv_before + A.local; }
+__device__ __forceinline__ int kslot(const AtomDesc& A) { return 15 * A.hv_before + A.local; }
+
+// vector accesses of 8-byte (double) and 4-byte (float) aligned data
+typedef double dbl2u __attribute__((ext_vector_type(2), aligned(8)));
+typedef float flt4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef float flt2u __attribute__((ext_vector_type(2), aligned(4)));
+
+template <int C, typename T>
+__device__ __forceinline__ void load_run(const double* __restrict__ q, T (&v)[C]) {
+  if constexpr (C == 1) {
+    v[0] = (T)q[0];
+  } else {
+#pragma unroll
+    for (int i = 0; i < C / 2; ++i) {
+      const dbl2u u = *(const dbl2u*)(q + 2 * i);
+      v[2 * i] = (T)u.x;
+      v[2 * i + 1] = (T)u.y;
+    }
+  }
+}
+template <int C, typename T>
+__device__ __forceinline__ void load_run(const float* __restrict__ q, T (&v)[C]) {
+  if constexpr (C == 1) {
+    v[0] = (T)q[0];
+  } else {
+    static_assert(C == 10, "runs of 1 or 10");
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const flt4u u = *(const flt4u*)(q + 4 * i);
+      v[4 * i] = (T)u.x;
+      v[4 * i + 1] = (T)u.y;
+      v[4 * i + 2] = (T)u.z;
+      v[4 * i + 3] = (T)u.w;
+    }
+    const flt2u u = *(const flt2u*)(q + 8);
+    v[8] = (T)u.x;
+    v[9] = (T)u.y;
+  }
+}
+
+__device__ __forceinline__ float uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uniform(double v) {
+  const int64_t u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(u & 0xffffffff));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(u >> 32));
+  return __longlong_as_double((int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// One block (A, B) of row atom A, B.local <= A.local (see above).
+template <bool HA, bool HB, typename T, typename S>
+__device__ __forceinline__ void fock_pair(const AtomDesc& A, const AtomDesc& B, int N,
+                                          const S* __restrict__ blk, const double* __restrict__ h,
+                                          const double* __restrict__ p, double* __restrict__ f,
+                                          const double* wl, double* jw, double* kaa,
+                                          const T (&wA)[HA ? 10 : 1], T (&jA)[HA ? 10 : 1],
+                                          double& e) {
+  constexpr int NA = HA ? 4 : 1, NB = HB ? 4 : 1;
+  constexpr int SA = HA ? 10 : 1, SB = HB ? 10 : 1;
+  const S* b = blk + block_offset(A, B);
+  const int sb = jslot(B);
+  T wB[SB];
+#pragma unroll
+  for (int l = 0; l < SB; ++l) wB[l] = (T)wl[sb + l];
+  T pab[NA][NB];
+#pragma unroll
+  for (int x = 0; x < NA; ++x) load_run<NB>(p + (int64_t)(A.ao0 + x) * N + B.ao0, pab[x]);
+  T K[NA][NB], jB[SB];
+#pragma unroll
+  for (int x = 0; x < NA; ++x)
+#pragma unroll
+    for (int y = 0; y < NB; ++y) K[x][y] = (T)0;
+#pragma unroll
+  for (int l = 0; l < SB; ++l) jB[l] = (T)0;
+  // the block is SA x SB entries, read in runs of 10 (one run for X-H and
+  // H-X blocks)
+  constexpr int RUN = SA * SB == 1 ? 1 : 10;
+  T run[RUN];
+#pragma unroll
+  for (int k = 0; k < SA; ++k) {
+    // each row of an X-X block in a basic block of its own (an opaque branch),
+    // so that the compiler keeps one row of the block in registers at a time
+    // instead of issuing every load first
+    int go = 1;
+    if constexpr (HB) asm volatile("" : "+s"(go));
+    if (!go) continue;
+    const int mu = mop_a(k), nu = mop_b(k);
+    if constexpr (HB) load_run<10>(b + 10 * k, run);
+    else if (k == 0) load_run<RUN>(b, run);
+#pragma unroll
+    for (int l = 0; l < SB; ++l) {
+      const int b1 = mop_a(l), b2 = mop_b(l);
+      const T val = HB ? run[l] : run[k];
+      jA[k] += val * wB[l];
+      jB[l] += val * wA[k];
+      K[mu][b1] += val * pab[nu][b2];
+      if (mu != nu) K[nu][b1] += val * pab[mu][b2];
+      if (b1 != b2) K[mu][b2] += val * pab[nu][b1];
+      if (mu != nu && b1 != b2) K[nu][b2] += val * pab[mu][b1];
+    }
+  }
+  if (B.local == A.local) {
+    // the one-center block: K_AA for the diagonal block, written at the end
+#pragma unroll
+    for (int x = 0; x < NA; ++x)
+#pragma unroll
+      for (int y = 0; y < NB; ++y) kaa[kslot(A) + x * NB + y] = (double)K[x][y];
+    return;
+  }
+  // F_AB = H_AB - K_AB / 2 and its share 2 sum P (H + F) of the energy
+  double v[NA][NB];
+#pragma unroll
+  for (int x = 0; x < NA; ++x) {
+    const int64_t r = A.ao0 + x;
+    double hv[NB], pv[NB];
+    load_run<NB>(h + r * N + B.ao0, hv);
+    load_run<NB>(p + r * N + B.ao0, pv);
+#pragma unroll
+    for (int y = 0; y < NB; ++y) {
+      v[x][y] = hv[y] - 0.5 * (double)K[x][y];
+      e += 2.0 * pv[y] * (hv[y] + v[x][y]);
+    }
+    if constexpr (NB == 4) {
+      *(dbl2u*)(f + r * N + B.ao0) = dbl2u{v[x][0], v[x][1]};
+      *(dbl2u*)(f + r * N + B.ao0 + 2) = dbl2u{v[x][2], v[x][3]};
+    } else {
+      f[r * N + B.ao0] = v[x][0];
+    }
+  }
+  // F_BA = F_AB^T: row B.ao0 + y holds columns A.ao0 .. A.ao0 + NA - 1
+#pragma unroll
+  for (int y = 0; y < NB; ++y) {
+    double* q = f + (int64_t)(B.ao0 + y) * N + A.ao0;
+    if constexpr (NA == 4) {
+      *(dbl2u*)q = dbl2u{v[0][y], v[1][y]};
+      *(dbl2u*)(q + 2) = dbl2u{v[2][y], v[3][y]};
+    } else {
+      q[0] = v[0][y];
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < SB; ++l) jw[sb + l] += (double)jB[l];
+}
+
+// Row atom A of a molecule: its blocks with every partner B.local <= A.local,
+// heavy partners first (the first hv_before + HA entries of the molecule's
+// partner list), then hydrogens (the list's hydrogen part, from n_h).
+template <bool HA, typename T, typename S>
+__device__ __forceinline__ void fock_row(const AtomDesc& A, const MolDesc& M, int n_h,
+                                         const AtomDesc* __restrict__ partners,
+                                         const S* __restrict__ blk, const double* __restrict__ h,
+                                         const double* __restrict__ p, double* __restrict__ f,
+                                         const double* wl, double* jw, double* kaa, double* eacc,
+                                         int lane) {
+  constexpr int SA = HA ? 10 : 1;
+  const int sa = jslot(A), N = M.n_ao;
+  T wA[SA], jA[SA];
+#pragma unroll
+  for (int k = 0; k < SA; ++k) {
+    wA[k] = uniform((T)wl[sa + k]);  // the same in every lane: scalar registers
+    jA[k] = (T)0;
+  }
+  double e = 0.0;
+  const int nhp = A.hv_before + (HA ? 1 : 0), nHp = A.local + 1 - nhp;
+  const AtomDesc* pl = partners + M.atom0;
+  for (int c0 = 0; c0 < nhp; c0 += 64) {
+    const int i = c0 + lane;
+    if (i < nhp) fock_pair<HA, true, T>(A, pl[i], N, blk, h, p, f, wl, jw, kaa, wA, jA, e);
+  }
+  for (int c0 = 0; c0 < nHp; c0 += 64) {
+    const int i = c0 + lane;
+    if (i < nHp) fock_pair<HA, false, T>(A, pl[n_h + i], N, blk, h, p, f, wl, jw, kaa, wA, jA, e);
+  }
+#pragma unroll
+  for (int k = 0; k < SA; ++k) {
+    const double s = wave_sum((double)jA[k]);
+    if (lane == 0) jw[sa + k] += s;
+  }
+  e = wave_sum(e);
+  if (lane == 0) eacc[A.local] = e;
+}
+
+template <typename T, typename S>
+__device__ __forceinline__ void fock_rows(const MolDesc& M, int n_h,
+                                          const AtomDesc* __restrict__ atoms,
+                                          const AtomDesc* __restrict__ partners,
+                                          const S* __restrict__ blk, const double* __restrict__ h,
+                                          const double* __restrict__ p, double* __restrict__ f,
+                                          const double* wl, double* jw, double* kaa, double* eacc,
+                                          int wave, int lane) {
+  for (int a = wave; a < M.n_atoms; a += kFockMolWaves) {
+    const AtomDesc A = atoms[M.atom0 + a];
+    if (A.zn > 1) fock_row<true, T>(A, M, n_h, partners, blk, h, p, f, wl, jw, kaa, eacc, lane);
+    else fock_row<false, T>(A, M, n_h, partners, blk, h, p, f, wl, jw, kaa, eacc, lane);
+  }
+}
+
+// Diagonal block of atom A once every row is done: J_A = the wavefronts'
+// copies added in order, F_AA = H_AA + J_A - K_AA / 2; returns its share
+// sum P (H + F) of the energy.
+template <bool HA>
+__device__ __forceinline__ double fock_diag(const AtomDesc& A, int N, int S, const double* jall,
+                                            const double* kaa, const double* __restrict__ h,
+                                            const double* __restrict__ p, double* __restrict__ f) {
+  constexpr int NA = HA ? 4 : 1, SA = HA ? 10 : 1;
+  const int s = jslot(A), ks = kslot(A);
+  double J[SA];
+#pragma unroll
+  for (int k = 0; k < SA; ++k) {
+    double t = jall[s + k];
+#pragma unroll
+    for (int w = 1; w < kFockMolWaves; ++w) t += jall[w * S + s + k];
+    J[k] = t;
+  }
+  double ed = 0.0;
+#pragma unroll
+  for (int x = 0; x < NA; ++x)
+#pragma unroll
+    for (int y = 0; y < NA; ++y) {
+      const int64_t r = A.ao0 + x, c = A.ao0 + y;
+      const double hv = h[r * N + c];
+      const double fv = hv + (J[mop_idx(x, y)] - 0.5 * kaa[ks + x * NA + y]);
+      f[r * N + c] = fv;
+      ed += p[r * N + c] * (hv + fv);
+    }
+  return ed;
+}
+
+// T: the contraction's type.  The FP64 and FP32 builds are separate launches
+// (each skips the other's molecules), so that the FP32 one keeps its smaller
+// register count.
+template <typename T>
+__global__ void __launch_bounds__(64 * kFockMolWaves)
+__attribute__((amdgpu_waves_per_eu(std::is_same<T, float>::value ? 4 : 3)))
+k_fock_mol(int n_mol, const int* __restrict__ order, const MolDesc* __restrict__ mols,
+           const AtomDesc* __restrict__ atoms, const AtomDesc* __restrict__ partners,
+           const double* __restrict__ blocks, const float* __restrict__ blocks32,
+           const double* __restrict__ H, const double* __restrict__ P, double* __restrict__ F,
+           double* __restrict__ e_atom, const int* __restrict__ active,
+           const int* __restrict__ status, const int* __restrict__ prec) {
+  extern __shared__ double lds[];
+  const int m = order ? order[blockIdx.x] : blockIdx.x;
+  if (!mol_live(active, status, m)) return;
+  // FP32 contractions for the molecules whose step runs at reduced precision
+  if ((prec && prec[m]) != std::is_same<T, float>::value) return;
+  const MolDesc M = mols[m];
+  const int N = M.n_ao, na = M.n_atoms;
+  if (N > kFockMolMaxN || na == 0) return;  // k_fock's molecules
+  const int n_h = (N - na) / 3;             // heavy atoms: N = 4 n_h + (na - n_h)
+  const int S = 3 * N - 2 * na, KS = 5 * N - 4 * na;
+  double* wl = lds;
+  double* jall = wl + S;
+  double* kaa = jall + kFockMolWaves * S;
+  double* eacc = kaa + KS;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const double* h = H + M.mat0;
+  const double* p = P + M.mat0;
+  double* f = F + M.mat0;
+  // the Coulomb weights of every atom; the Coulomb sums start at zero
+  for (int a = tid; a < na; a += 64 * kFockMolWaves) {
+    const AtomDesc A = atoms[M.atom0 + a];
+    const int s = jslot(A), sb = A.zn > 1 ? 10 : 1;
+    for (int l = 0; l < sb; ++l) {
+      const int x = mop_a(l), y = mop_b(l);
+      const double v = p[(int64_t)(A.ao0 + x) * N + A.ao0 + y];
+      wl[s + l] = x == y ? v : 2.0 * v;
+    }
+  }
+  for (int i = tid; i < kFockMolWaves * S; i += 64 * kFockMolWaves) jall[i] = 0.0;
+  __syncthreads();
+  double* jw = jall + wave * S;
+  if constexpr (std::is_same<T, float>::value)
+    fock_rows<float>(M, n_h, atoms, partners, blocks32 + M.blk0, h, p, f, wl, jw, kaa, eacc, wave,
+                     lane);
+  else
+    fock_rows<double>(M, n_h, atoms, partners, blocks + M.blk0, h, p, f, wl, jw, kaa, eacc, wave,
+                      lane);
+  __syncthreads();
+  // diagonal blocks: F_AA = H_AA + J_A - K_AA / 2, and each atom's energy share
+  for (int a = tid; a < na; a += 64 * kFockMolWaves) {
+    const AtomDesc A = atoms[M.atom0 + a];
+    const double ed = A.zn > 1 ? fock_diag<true>(A, N, S, jall, kaa, h, p, f)
+                               : fock_diag<false>(A, N, S, jall, kaa, h, p, f);
+    e_atom[M.atom0 + a] = eacc[a] + ed;
+  }
+}
 // (ElectronicEnergy::ComputeEnergy, electronicenergy.cpp:12-36, of the F
 // just built).
 __global__ void k_energy_atoms(int n_mol, const MolDesc* __restrict__ mols,
@@ -469,11 +780,31 @@ void launch_guess_density(const BatchDev& b, hipStream_t s, const int* mask) {
 
 void launch_fock(const BatchDev& b, hipStream_t s, bool all, const int* mask) {
   const int* act = mask ? mask : (all ? nullptr : b.active);
-  k_wdiag<<<blocks_for(b.n_atoms, 256), 256, 0, s>>>(b.n_atoms, b.atoms, b.mols, b.P, b.wdiag, act,
-                                                     b.status);
-  k_fock<<<blocks_for(b.n_atoms, 4), 256, 0, s>>>(b.n_atoms, b.atoms, b.partners, b.wdiag, b.mols,
-                                                  b.blocks, b.blocks32, b.H, b.P, b.F, b.e_atom,
-                                                  act, b.status, all ? nullptr : b.prec);
+  const int* prec = all ? nullptr : b.prec;
+  if (b.fock_lds > 0 && b.n_mol > 0) {
+    k_fock_mol<double><<<b.n_mol, 64 * kFockMolWaves, b.fock_lds, s>>>(
+        b.n_mol, b.order, b.mols, b.atoms, b.partners, b.blocks, b.blocks32, b.H, b.P, b.F,
+        b.e_atom, act, b.status, prec);
+    if (prec && b.mixed)
+      k_fock_mol<float><<<b.n_mol, 64 * kFockMolWaves, b.fock_lds, s>>>(
+          b.n_mol, b.order, b.mols, b.atoms, b.partners, b.blocks, b.blocks32, b.H, b.P, b.F,
+          b.e_atom, act, b.status, prec);
+  }
+  if (b.max_n > kFockMolMaxN) {
+    k_wdiag<<<blocks_for(b.n_atoms, 256), 256, 0, s>>>(b.n_atoms, b.atoms, b.mols, b.P, b.wdiag,
+                                                       act, b.status);
+    k_fock<<<blocks_for(b.n_atoms, 4), 256, 0, s>>>(b.n_atoms, b.atoms, b.partners, b.wdiag,
+                                                    b.mols, b.blocks, b.blocks32, b.H, b.P, b.F,
+                                                    b.e_atom, act, b.status, prec);
+  }
+}
+
+void set_fock_lds(BatchDev& d, const MolDesc* mols, int n) {
+  int64_t lds = 0;
+  for (int m = 0; m < n; ++m)
+    if (mols[m].n_ao <= kFockMolMaxN && mols[m].n_atoms > 0)
+      lds = std::max(lds, fock_mol_lds(mols[m].n_ao, mols[m].n_atoms));
+  d.fock_lds = (int)lds;
 }
 
 void launch_energy_fock(const BatchDev& b, hipStream_t s, double* out, bool all, const int* mask) {

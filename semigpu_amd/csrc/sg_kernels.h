@@ -18,6 +18,19 @@ constexpr int kMaxLarge = 8192;
 constexpr int kMinLarge = 128;
 constexpr int kLargeNb = 32;  // reflectors per panel of the multi-workgroup solver
 constexpr int kDiisMax = 8;   // largest DIIS depth
+// Molecules up to this basis size take the Fock build with one workgroup per
+// molecule (k_fock_mol, every pair block read once); larger ones the build
+// with one wavefront per atom (k_fock).
+constexpr int kFockMolMaxN = 512;
+constexpr int kFockMolWaves = 4;
+// Dynamic LDS of k_fock_mol for a molecule of N basis functions and na atoms:
+// the Coulomb weights and kFockMolWaves copies of the Coulomb sums (10 entries
+// per heavy atom, 1 per hydrogen: 3 N - 2 na), the one-center exchange blocks
+// (16 or 1: 5 N - 4 na) and one energy share per atom.
+__host__ __device__ inline int64_t fock_mol_lds(int n, int na) {
+  return (int64_t)sizeof(double) *
+         ((int64_t)(kFockMolWaves + 1) * (3 * n - 2 * na) + (5 * n - 4 * na) + na);
+}
 
 // Device view of a batch of molecules.  All matrices are dense N x N per
 // molecule, row-major (symmetric ones are stored in full); eigenvectors are
@@ -110,6 +123,9 @@ struct BatchDev {
   int* diis_k = nullptr;
   int* diis_p = nullptr;
   int max_n = 0;             // largest N in the batch
+  // dynamic LDS of the one-workgroup-per-molecule Fock build: the largest
+  // fock_mol_lds over the molecules up to kFockMolMaxN (0: none)
+  int fock_lds = 0;
   // Molecule indices by decreasing N (or null for 0..n_mol-1): the per-molecule
   // eigensolver and density launches take their workgroups in this order, so
   // the longest jobs start first and the launch tail is short.
@@ -210,6 +226,8 @@ void launch_hcore(const BatchDev& b, hipStream_t s);
 void launch_core_core(const BatchDev& b, hipStream_t s, const int* mask = nullptr);
 void launch_guess_density(const BatchDev& b, hipStream_t s, const int* mask = nullptr);
 void launch_fock(const BatchDev& b, hipStream_t s, bool all, const int* mask = nullptr);
+// fock_lds from the host descriptors of the molecules a launch can hold
+void set_fock_lds(BatchDev& d, const MolDesc* mols, int n);
 void launch_energy(const BatchDev& b, hipStream_t s, double* out, bool all,
                    const int* mask = nullptr);
 // E_el of the Fock matrices just built by launch_fock (same all / mask), from

@@ -416,11 +416,14 @@ int lane_launch(sg_server* S, Lane* L) {
   }
   // high-water marks of the slot and atom ranges in use
   int hw = 0, hw_atoms = 0, max_n = 1;
+  int64_t fock_lds = 0;
   for (int i = 0; i < L->max_slots; ++i)
     if (L->slots[i].used) {
+      const MolDesc& M = L->mols[i];
       hw = i + 1;
-      hw_atoms = std::max(hw_atoms, L->mols[i].atom0 + L->mols[i].n_atoms);
-      max_n = std::max(max_n, L->mols[i].n_ao);
+      hw_atoms = std::max(hw_atoms, M.atom0 + M.n_atoms);
+      max_n = std::max(max_n, M.n_ao);
+      if (M.n_ao <= kFockMolMaxN) fock_lds = std::max(fock_lds, fock_mol_lds(M.n_ao, M.n_atoms));
     }
   L->hw = hw;
   if (hw == 0) return SG_OK;
@@ -429,6 +432,7 @@ int lane_launch(sg_server* S, Lane* L) {
   it.n_mol = hw;
   it.n_atoms = hw_atoms;
   it.max_n = max_n;
+  it.fock_lds = (int)fock_lds;
   if (!L->fresh.empty()) L->order_dirty = true;
   if (L->order_dirty) {
     // slots in flight by decreasing N, then the free slots below the high-water mark
