@@ -49,16 +49,6 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# The serving loop runs each size class on its own stream.  HIP maps a
-# process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 by default) and
-# streams sharing a queue run in sequence: after the batch run (graph capture
-# and replay) 4 queues left two of the three classes on one queue (their p50
-# 3.7 s instead of 0.5 s).  8 gives every stream its own queue.  Set before
-# the HIP runtime starts (the first HIP call).
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
-
-
 from semigpu_amd import api, geometry  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -275,8 +265,16 @@ SERVE_CLASSES = (300, 700)
 
 
 def serve(eng, rate: float, seconds: float, threshold: float, policy: str, seed: int = 7,
-          limit_s: float = 90.0, diis: int = 0):
-    """Poisson arrivals of alkanes C_nH_2n+2 (n <= 60) and water clusters (n <= 200)."""
+          limit_s: float = None, diis: int = 0):
+    """Poisson arrivals of alkanes C_nH_2n+2 (n <= 60) and water clusters (n <= 200).
+
+    `value` counts every request over the time to the last result (the drain
+    after the window included).  `steady` looks at the window itself after a
+    warm-up of a sixth of it: results completed inside [warm-up, window] per
+    second against the arrivals of the same interval, and the latency of the
+    requests that arrived in it."""
+    if limit_s is None:
+        limit_s = seconds + 60.0
     reqs = geometry.serving_stream(rate, seconds, seed)
     srv = eng.server(hbm_bytes=64e9, max_slots=16384, max_atoms=1 << 21, threshold=threshold,
                      max_steps=100, policy=policy, classes=SERVE_CLASSES)
@@ -286,8 +284,13 @@ def serve(eng, rate: float, seconds: float, threshold: float, policy: str, seed:
     i = 0
     t0 = srv.clock()
     steps = 0
+    t_note = 10.0
     while True:
         now = srv.clock() - t0
+        if now > t_note:  # a progress line for long windows
+            print("serving: %.0f s, %d submitted, %d done" % (now, i, len(done)),
+                  file=sys.stderr, flush=True)
+            t_note += 10.0
         while i < len(reqs) and reqs[i][0] <= now:
             srv.submit(i, reqs[i][2])
             i += 1
@@ -311,9 +314,21 @@ def serve(eng, rate: float, seconds: float, threshold: float, policy: str, seed:
         by_class.append({"n_ao": [lo, min(hi, 1280)], "completed": int(len(lc)),
                          "p50_ms": round(float(np.percentile(lc, 50)) * 1e3, 2) if len(lc) else None,
                          "p99_ms": round(float(np.percentile(lc, 99)) * 1e3, 2) if len(lc) else None})
+    warm = seconds / 6.0
+    span = max(seconds - warm, 1e-9)
+    done_in = sum(1 for r in done if warm <= r["t_done"] - t0 <= seconds)
+    arr_in = sum(1 for q in reqs if warm <= q[0] <= seconds)
+    lat_in = np.array([r["t_done"] - t0 - reqs[r["id"]][0] for r in done
+                       if reqs[r["id"]][0] >= warm])
+    steady = {"interval_s": [round(warm, 3), seconds],
+              "completed_per_s": round(done_in / span, 3),
+              "arrivals_per_s": round(arr_in / span, 3),
+              "p50_ms": round(float(np.percentile(lat_in, 50)) * 1e3, 2) if len(lat_in) else None,
+              "p99_ms": round(float(np.percentile(lat_in, 99)) * 1e3, 2) if len(lat_in) else None}
     return {"rate_per_s": rate, "window_s": seconds, "requests": len(reqs), "completed": len(done),
             "converged": conv, "server_steps": steps,
             "value": round(len(done) / max(t_end, 1e-9), 3), "unit": "molecules/s",
+            "steady": steady,
             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2) if len(lat) else None,
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2) if len(lat) else None,
             "size_classes": list(SERVE_CLASSES), "by_class": by_class,
@@ -505,6 +520,9 @@ def main():
                     help="more arrival rates, RATE:SECONDS[,RATE:SECONDS...], reported under "
                          "serving_rates (e.g. 1000:1,5000:1)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="instead: only the serving stream (--serve-rate, --serve-seconds), "
+                         "one JSON line")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shapes of the reduction buckets")
     ap.add_argument("--calib-copy", action="store_true",
@@ -532,6 +550,15 @@ def main():
 
     if args.water_box:
         water_box(args)
+        return
+    if args.serve_only:
+        eng = api.Engine(0)
+        t = time.time()
+        res = serve(eng, args.serve_rate, args.serve_seconds, args.threshold, args.policy)
+        print("serving at %g/s for %g s: %.1f s" % (args.serve_rate, args.serve_seconds,
+                                                   time.time() - t), file=sys.stderr, flush=True)
+        eng.close()
+        print(json.dumps({"serving": res, "policy": args.policy}), flush=True)
         return
 
     mols = workload(args.count, args.seed)

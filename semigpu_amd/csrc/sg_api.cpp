@@ -215,7 +215,12 @@ int sg_ctx_create(int device, sg_context** out) {
   auto* c = new sg_context();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  for (int i = 0; i < kCtxClassStreams && e == hipSuccess; ++i)
+    e = hipStreamCreateWithFlags(&c->class_streams[i], hipStreamNonBlocking);
   if (e != hipSuccess) {
+    for (hipStream_t cs : c->class_streams)
+      if (cs) (void)hipStreamDestroy(cs);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return fail(SG_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
@@ -233,6 +238,11 @@ int sg_ctx_destroy(sg_context* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& blk : c->cache) (void)hipFree(blk.first);
   if (c->staging) (void)hipHostFree(c->staging);
+  for (hipStream_t cs : c->class_streams)
+    if (cs) {
+      (void)hipStreamSynchronize(cs);
+      (void)hipStreamDestroy(cs);
+    }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return SG_OK;
@@ -458,7 +468,7 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   d.n_vec = vec0;
   d.n_blocks = blk0;
   d.max_n = max_n;
-  set_fock_lds(d, b->mols.data(), n_mol);
+  set_launch_shapes(d, b->mols.data(), n_mol);
   d.n_xx = (int)n_xx;
   d.n_hx = (int)n_hx;
   d.n_hh = (int)n_hh;

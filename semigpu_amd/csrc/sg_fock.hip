@@ -799,12 +799,17 @@ void launch_fock(const BatchDev& b, hipStream_t s, bool all, const int* mask) {
   }
 }
 
-void set_fock_lds(BatchDev& d, const MolDesc* mols, int n) {
+void set_launch_shapes(BatchDev& d, const MolDesc* mols, int n, const int* used) {
   int64_t lds = 0;
-  for (int m = 0; m < n; ++m)
-    if (mols[m].n_ao <= kFockMolMaxN && mols[m].n_atoms > 0)
+  int rounds = 0;
+  for (int m = 0; m < n; ++m) {
+    if ((used && !used[m]) || mols[m].n_atoms == 0) continue;
+    if (mols[m].n_ao <= kFockMolMaxN)
       lds = std::max(lds, fock_mol_lds(mols[m].n_ao, mols[m].n_atoms));
+    rounds = std::max(rounds, core_rounds(mols[m].n_atoms));
+  }
   d.fock_lds = (int)lds;
+  d.core_rounds = rounds;
 }
 
 void launch_energy_fock(const BatchDev& b, hipStream_t s, double* out, bool all, const int* mask) {

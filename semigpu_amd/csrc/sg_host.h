@@ -41,9 +41,20 @@ inline int reduced_prec(int policy) {
 
 }  // namespace sg
 
+// Streams a context keeps for the size classes of its servers (class i takes
+// stream i; classes beyond these create their own).
+constexpr int kCtxClassStreams = 3;
+
 struct sg_context {
   int device = 0;
   hipStream_t stream = nullptr;
+  // Created with the context, right after `stream`: HIP maps a process's
+  // streams onto its hardware queues (GPU_MAX_HW_QUEUES, 4 by default) in the
+  // order they are created, and streams that share a queue run in sequence.
+  // Streams made this early land on queues of their own however many streams
+  // (graph launches, other libraries) the process creates later, so the
+  // classes of a server overlap on the device with the default queue count.
+  hipStream_t class_streams[kCtxClassStreams] = {};
   bool timing = false;
   int large_min = sg::kMaxBasis + 1;  // batches created from now on
   bool graphs = true;                 // capture SCF iterations as HIP graphs

@@ -311,11 +311,9 @@ __device__ __forceinline__ void pair_of(int64_t p, int na, int& i, int& j) {
 // One workgroup per molecule.  The pairs are taken in the reference order in
 // rounds of kCoreChunk terms: waves 1-3 evaluate round r + 1 into one LDS
 // buffer while lane 0 of wave 0 adds round r from the other to the running
-// sum, in order, so the sum is the reference's serial sum bit for bit.  Above
-// kCoreSerialMax pairs every thread evaluates and sums its share of a round
-// (terms t, t + 256, ... in order), the 256 shares are added by a fixed tree
-// and the rounds in order: the same sum for the molecule wherever it is run,
-// with a smaller rounding error than the serial one.
+// sum, in order, so the sum is the reference's serial sum bit for bit.
+// Molecules above kCoreSerialMax pairs take k_core_rounds and k_core_sum
+// instead.
 __global__ void __launch_bounds__(256) k_core_core(const AtomDesc* __restrict__ atoms,
                                                     const MolDesc* __restrict__ mols,
                                                     const double* __restrict__ blocks,
@@ -330,28 +328,7 @@ __global__ void __launch_bounds__(256) k_core_core(const AtomDesc* __restrict__ 
   const AtomDesc* at = atoms + M.atom0;
   __shared__ double term[2][kCoreChunk];
   const int nr = (int)((npair + kCoreChunk - 1) / kCoreChunk);
-  if (npair > kCoreSerialMax) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double e = 0.0;
-    for (int r = 0; r < nr; ++r) {
-      const int64_t p0 = (int64_t)r * kCoreChunk;
-      const int cnt = (int)(npair - p0 < kCoreChunk ? npair - p0 : kCoreChunk);
-      double part = 0.0;
-      for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-        int i, j;
-        pair_of(p0 + q, na, i, j);
-        part += core_pair(at[i], at[j], blk);
-      }
-#pragma unroll
-      for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o, 64);
-      if (lane == 0) term[0][wave] = part;
-      __syncthreads();
-      if (threadIdx.x == 0) e += (term[0][0] + term[0][1]) + (term[0][2] + term[0][3]);
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) e_cc[m] = e;
-    return;
-  }
+  if (npair > kCoreSerialMax) return;  // k_core_rounds and k_core_sum
   auto fill = [&](int r, int t0, int nt) {
     const int64_t p0 = (int64_t)r * kCoreChunk;
     const int cnt = (int)(npair - p0 < kCoreChunk ? npair - p0 : kCoreChunk);
@@ -389,6 +366,63 @@ __global__ void __launch_bounds__(256) k_core_core(const AtomDesc* __restrict__ 
   if (threadIdx.x == 0) e_cc[m] = e;
 }
 
+// Molecules above kCoreSerialMax atom pairs: one workgroup per round of
+// kCoreChunk terms (grid: rounds of the largest such molecule x molecules).
+// Every thread evaluates and sums its share of the round (terms t, t + 256, ...
+// in order), the 256 shares are added by a fixed tree, and k_core_sum adds
+// the rounds in order: the same sum for the molecule wherever it is run, with
+// a smaller rounding error than the serial one.  The round sums wait in the
+// molecule's eigensolver workspace (unused before the SCF).
+__global__ void __launch_bounds__(256) k_core_rounds(const AtomDesc* __restrict__ atoms,
+                                                      const MolDesc* __restrict__ mols,
+                                                      const double* __restrict__ blocks,
+                                                      double* __restrict__ work,
+                                                      const int* __restrict__ mask) {
+  const int m = blockIdx.y, r = blockIdx.x;
+  if (mask && !mask[m]) return;
+  const MolDesc M = mols[m];
+  const int na = M.n_atoms;
+  const int64_t npair = (int64_t)na * (na - 1) / 2;
+  if (npair <= kCoreSerialMax || (int64_t)r * kCoreChunk >= npair) return;
+  const double* blk = blocks + M.blk0;
+  const AtomDesc* at = atoms + M.atom0;
+  __shared__ double sh[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t p0 = (int64_t)r * kCoreChunk;
+  const int cnt = (int)(npair - p0 < kCoreChunk ? npair - p0 : kCoreChunk);
+  double part = 0.0;
+  for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+    int i, j;
+    pair_of(p0 + q, na, i, j);
+    part += core_pair(at[i], at[j], blk);
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o, 64);
+  if (lane == 0) sh[wave] = part;
+  __syncthreads();
+  if (threadIdx.x == 0) work[5 * M.mat0 + r] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ void k_core_sum(int n_mol, const MolDesc* __restrict__ mols,
+                           const double* __restrict__ work, double* __restrict__ e_cc,
+                           const int* __restrict__ mask) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_mol || (mask && !mask[m])) return;
+  const MolDesc M = mols[m];
+  const int64_t npair = (int64_t)M.n_atoms * (M.n_atoms - 1) / 2;
+  if (npair <= kCoreSerialMax) return;
+  const int nr = (int)((npair + kCoreChunk - 1) / kCoreChunk);
+  const double* part = work + 5 * M.mat0;
+  double e = 0.0;
+  for (int r = 0; r < nr; ++r) e += part[r];
+  e_cc[m] = e;
+}
+
+int core_rounds(int n_atoms) {
+  const int64_t npair = (int64_t)n_atoms * (n_atoms - 1) / 2;
+  return npair > kCoreSerialMax ? (int)((npair + kCoreChunk - 1) / kCoreChunk) : 0;
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -424,6 +458,11 @@ void launch_hcore(const BatchDev& b, hipStream_t s) {
 
 void launch_core_core(const BatchDev& b, hipStream_t s, const int* mask) {
   k_core_core<<<b.n_mol, 256, 0, s>>>(b.atoms, b.mols, b.blocks, b.e_cc, mask);
+  if (b.core_rounds > 0 && b.n_mol > 0) {
+    k_core_rounds<<<dim3(b.core_rounds, b.n_mol), 256, 0, s>>>(b.atoms, b.mols, b.blocks, b.W,
+                                                               mask);
+    k_core_sum<<<blocks_for(b.n_mol, 64), 64, 0, s>>>(b.n_mol, b.mols, b.W, b.e_cc, mask);
+  }
 }
 
 }  // namespace sg

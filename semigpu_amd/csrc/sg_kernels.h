@@ -126,6 +126,9 @@ struct BatchDev {
   // dynamic LDS of the one-workgroup-per-molecule Fock build: the largest
   // fock_mol_lds over the molecules up to kFockMolMaxN (0: none)
   int fock_lds = 0;
+  // rounds of the multi-workgroup core-core sum: the largest core_rounds over
+  // the molecules (0: none takes it)
+  int core_rounds = 0;
   // Molecule indices by decreasing N (or null for 0..n_mol-1): the per-molecule
   // eigensolver and density launches take their workgroups in this order, so
   // the longest jobs start first and the launch tail is short.
@@ -226,8 +229,12 @@ void launch_hcore(const BatchDev& b, hipStream_t s);
 void launch_core_core(const BatchDev& b, hipStream_t s, const int* mask = nullptr);
 void launch_guess_density(const BatchDev& b, hipStream_t s, const int* mask = nullptr);
 void launch_fock(const BatchDev& b, hipStream_t s, bool all, const int* mask = nullptr);
-// fock_lds from the host descriptors of the molecules a launch can hold
-void set_fock_lds(BatchDev& d, const MolDesc* mols, int n);
+// Rounds of kCoreChunk atom pairs of the multi-workgroup core-core sum of a
+// molecule with n_atoms atoms (0: its sum runs in one workgroup).
+int core_rounds(int n_atoms);
+// fock_lds and core_rounds from the host descriptors of the molecules a launch
+// can hold (used: null, or non-zero for the slots that hold a molecule)
+void set_launch_shapes(BatchDev& d, const MolDesc* mols, int n, const int* used = nullptr);
 void launch_energy(const BatchDev& b, hipStream_t s, double* out, bool all,
                    const int* mask = nullptr);
 // E_el of the Fock matrices just built by launch_fock (same all / mask), from

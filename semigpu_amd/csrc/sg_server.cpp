@@ -138,6 +138,7 @@ struct Lane {
   int64_t pair_cap = 0;  // scratch entries for the pair lists set up per step
   hipStream_t stream = nullptr;
   hipEvent_t ev = nullptr;
+  bool own_stream = false;  // created here (else one of the context's class streams)
   void* dev = nullptr;
   BatchDev d;
   int* mask_new = nullptr;  // device, per slot (+ dead slot)
@@ -150,6 +151,7 @@ struct Lane {
   Pinned<AtomDesc> partners;  // the Fock build's partner order per molecule (descriptor copies)
   Pinned<int> mask, order;
   std::vector<int> tri_n;  // N of the slots in launch order (reduction buckets)
+  std::vector<int> slot_used;  // slots that hold a molecule (launch shapes)
   Pinned<int2> pairs;
   Pinned<int> rb_act, rb_steps, rb_conv, rb_stat;
   Pinned<double> rb_e, rb_ecc;
@@ -188,7 +190,7 @@ struct sg_server {
       if (L->dev) (void)hipFree(L->dev);
       if (L->diis_dev) (void)hipFree(L->diis_dev);
       if (L->ev) (void)hipEventDestroy(L->ev);
-      if (L->stream) (void)hipStreamDestroy(L->stream);
+      if (L->stream && L->own_stream) (void)hipStreamDestroy(L->stream);
       delete L;
     }
     if (pool) (void)hipFree(pool);
@@ -205,7 +207,8 @@ size_t lane_bytes(int max_slots, int max_atoms, int64_t pair_cap) {
          ns * (5 * 8 + 9 * 4) + 256 + 16 * 256;
 }
 
-int lane_create(const BatchDev& shared, Lane* L, int max_slots, int max_atoms, int64_t pair_cap) {
+int lane_create(const BatchDev& shared, Lane* L, int max_slots, int max_atoms, int64_t pair_cap,
+                hipStream_t stream) {
   L->max_slots = max_slots;
   L->max_atoms = max_atoms;
   L->pair_cap = pair_cap;
@@ -225,7 +228,12 @@ int lane_create(const BatchDev& shared, Lane* L, int max_slots, int max_atoms, i
                o_stat = take(4 * ns), o_nact = take(256), o_prec = take(4 * ns),
                o_pprec = take(4 * ns), o_mask = take(4 * ns), o_order = take(4 * ns),
                o_dep = take(8 * ns), o_wdiag = take(80 * (size_t)max_atoms);
-  HIP_TRY(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+  if (stream) {
+    L->stream = stream;
+  } else {
+    HIP_TRY(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+    L->own_stream = true;
+  }
   HIP_TRY(hipEventCreateWithFlags(&L->ev, hipEventDisableTiming));
   hipError_t e = hipMalloc(&L->dev, off);
   if (e != hipSuccess) {
@@ -416,14 +424,12 @@ int lane_launch(sg_server* S, Lane* L) {
   }
   // high-water marks of the slot and atom ranges in use
   int hw = 0, hw_atoms = 0, max_n = 1;
-  int64_t fock_lds = 0;
   for (int i = 0; i < L->max_slots; ++i)
     if (L->slots[i].used) {
       const MolDesc& M = L->mols[i];
       hw = i + 1;
       hw_atoms = std::max(hw_atoms, M.atom0 + M.n_atoms);
       max_n = std::max(max_n, M.n_ao);
-      if (M.n_ao <= kFockMolMaxN) fock_lds = std::max(fock_lds, fock_mol_lds(M.n_ao, M.n_atoms));
     }
   L->hw = hw;
   if (hw == 0) return SG_OK;
@@ -432,7 +438,12 @@ int lane_launch(sg_server* S, Lane* L) {
   it.n_mol = hw;
   it.n_atoms = hw_atoms;
   it.max_n = max_n;
-  it.fock_lds = (int)fock_lds;
+  {
+    std::vector<int>& used = L->slot_used;
+    used.assign(hw, 0);
+    for (int i = 0; i < hw; ++i) used[i] = L->slots[i].used ? 1 : 0;
+    set_launch_shapes(it, L->mols.p, hw, used.data());
+  }
   if (!L->fresh.empty()) L->order_dirty = true;
   if (L->order_dirty) {
     // slots in flight by decreasing N, then the free slots below the high-water mark
@@ -606,7 +617,8 @@ int server_create(sg_context* ctx, double hbm_bytes, int max_slots, int max_atom
     S->lanes.push_back(L);
     L->n_lo = i ? bounds[i - 1] : 0;
     L->n_hi = i + 1 < n_classes ? bounds[i] : kMaxBasis + 1;
-    const int r = lane_create(S->shared, L, ls, la, pc);
+    const int r = lane_create(S->shared, L, ls, la, pc,
+                              i < kCtxClassStreams ? ctx->class_streams[i] : nullptr);
     if (r) {
       delete S;
       return r;
