@@ -752,9 +752,12 @@ __global__ void __launch_bounds__(256) k_bisect(const MolDesc* __restrict__ mols
     lo_[u] = a;
     hi_[u] = b;
   }
-  // bisection keeping count(a) <= idx < count(b), to a few ulps; in an FP32
-  // step to kFp32BisectTol ||T|| (T itself carries FP32 rounding there)
-  const double atol = pivmin + (prec && prec[m] ? kFp32BisectTol * tnorm : 0.0);
+  // bisection keeping count(a) <= idx < count(b), to a few ulps of the
+  // eigenvalue or 2 ulps of ||T||, whichever is wider (the default absolute
+  // tolerance of LAPACK dstebz: the Sturm recurrence's own rounding is of that
+  // order); in an FP32 step to kFp32BisectTol ||T|| (T itself carries FP32
+  // rounding there)
+  const double atol = pivmin + (prec && prec[m] ? kFp32BisectTol : 2.0 * kEps) * tnorm;
   bool done[2];
 #pragma unroll
   for (int u = 0; u < 2; ++u)
