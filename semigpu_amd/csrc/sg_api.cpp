@@ -215,11 +215,17 @@ int sg_ctx_create(int device, sg_context** out) {
   auto* c = new sg_context();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
   for (int i = 0; i < kCtxClassStreams && e == hipSuccess; ++i)
     e = hipStreamCreateWithFlags(&c->class_streams[i], hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (e != hipSuccess) {
     for (hipStream_t cs : c->class_streams)
       if (cs) (void)hipStreamDestroy(cs);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return fail(SG_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
@@ -243,6 +249,12 @@ int sg_ctx_destroy(sg_context* c) {
       (void)hipStreamSynchronize(cs);
       (void)hipStreamDestroy(cs);
     }
+  if (c->side) {
+    (void)hipStreamSynchronize(c->side);
+    (void)hipStreamDestroy(c->side);
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return SG_OK;
@@ -940,7 +952,7 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
     b->iter_exec = nullptr;
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    scf_iteration(c, b->d, s, threshold, max_steps, switch_ev, fp32_cap);
+    scf_iteration(c, b->d, s, threshold, max_steps, switch_ev, fp32_cap, true);
     HIP_TRY(hipStreamEndCapture(s, &g));
     if ((r = check_launch("scf step capture"))) {
       (void)hipGraphDestroy(g);
@@ -962,7 +974,7 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
     if (use_graph)
       HIP_TRY(hipGraphLaunch(b->iter_exec, s));
     else
-      scf_iteration(c, b->d, s, threshold, max_steps, switch_ev, fp32_cap);
+      scf_iteration(c, b->d, s, threshold, max_steps, switch_ev, fp32_cap, true);
     if ((r = check_launch("scf step"))) return r;
     HIP_TRY(hipMemcpyAsync(&host_active, b->d.n_active, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));

@@ -4,6 +4,7 @@
 
 #include <string>
 #include <utility>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/semigpu.h"
@@ -55,6 +56,11 @@ struct sg_context {
   // (graph launches, other libraries) the process creates later, so the
   // classes of a server overlap on the device with the default queue count.
   hipStream_t class_streams[kCtxClassStreams] = {};
+  // A second stream for the SCF of a batch (its small molecules beside the
+  // large ones, scf_iteration), created right after `stream` so that the two
+  // run on different hardware queues, and the events that fork and join them.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool timing = false;
   int large_min = sg::kMaxBasis + 1;  // batches created from now on
   bool graphs = true;                 // capture SCF iterations as HIP graphs
@@ -109,7 +115,7 @@ struct StageTimer {
 // One SCF iteration of the active molecules (scfcalculation.cpp:82-146):
 // eigensolve, density, Fock, energy, convergence bookkeeping.
 inline void scf_iteration(sg_context* c, const BatchDev& d, hipStream_t s, double threshold,
-                          int max_steps, double switch_ev, int fp32_cap);
+                          int max_steps, double switch_ev, int fp32_cap, bool split);
 
 // The eigensolver's kernels, each timed as its own stage (the reduction as
 // its head, on-chip and FP32 launches).
@@ -140,8 +146,8 @@ inline void timed_eigh(sg_context* c, const BatchDev& d, hipStream_t s, bool all
   }
 }
 
-inline void scf_iteration(sg_context* c, const BatchDev& d, hipStream_t s, double threshold,
-                          int max_steps, double switch_ev, int fp32_cap) {
+// Eigensolve, density and Fock build of the molecules of d, on stream s.
+inline void scf_solve_build(sg_context* c, const BatchDev& d, hipStream_t s) {
   timed_eigh(c, d, s, false);
   {
     StageTimer t(c, ST_DENSITY, s);
@@ -152,6 +158,48 @@ inline void scf_iteration(sg_context* c, const BatchDev& d, hipStream_t s, doubl
   {
     StageTimer t(c, ST_FOCK, s);
     launch_fock(d, s, false);
+  }
+}
+
+// The molecules of a batch in two groups by the launch order (decreasing N):
+// those above kRegRows, whose reduction starts with k_tridiag (bound by HBM),
+// and the rest (bound by the issue of their on-chip reduction and the other
+// stages).  Each is a view of the batch over its range of the order; the
+// reduction's bucket ranges are shifted to it.
+inline void split_views(const BatchDev& d, BatchDev& big, BatchDev& small) {
+  const int nb = d.tri_head;
+  big = d;
+  big.n_mol = nb;
+  for (int k = 1; k <= kTriBuckets; ++k) big.tri_start[k] = nb;
+  big.tri_start[0] = 0;
+  small = d;
+  small.order = d.order + nb;
+  small.n_mol = d.n_mol - nb;
+  small.max_n = kRegRows;  // a bound: every molecule here has N <= kRegRows
+  small.tri_head = 0;
+  for (int k = 0; k <= kTriBuckets; ++k) small.tri_start[k] = std::max(0, d.tri_start[k] - nb);
+}
+
+// split: a batch with molecules on both sides of kRegRows (and none on the
+// multi-workgroup eigensolver, no damping) runs the two groups' eigensolve,
+// density and Fock build on two streams at once, the small molecules on the
+// context's side stream, and joins them for the energies and the convergence
+// test; every molecule runs the same kernels as otherwise, so its results do
+// not change.
+inline void scf_iteration(sg_context* c, const BatchDev& d, hipStream_t s, double threshold,
+                          int max_steps, double switch_ev, int fp32_cap, bool split) {
+  if (split && c->side && d.order && d.n_large == 0 && d.damping == 0.0 && d.tri_head > 0 &&
+      d.tri_head < d.n_mol) {
+    BatchDev big, small;
+    split_views(d, big, small);
+    (void)hipEventRecord(c->ev_fork, s);
+    (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
+    scf_solve_build(c, small, c->side);
+    scf_solve_build(c, big, s);
+    (void)hipEventRecord(c->ev_join, c->side);
+    (void)hipStreamWaitEvent(s, c->ev_join, 0);
+  } else {
+    scf_solve_build(c, d, s);
   }
   {
     StageTimer t(c, ST_ENERGY, s);
