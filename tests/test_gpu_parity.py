@@ -106,6 +106,29 @@ def test_batch_equals_alone_bitwise(engine):
         assert alone == big[i]
 
 
+@pytest.mark.parametrize("policy", ["fp64", "mixed"])
+def test_two_groups_equal_alone_bitwise(engine, policy):
+    """A batch with molecules on both sides of 256 basis functions runs its
+    large and small molecules' eigensolve, density and Fock build on two
+    streams at once (graph capture and direct launches): every molecule's
+    result equals the molecule alone, bit for bit."""
+    rng = geometry.SplitMix64(31)
+    mols = [geometry.water_cluster(n, rng) for n in (60, 5, 47, 30, 64, 2, 44, 12)]
+    mols.insert(2, geometry.alkane(45))  # N = 272
+    mols.insert(5, geometry.pyridine())
+    sizes = [geometry.n_basis(m) for m in mols]
+    assert any(s > 256 for s in sizes) and any(s <= 256 for s in sizes)
+    for graphs in (True, False):
+        engine.graphs(graphs)
+        try:
+            big = engine.batch(mols).scf(threshold=1e-6, policy=policy)
+            for i, m in enumerate(mols):
+                alone = engine.batch([m]).scf(threshold=1e-6, policy=policy)[0]
+                assert alone == big[i], (graphs, i, sizes[i])
+        finally:
+            engine.graphs(True)
+
+
 def test_eigensolver_properties(engine):
     g = load_golden("ref_02_benzene")
     n = int(g["n_ao"])
