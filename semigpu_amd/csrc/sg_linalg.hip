@@ -22,7 +22,7 @@
 //                   Sturm count, two eigenvalues per lane (only the indices the
 //                   caller selected).
 //  k_invit          eigenvectors of the tridiagonal matrix by inverse iteration
-//                   (2 sweeps, 1 in FP32 steps; Gaussian elimination with
+//                   (1 sweep in the SCF, 2 for the full decomposition; Gaussian elimination with
 //                   partial pivoting, one lane per vector).  Eigenvalues closer
 //                   than 1e-3*||T|| (full
 //                   decomposition) or 1e-5*||T|| (SCF subspace) form a
@@ -64,10 +64,21 @@ constexpr double kEps = 2.220446049250313e-16;
 // closer ones are orthogonalised as a cluster).
 constexpr double kFp32BisectTol = 1e-10;
 // Inverse-iteration solves per vector.  With eigenvalues bisected to a few
-// ulps, one solve from a random start already leaves the wanted direction
-// dominant by ~1/eps; the second solve (after the cluster orthogonalisation)
-// settles the vector to working accuracy.
-constexpr int kInvitSweeps = 2;
+// ulps of ||T||, one solve from a random start already leaves the wanted
+// direction dominant: another eigenvalue at distance g keeps a share of about
+// eps ||T|| / g, at most ~1e-10 outside the eigenvalue's cluster (closer
+// eigenvalues are orthogonalised together).  The SCF takes one solve: the
+// density is the occupied subspace, and the energy, stationary under orbital
+// rotations at self-consistency, moves by the square of that share.  The full
+// decomposition (sg_eigh) takes a second solve after the cluster
+// orthogonalisation, which settles each vector to working accuracy, and so
+// does the eigensolver of molecules above one workgroup (N > kMaxBasis),
+// where the shares of ~N eigenvalues add up in the density (one solve left
+// |P^2 - 2P| at 3.6e-9 for N = 1500, two leave ~1e-12).
+__host__ __device__ inline int invit_sweeps(bool fp32, int want_all) {
+  return fp32 || !want_all ? 1 : 2;
+}
+constexpr int kInvitSweepsLarge = 2;
 constexpr int kSolveBatch = 4;  // rows whose loads are in flight ahead of the solve
 
 // Cross-lane moves without the LDS crossbar (gfx950): DPP within rows of 16
@@ -987,7 +998,7 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(3))) k_
   if (k == 0) return;
   // FP32 steps: one solve (their T and eigenvalues carry FP32-level error)
   const bool fp32 = prec && prec[m];
-  const int sweeps = fp32 ? 1 : kInvitSweeps;
+  const int sweeps = invit_sweeps(fp32, want_all);
   double* d = lds;
   double* e = d + n;
   double* sh = e + n;                     // shifts, k
@@ -1073,7 +1084,7 @@ __global__ void __launch_bounds__(64) k_li_solve(const MolDesc* __restrict__ mol
             ? tri_solve<true, double>(d, e, n, W.sh[j], tiny, W.Y, k, j, W.fac + j0, L, lo + j)
             : tri_solve<false, double>(d, e, n, W.sh[j], tiny, W.Y, k, j, W.fac + j0, L, lo + j);
     const bool single = W.cstart[j] == j && (j + 1 == k || W.cstart[j + 1] != j);
-    if (single && sweep == kInvitSweeps - 1) scale_column(W.Y, n, k, j, 1.0 / nrm);
+    if (single && sweep == kInvitSweepsLarge - 1) scale_column(W.Y, n, k, j, 1.0 / nrm);
   }
 }
 
@@ -1787,7 +1798,7 @@ void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all) {
     if (k == 0) continue;
     const int L = LargeWork::lanes(M.n_ao, k);
     k_li_setup<<<1, 64, 0, s>>>(b.mols, b.eig, b.tnorm, b.W, all ? 1 : 0, act, b.status, m);
-    for (int sweep = 0; sweep < kInvitSweeps; ++sweep) {
+    for (int sweep = 0; sweep < kInvitSweepsLarge; ++sweep) {
       k_li_solve<<<(L + 63) / 64, 64, 0, s>>>(b.mols, b.dT, b.eT, b.tnorm, b.W, all ? 1 : 0, act,
                                               b.status, m, sweep);
       k_li_mgs<<<min(1024, (k + 3) / 4), 256, 0, s>>>(b.mols, b.W, all ? 1 : 0, act, b.status, m);

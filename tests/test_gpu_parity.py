@@ -159,6 +159,26 @@ def test_water_clusters_against_oracle(engine):
         assert abs(r["e_tot"] - o["e_tot"]) <= 1e-6
 
 
+@pytest.mark.parametrize("waters", [64, 200])
+def test_converged_density_idempotent(engine, waters):
+    """FP64 SCF of N = 384 and N = 1200 (one workgroup's eigensolver, one
+    inverse-iteration solve per vector): the converged density is idempotent
+    (P^2 = 2P for the closed shell) with the right trace."""
+    import pyoracle as po
+    mol = geometry.water_cluster(waters, geometry.SplitMix64(19))
+    b = engine.batch([mol])
+    r = b.scf(threshold=1e-6)[0]
+    n = b.n_ao(0)
+    P = po.packed_to_full(b.packed(0, api.SG_DENSITY), n)
+    n_occ = b.info[0][2]
+    b.close()
+    idem = np.abs(P @ P - 2.0 * P).max()
+    say("N", n, "steps", r["steps"], "idempotency", idem, "trace", np.trace(P) - 2 * n_occ)
+    assert r["converged"]
+    assert abs(np.trace(P) - 2 * n_occ) <= 1e-9
+    assert idem <= 1e-9
+
+
 def test_energy_history_against_oracle(engine):
     """The per-iteration record (sg_scf_record_energies) against the oracle's
     SCF energies step by step (scfcalculation.cpp:66-146: E_0 from the guess
