@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <string>
 #include <vector>
 
@@ -49,13 +50,21 @@ struct sg_batch {
   std::vector<int> large_idx;       // molecules on the multi-workgroup eigensolver
   std::vector<MolDesc> large_mols;  // their descriptors
   size_t dev_bytes = 0;             // size of dev_mem (returned to the context's cache)
-  hipGraphExec_t iter_exec = nullptr;  // captured SCF iteration
+  // captured SCF iteration (of the first group when the SCF runs as two
+  // groups, split_views) and of the second group
+  hipGraphExec_t iter_exec = nullptr, iter_exec2 = nullptr;
   void* hist_mem = nullptr;             // per-iteration energies (sg_scf_record_energies)
   double g_threshold = -1.0, g_switch = -1.0;
   int g_max_steps = -1, g_mixed = -1;
   double g_damping = -1.0;
   void* diis_mem = nullptr;  // DIIS histories (sg_scf_set_diis)
 };
+
+static void drop_graphs(sg_batch* b) {
+  if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
+  if (b->iter_exec2) (void)hipGraphExecDestroy(b->iter_exec2);
+  b->iter_exec = b->iter_exec2 = nullptr;
+}
 
 // Where atom i's row starts in each list of a batch: the all-pairs list, the
 // type-sorted lists (-1 where the atom's kind has no row there), the
@@ -196,6 +205,50 @@ hipError_t ctx_staging(sg_context* c, size_t bytes, void** out) {
 
 }  // namespace sg
 
+// Synchronizes and destroys a context's streams, events and pinned counts
+// (those created so far).
+static void ctx_free_handles(sg_context* c) {
+  for (hipStream_t cs : c->class_streams)
+    if (cs) {
+      (void)hipStreamSynchronize(cs);
+      (void)hipStreamDestroy(cs);
+    }
+  if (c->side) {
+    (void)hipStreamSynchronize(c->side);
+    (void)hipStreamDestroy(c->side);
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  for (hipEvent_t ev : c->ev_count)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->host_counts) (void)hipHostFree(c->host_counts);
+  if (c->stream) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+  }
+}
+
+// Captures one SCF iteration of the molecules of d on stream s as a graph.
+static int capture_iteration(sg_context* c, const BatchDev& d, hipStream_t s, double threshold,
+                             int max_steps, double switch_ev, int fp32_cap, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  scf_iteration(c, d, s, threshold, max_steps, switch_ev, fp32_cap);
+  HIP_TRY(hipStreamEndCapture(s, &g));
+  int r;
+  if ((r = check_launch("scf step capture"))) {
+    (void)hipGraphDestroy(g);
+    return r;
+  }
+  hipError_t ge = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ge != hipSuccess) {
+    *out = nullptr;
+    return fail(SG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ge));
+  }
+  return SG_OK;
+}
+
 extern "C" {
 
 const char* sg_last_error(void) { return g_err.c_str(); }
@@ -220,15 +273,13 @@ int sg_ctx_create(int device, sg_context** out) {
     e = hipStreamCreateWithFlags(&c->class_streams[i], hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+  for (hipEvent_t& ev : c->ev_count)
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->host_counts, 4 * sizeof(int));
   if (e != hipSuccess) {
-    for (hipStream_t cs : c->class_streams)
-      if (cs) (void)hipStreamDestroy(cs);
-    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
-    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
-    if (c->side) (void)hipStreamDestroy(c->side);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    ctx_free_handles(c);
     delete c;
-    return fail(SG_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    return fail(SG_ERR_HIP, std::string("context set-up: ") + hipGetErrorString(e));
   }
   *out = c;
   return SG_OK;
@@ -244,18 +295,7 @@ int sg_ctx_destroy(sg_context* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& blk : c->cache) (void)hipFree(blk.first);
   if (c->staging) (void)hipHostFree(c->staging);
-  for (hipStream_t cs : c->class_streams)
-    if (cs) {
-      (void)hipStreamSynchronize(cs);
-      (void)hipStreamDestroy(cs);
-    }
-  if (c->side) {
-    (void)hipStreamSynchronize(c->side);
-    (void)hipStreamDestroy(c->side);
-  }
-  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
-  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  ctx_free_handles(c);
   delete c;
   return SG_OK;
 }
@@ -633,7 +673,8 @@ int sg_batch_destroy(sg_batch* b) {
   if (!b) return SG_OK;
   (void)hipSetDevice(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
-  if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
+  (void)hipStreamSynchronize(b->ctx->side);
+  drop_graphs(b);
   if (b->dev_mem) ctx_release(b->ctx, b->dev_mem, b->dev_bytes);
   if (b->diis_mem) (void)hipFree(b->diis_mem);
   if (b->hist_mem) (void)hipFree(b->hist_mem);
@@ -812,8 +853,7 @@ int sg_scf_record_energies(sg_batch* b, int cap) {
   HIP_TRY(hipSetDevice(b->ctx->device));
   HIP_TRY(hipStreamSynchronize(b->ctx->stream));
   // the captured iteration holds the old record pointer
-  if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
-  b->iter_exec = nullptr;
+  drop_graphs(b);
   if (b->hist_mem) (void)hipFree(b->hist_mem);
   b->hist_mem = nullptr;
   b->d.e_hist = nullptr;
@@ -855,8 +895,7 @@ int sg_scf_set_diis(sg_batch* b, int depth) {
   HIP_TRY(hipSetDevice(b->ctx->device));
   HIP_TRY(hipStreamSynchronize(b->ctx->stream));
   // the captured iteration holds the old depth and buffers
-  if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
-  b->iter_exec = nullptr;
+  drop_graphs(b);
   if (b->diis_mem) (void)hipFree(b->diis_mem);
   b->diis_mem = nullptr;
   BatchDev& d = b->d;
@@ -935,50 +974,87 @@ int sg_scf_run_policy(sg_batch* b, double threshold, int max_steps, int policy, 
     launch_diis_reset(b->d, s);
   }
   if ((r = check_launch("scf init"))) return r;
-  int n_active = max_steps > 0 ? b->d.n_mol : 0;
-  int host_active = 0;
   const int fp32_cap = 40;
-  // Without stage timing, one iteration is captured once as a HIP graph and
-  // replayed; its kernel arguments depend on the batch and on these settings.
-  // Batches with molecules on the multi-workgroup eigensolver (thousands of
-  // launches per iteration) launch directly.
   b->d.mixed = policy == SG_POLICY_FP64 ? 0 : 1;
   b->d.eig32 = policy == SG_POLICY_MIXED_EIG32 ? 1 : 0;
+  // The molecules iterate as one group on the context's stream, or as two
+  // (split_views) on it and on the side stream, each with its own iteration
+  // and active count, so that neither group waits for the other.
+  BatchDev views[2];
+  hipStream_t streams[2] = {s, c->side};
+  int n_groups = 1;
+  if (can_split(c, b->d)) {
+    split_views(b->d, views[0], views[1]);
+    n_groups = 2;
+  } else {
+    views[0] = b->d;
+  }
+  // Without stage timing, each group's iteration is captured once as a HIP
+  // graph and replayed; its kernel arguments depend on the batch and on these
+  // settings.  Batches with molecules on the multi-workgroup eigensolver
+  // (thousands of launches per iteration) launch directly.
   const bool use_graph = c->graphs && !c->timing && b->d.n_large == 0;
   if (use_graph && (!b->iter_exec || b->g_threshold != threshold || b->g_max_steps != max_steps ||
                     b->g_switch != switch_ev || b->g_mixed != policy ||
                     b->g_damping != b->d.damping)) {
-    if (b->iter_exec) (void)hipGraphExecDestroy(b->iter_exec);
-    b->iter_exec = nullptr;
-    hipGraph_t g = nullptr;
-    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    scf_iteration(c, b->d, s, threshold, max_steps, switch_ev, fp32_cap, true);
-    HIP_TRY(hipStreamEndCapture(s, &g));
-    if ((r = check_launch("scf step capture"))) {
-      (void)hipGraphDestroy(g);
-      return r;
-    }
-    hipError_t ge = hipGraphInstantiate(&b->iter_exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (ge != hipSuccess) {
-      b->iter_exec = nullptr;
-      return fail(SG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ge));
-    }
+    drop_graphs(b);
+    for (int g = 0; g < n_groups; ++g)
+      if ((r = capture_iteration(c, views[g], streams[g], threshold, max_steps, switch_ev,
+                                 fp32_cap, g ? &b->iter_exec2 : &b->iter_exec))) {
+        drop_graphs(b);
+        return r;
+      }
     b->g_threshold = threshold;
     b->g_max_steps = max_steps;
     b->g_switch = switch_ev;
     b->g_mixed = policy;
     b->g_damping = b->d.damping;
   }
-  for (int it = 0; it < max_steps && n_active > 0; ++it) {
+  if (n_groups == 2) {  // the side stream starts after the set-up above
+    HIP_TRY(hipEventRecord(c->ev_fork, s));
+    HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+  }
+  // Each group keeps up to two iterations in flight on its stream: the host
+  // reads a group's active count (a copy to pinned memory, then an event) one
+  // iteration behind its launches, so the device does not idle while the host
+  // reads.  The iteration launched after a group's count reached zero finds no
+  // active molecule and changes nothing.  Multi-workgroup batches keep one.
+  const int depth = b->d.n_large ? 1 : 2;
+  int launched[2] = {0, 0}, read[2] = {0, 0};
+  const bool none = max_steps == 0 || b->d.n_mol == 0;
+  bool done[2] = {none, none};
+  std::deque<int> fifo;  // the group of each iteration in flight, oldest first
+  auto launch = [&](int g) -> int {
+    const int slot = 2 * g + (launched[g] & 1);
     if (use_graph)
-      HIP_TRY(hipGraphLaunch(b->iter_exec, s));
+      HIP_TRY(hipGraphLaunch(g ? b->iter_exec2 : b->iter_exec, streams[g]));
     else
-      scf_iteration(c, b->d, s, threshold, max_steps, switch_ev, fp32_cap, true);
-    if ((r = check_launch("scf step"))) return r;
-    HIP_TRY(hipMemcpyAsync(&host_active, b->d.n_active, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    n_active = host_active;
+      scf_iteration(c, views[g], streams[g], threshold, max_steps, switch_ev, fp32_cap);
+    int rc;
+    if ((rc = check_launch("scf step"))) return rc;
+    HIP_TRY(hipMemcpyAsync(c->host_counts + slot, views[g].n_active, sizeof(int),
+                           hipMemcpyDeviceToHost, streams[g]));
+    HIP_TRY(hipEventRecord(c->ev_count[slot], streams[g]));
+    ++launched[g];
+    fifo.push_back(g);
+    return SG_OK;
+  };
+  for (int g = 0; g < n_groups; ++g)
+    for (int k = 0; k < depth && !done[g] && launched[g] < max_steps; ++k)
+      if ((r = launch(g))) return r;
+  while (!fifo.empty()) {
+    const int g = fifo.front();
+    fifo.pop_front();
+    const int slot = 2 * g + (read[g] & 1);
+    HIP_TRY(hipEventSynchronize(c->ev_count[slot]));
+    ++read[g];
+    if (done[g]) continue;
+    if (c->host_counts[slot] == 0) done[g] = true;
+    if (!done[g] && launched[g] < max_steps && (r = launch(g))) return r;
+  }
+  if (n_groups == 2) {
+    HIP_TRY(hipEventRecord(c->ev_join, c->side));
+    HIP_TRY(hipStreamWaitEvent(s, c->ev_join, 0));
   }
   b->have_scf = true;
   if (res) {

@@ -40,7 +40,8 @@ constexpr int kErrK = 16;     // k columns staged through LDS per pass
 // v_mfma_f64_16x16x4f64 tiles (operand lane (r, q) = row r & 15, k column q;
 // result lane (c, q) holds rows 4 i + q, column c):
 //   e_ij = sum_k F_ik P_jk - P_ik F_jk   (P_kj = P_jk, F_kj = F_jk)
-__global__ void __launch_bounds__(256) k_diis_err(const MolDesc* __restrict__ mols,
+__global__ void __launch_bounds__(256) k_diis_err(const int* __restrict__ sel,
+                                                  const MolDesc* __restrict__ mols,
                                                   const double* __restrict__ F,
                                                   const double* __restrict__ P,
                                                   double* __restrict__ histF,
@@ -48,7 +49,7 @@ __global__ void __launch_bounds__(256) k_diis_err(const MolDesc* __restrict__ mo
                                                   const int* __restrict__ kcount, int depth,
                                                   const int* __restrict__ active,
                                                   const int* __restrict__ status) {
-  const int m = blockIdx.y;
+  const int m = sel ? sel[blockIdx.y] : blockIdx.y;
   if (!mol_live(active, status, m)) return;
   const MolDesc M = mols[m];
   const int n = M.n_ao, nt = (n + kErrTile - 1) / kErrTile;
@@ -129,13 +130,14 @@ __global__ void __launch_bounds__(256) k_diis_err(const MolDesc* __restrict__ mo
 }
 
 // part[(m * kChunks + chunk) * kDiisMax + j] = chunk's share of <e_s, e_j>.
-__global__ void __launch_bounds__(256) k_diis_dot(const MolDesc* __restrict__ mols,
+__global__ void __launch_bounds__(256) k_diis_dot(const int* __restrict__ sel,
+                                                  const MolDesc* __restrict__ mols,
                                                   const double* __restrict__ histE, int64_t stride,
                                                   const int* __restrict__ kcount, int depth,
                                                   double* __restrict__ part,
                                                   const int* __restrict__ active,
                                                   const int* __restrict__ status) {
-  const int m = blockIdx.y;
+  const int m = sel ? sel[blockIdx.y] : blockIdx.y;
   if (!mol_live(active, status, m)) return;
   const MolDesc M = mols[m];
   const int n = M.n_ao;
@@ -169,12 +171,15 @@ __global__ void __launch_bounds__(256) k_diis_dot(const MolDesc* __restrict__ mo
 }
 
 // One thread per molecule: B update, the bordered system, the coefficients.
-__global__ void k_diis_solve(int n_mol, int* __restrict__ kcount, int depth,
+__global__ void k_diis_solve(int n_mol, const int* __restrict__ sel, int* __restrict__ kcount,
+                             int depth,
                              const double* __restrict__ part, double* __restrict__ B,
                              double* __restrict__ coef, const int* __restrict__ active,
                              const int* __restrict__ status) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n_mol || !mol_live(active, status, m)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_mol) return;
+  const int m = sel ? sel[i] : i;
+  if (!mol_live(active, status, m)) return;
   const int kc = kcount[m];
   const int s = kc % depth, nv = min(kc + 1, depth);
   double* b = B + (int64_t)m * kDiisMax * kDiisMax;
@@ -242,13 +247,14 @@ __global__ void k_diis_solve(int n_mol, int* __restrict__ kcount, int depth,
   kcount[m] = kc + 1;
 }
 
-__global__ void __launch_bounds__(256) k_diis_mix(const MolDesc* __restrict__ mols,
+__global__ void __launch_bounds__(256) k_diis_mix(const int* __restrict__ sel,
+                                                  const MolDesc* __restrict__ mols,
                                                   double* __restrict__ F,
                                                   const double* __restrict__ histF, int64_t stride,
                                                   const double* __restrict__ coef,
                                                   const int* __restrict__ active,
                                                   const int* __restrict__ status) {
-  const int m = blockIdx.y;
+  const int m = sel ? sel[blockIdx.y] : blockIdx.y;
   if (!mol_live(active, status, m)) return;
   const MolDesc M = mols[m];
   const int64_t nn = (int64_t)M.n_ao * M.n_ao;
@@ -277,11 +283,13 @@ __global__ void k_diis_reset(int n_mol, int* __restrict__ kcount, int* __restric
 
 // The history holds Fock matrices of one precision: it restarts when the
 // precision of the step just done differs from the stored ones'.
-__global__ void k_diis_begin(int n_mol, int* __restrict__ kcount, int* __restrict__ hist_prec,
-                             const int* __restrict__ step_prec, const int* __restrict__ active,
-                             const int* __restrict__ status) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n_mol || !mol_live(active, status, m)) return;
+__global__ void k_diis_begin(int n_mol, const int* __restrict__ sel, int* __restrict__ kcount,
+                             int* __restrict__ hist_prec, const int* __restrict__ step_prec,
+                             const int* __restrict__ active, const int* __restrict__ status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_mol) return;
+  const int m = sel ? sel[i] : i;
+  if (!mol_live(active, status, m)) return;
   if (hist_prec[m] != step_prec[m]) {
     kcount[m] = 0;
     hist_prec[m] = step_prec[m];
@@ -298,16 +306,19 @@ void launch_diis_reset(const BatchDev& b, hipStream_t s, const int* mask) {
 void launch_diis(const BatchDev& b, hipStream_t s) {
   if (!b.n_mol || !b.diis_m) return;
   const int nt = (b.max_n + kErrTile - 1) / kErrTile;
-  k_diis_begin<<<(b.n_mol + 255) / 256, 256, 0, s>>>(b.n_mol, b.diis_k, b.diis_p, b.prev_prec,
-                                                     b.active, b.status);
-  k_diis_err<<<dim3(nt * (nt + 1) / 2, b.n_mol), 256, 0, s>>>(
-      b.mols, b.F, b.P, b.diis_F, b.diis_E, b.diis_stride, b.diis_k, b.diis_m, b.active, b.status);
-  k_diis_dot<<<dim3(kChunks, b.n_mol), 256, 0, s>>>(b.mols, b.diis_E, b.diis_stride, b.diis_k, b.diis_m,
-                                                    b.diis_part, b.active, b.status);
-  k_diis_solve<<<(b.n_mol + 63) / 64, 64, 0, s>>>(b.n_mol, b.diis_k, b.diis_m, b.diis_part,
-                                                  b.diis_B, b.diis_c, b.active, b.status);
-  k_diis_mix<<<dim3(kChunks, b.n_mol), 256, 0, s>>>(b.mols, b.F, b.diis_F, b.diis_stride, b.diis_c,
-                                                    b.active, b.status);
+  k_diis_begin<<<(b.n_mol + 255) / 256, 256, 0, s>>>(b.n_mol, b.sel, b.diis_k, b.diis_p,
+                                                     b.prev_prec, b.active, b.status);
+  k_diis_err<<<dim3(nt * (nt + 1) / 2, b.n_mol), 256, 0, s>>>(b.sel, b.mols, b.F, b.P, b.diis_F,
+                                                              b.diis_E, b.diis_stride, b.diis_k,
+                                                              b.diis_m, b.active, b.status);
+  k_diis_dot<<<dim3(kChunks, b.n_mol), 256, 0, s>>>(b.sel, b.mols, b.diis_E, b.diis_stride,
+                                                    b.diis_k, b.diis_m, b.diis_part, b.active,
+                                                    b.status);
+  k_diis_solve<<<(b.n_mol + 63) / 64, 64, 0, s>>>(b.n_mol, b.sel, b.diis_k, b.diis_m,
+                                                  b.diis_part, b.diis_B, b.diis_c, b.active,
+                                                  b.status);
+  k_diis_mix<<<dim3(kChunks, b.n_mol), 256, 0, s>>>(b.sel, b.mols, b.F, b.diis_F, b.diis_stride,
+                                                    b.diis_c, b.active, b.status);
 }
 
 int64_t diis_part_doubles(int n_mol) { return (int64_t)n_mol * kChunks * kDiisMax; }

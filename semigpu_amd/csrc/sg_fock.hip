@@ -548,11 +548,13 @@ k_fock_mol(int n_mol, const int* __restrict__ order, const MolDesc* __restrict__
 }
 // (ElectronicEnergy::ComputeEnergy, electronicenergy.cpp:12-36, of the F
 // just built).
-__global__ void k_energy_atoms(int n_mol, const MolDesc* __restrict__ mols,
+__global__ void k_energy_atoms(int n_mol, const int* __restrict__ sel,
+                               const MolDesc* __restrict__ mols,
                                const double* __restrict__ e_atom, double* __restrict__ out,
                                const int* __restrict__ active, const int* __restrict__ status) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n_mol) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_mol) return;
+  const int m = sel ? sel[i] : i;
   if (active && (!active[m] || status[m])) return;
   const MolDesc M = mols[m];
   const double* ea = e_atom + M.atom0;
@@ -683,16 +685,19 @@ __global__ void k_set_prec(int n_mol, int v, int* __restrict__ prec, const int* 
 // with FP32 contractions never counts as converged; the molecule switches to
 // FP64 once |dE| < switch_ev (or after fp32_cap FP32 steps), and convergence
 // is tested only between two FP64 steps.
-__global__ void k_scf_update(int n_mol, double threshold, int max_steps, double switch_ev,
-                             int fp32_cap, const double* __restrict__ e_new,
+__global__ void k_scf_update(int n_mol, const int* __restrict__ sel, double threshold,
+                             int max_steps, double switch_ev, int fp32_cap,
+                             const double* __restrict__ e_new,
                              double* __restrict__ e_el, int* __restrict__ steps,
                              int* __restrict__ active, int* __restrict__ converged,
                              const int* __restrict__ status, int* __restrict__ n_active,
                              int* __restrict__ prec, int* __restrict__ prev_prec,
                              int* __restrict__ steps32, double* __restrict__ de_prev,
                              double* __restrict__ e_hist, int e_hist_cap) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n_mol || !active[m]) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_mol) return;
+  const int m = sel ? sel[i] : i;
+  if (!active[m]) return;
   if (status[m]) {
     active[m] = 0;
     return;
@@ -813,7 +818,7 @@ void set_launch_shapes(BatchDev& d, const MolDesc* mols, int n, const int* used)
 }
 
 void launch_energy_fock(const BatchDev& b, hipStream_t s, double* out, bool all, const int* mask) {
-  k_energy_atoms<<<blocks_for(b.n_mol, 64), 64, 0, s>>>(b.n_mol, b.mols, b.e_atom, out,
+  k_energy_atoms<<<blocks_for(b.n_mol, 64), 64, 0, s>>>(b.n_mol, b.sel, b.mols, b.e_atom, out,
                                                         mask ? mask : (all ? nullptr : b.active),
                                                         b.status);
 }
@@ -857,11 +862,10 @@ void launch_set_prec(const BatchDev& b, hipStream_t s, int v, const int* mask) {
 
 void launch_scf_update(const BatchDev& b, hipStream_t s, double threshold, int max_steps,
                        double switch_ev, int fp32_cap) {
-  k_scf_update<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(b.n_mol, threshold, max_steps, switch_ev,
-                                                        fp32_cap, b.e_new, b.e_el, b.steps,
-                                                        b.active, b.converged, b.status, b.n_active,
-                                                        b.prec, b.prev_prec, b.steps32,
-                                                        b.de_prev, b.e_hist, b.e_hist_cap);
+  k_scf_update<<<blocks_for(b.n_mol, 256), 256, 0, s>>>(
+      b.n_mol, b.sel, threshold, max_steps, switch_ev, fp32_cap, b.e_new, b.e_el, b.steps,
+      b.active, b.converged, b.status, b.n_active, b.prec, b.prev_prec, b.steps32, b.de_prev,
+      b.e_hist, b.e_hist_cap);
 }
 
 }  // namespace sg

@@ -45,6 +45,9 @@ inline int reduced_prec(int policy) {
 // Streams a context keeps for the size classes of its servers (class i takes
 // stream i; classes beyond these create their own).
 constexpr int kCtxClassStreams = 3;
+// Where the second group's active count lives in a batch's count buffer (ints
+// from n_active; the buffer holds 64).
+constexpr int kSplitCountOffset = 32;
 
 struct sg_context {
   int device = 0;
@@ -57,10 +60,14 @@ struct sg_context {
   // classes of a server overlap on the device with the default queue count.
   hipStream_t class_streams[kCtxClassStreams] = {};
   // A second stream for the SCF of a batch (its small molecules beside the
-  // large ones, scf_iteration), created right after `stream` so that the two
-  // run on different hardware queues, and the events that fork and join them.
+  // large ones, split_views), created right after `stream` so that the two
+  // run on different hardware queues; the events that fork and join them; and
+  // per group and in-flight iteration, a pinned active count and the event
+  // after its copy.
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int* host_counts = nullptr;  // 4 ints
+  hipEvent_t ev_count[4] = {};
   bool timing = false;
   int large_min = sg::kMaxBasis + 1;  // batches created from now on
   bool graphs = true;                 // capture SCF iterations as HIP graphs
@@ -112,11 +119,6 @@ struct StageTimer {
   }
 };
 
-// One SCF iteration of the active molecules (scfcalculation.cpp:82-146):
-// eigensolve, density, Fock, energy, convergence bookkeeping.
-inline void scf_iteration(sg_context* c, const BatchDev& d, hipStream_t s, double threshold,
-                          int max_steps, double switch_ev, int fp32_cap, bool split);
-
 // The eigensolver's kernels, each timed as its own stage (the reduction as
 // its head, on-chip and FP32 launches).
 inline void timed_eigh(sg_context* c, const BatchDev& d, hipStream_t s, bool all) {
@@ -164,43 +166,39 @@ inline void scf_solve_build(sg_context* c, const BatchDev& d, hipStream_t s) {
 // The molecules of a batch in two groups by the launch order (decreasing N):
 // those above kRegRows, whose reduction starts with k_tridiag (bound by HBM),
 // and the rest (bound by the issue of their on-chip reduction and the other
-// stages).  Each is a view of the batch over its range of the order; the
-// reduction's bucket ranges are shifted to it.
+// stages).  Each is a view of the batch over its range of the order (sel),
+// with the reduction's bucket ranges shifted to it and an active count of its
+// own.
 inline void split_views(const BatchDev& d, BatchDev& big, BatchDev& small) {
   const int nb = d.tri_head;
   big = d;
   big.n_mol = nb;
+  big.sel = d.order;
   for (int k = 1; k <= kTriBuckets; ++k) big.tri_start[k] = nb;
   big.tri_start[0] = 0;
   small = d;
   small.order = d.order + nb;
+  small.sel = small.order;
   small.n_mol = d.n_mol - nb;
   small.max_n = kRegRows;  // a bound: every molecule here has N <= kRegRows
   small.tri_head = 0;
   for (int k = 0; k <= kTriBuckets; ++k) small.tri_start[k] = std::max(0, d.tri_start[k] - nb);
+  small.n_active = d.n_active + kSplitCountOffset;
 }
 
-// split: a batch with molecules on both sides of kRegRows (and none on the
-// multi-workgroup eigensolver, no damping) runs the two groups' eigensolve,
-// density and Fock build on two streams at once, the small molecules on the
-// context's side stream, and joins them for the energies and the convergence
-// test; every molecule runs the same kernels as otherwise, so its results do
-// not change.
+// Whether the SCF of a batch runs as two groups (split_views) on two streams:
+// molecules on both sides of kRegRows, none on the multi-workgroup
+// eigensolver, no damping.
+inline bool can_split(const sg_context* c, const BatchDev& d) {
+  return c->side && d.order && d.n_large == 0 && d.damping == 0.0 && d.tri_head > 0 &&
+         d.tri_head < d.n_mol;
+}
+
+// One SCF iteration of the active molecules of d (scfcalculation.cpp:82-146):
+// eigensolve, density, Fock, energy, convergence bookkeeping, on stream s.
 inline void scf_iteration(sg_context* c, const BatchDev& d, hipStream_t s, double threshold,
-                          int max_steps, double switch_ev, int fp32_cap, bool split) {
-  if (split && c->side && d.order && d.n_large == 0 && d.damping == 0.0 && d.tri_head > 0 &&
-      d.tri_head < d.n_mol) {
-    BatchDev big, small;
-    split_views(d, big, small);
-    (void)hipEventRecord(c->ev_fork, s);
-    (void)hipStreamWaitEvent(c->side, c->ev_fork, 0);
-    scf_solve_build(c, small, c->side);
-    scf_solve_build(c, big, s);
-    (void)hipEventRecord(c->ev_join, c->side);
-    (void)hipStreamWaitEvent(s, c->ev_join, 0);
-  } else {
-    scf_solve_build(c, d, s);
-  }
+                          int max_steps, double switch_ev, int fp32_cap) {
+  scf_solve_build(c, d, s);
   {
     StageTimer t(c, ST_ENERGY, s);
     launch_energy_fock(d, s, d.e_new, false);

@@ -133,6 +133,10 @@ struct BatchDev {
   // eigensolver and density launches take their workgroups in this order, so
   // the longest jobs start first and the launch tail is short.
   const int* order = nullptr;
+  // A view over some of a batch's molecules: the kernels that index molecules
+  // directly (energies, SCF control, DIIS) take molecule sel[i] for i < n_mol
+  // (null: molecule i).
+  const int* sel = nullptr;
   // Molecules with N >= large_min take the multi-workgroup eigensolver (one
   // molecule at a time over the whole GPU).  They are the first n_large
   // entries of `order`; large_idx / large_mols are host copies of their

@@ -527,7 +527,7 @@ int lane_launch(sg_server* S, Lane* L) {
     int r = check_launch("server setup");
     if (r) return r;
   }
-  scf_iteration(c, it, s, S->threshold, S->max_steps, S->switch_ev, 40, false);
+  scf_iteration(c, it, s, S->threshold, S->max_steps, S->switch_ev, 40);
   int r = check_launch("server iteration");
   if (r) return r;
   HIP_TRY(hipMemcpyAsync(L->rb_act.p, it.active, 4 * hw, hipMemcpyDeviceToHost, s));
