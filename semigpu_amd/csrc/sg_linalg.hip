@@ -1123,6 +1123,14 @@ __global__ void __launch_bounds__(256) k_li_mgs(const MolDesc* __restrict__ mols
 // result does not depend on its batch), and the launch allocates what its
 // largest molecule needs (two workgroups per CU up to N = 384, where G = 16).
 constexpr int kBackBudget = 20480;  // doubles (160 KB); twice as many floats
+// Loads in flight per thread (V staging entries, W1 k-steps, update tiles):
+// more in the wide form (113 registers, launches with two workgroups per CU),
+// fewer in the narrow one (68 registers, launches with three or more).  Both
+// issue the same products in the same order.
+template <bool WIDE>
+struct BackShape {
+  static constexpr int kStage = WIDE ? 8 : 4, kSteps = WIDE ? 8 : 4, kTiles = WIDE ? 4 : 1;
+};
 __host__ __device__ inline int bt_need(int g, int n, int k) {
   return g * n + (g * k > 2048 ? g * k : 2048) + 512;
 }
@@ -1135,13 +1143,15 @@ __host__ __device__ inline int bt_group(int n, int k, int budget) {
 // In FP32 steps the reflectors are FP32 (first half of the work-matrix room)
 // and Y is converted once into FP32 vectors at 2 N^2 (the inverse-iteration
 // factors' room, free by then), where the density reads them.
-template <typename R>
+template <bool WIDE, typename R>
 __device__ __forceinline__ void back_mol(const MolDesc& M, const R* __restrict__ A,
                                          const double* __restrict__ Ysrc, R* __restrict__ Y,
                                          const double* __restrict__ tau, int lo, int k,
                                          R* lds) {
   typedef Mfma16<R> Mf;
   typedef typename Mf::v4 v4;
+  constexpr int kStage = BackShape<WIDE>::kStage, kSteps = BackShape<WIDE>::kSteps,
+                kTiles = BackShape<WIDE>::kTiles;
   const int n = M.n_ao, tid = threadIdx.x, nt = blockDim.x;
   const int wave = tid >> 6, lane = tid & 63, nwave = nt >> 6;
   const int lr = lane & 15, lk = lane >> 4;
@@ -1158,9 +1168,22 @@ __device__ __forceinline__ void back_mol(const MolDesc& M, const R* __restrict__
   if (nref == 0) return;  // Q = I
   for (int k0 = ((nref - 1) / G) * G; k0 >= 0; k0 -= G) {
     const int b = min(G, nref - k0);
-    for (int t = tid; t < G * n; t += nt) {
-      const int q = t / n, i = t - q * n, c = k0 + q;
-      Vs[t] = (q < b && i > c) ? A[(int64_t)c * n + i] : (R)0;
+    // V into LDS, kStage entries per thread in flight (addresses clamped into
+    // the molecule's matrix, unused entries zeroed after the load)
+    for (int t0 = tid; t0 < G * n; t0 += kStage * nt) {
+      R v[kStage];
+#pragma unroll
+      for (int u = 0; u < kStage; ++u) {
+        const int t = min(t0 + u * nt, G * n - 1);
+        const int q = t / n, i = t - q * n;
+        v[u] = A[(int64_t)min(k0 + q, n - 1) * n + i];
+      }
+#pragma unroll
+      for (int u = 0; u < kStage; ++u) {
+        const int t = t0 + u * nt;
+        const int q = t / n, i = t - q * n;
+        if (t < G * n) Vs[t] = (q < b && i > k0 + q) ? v[u] : (R)0;
+      }
     }
     for (int t = tid; t < 256; t += nt) Tm[t] = (R)0;
     __syncthreads();
@@ -1200,12 +1223,24 @@ __device__ __forceinline__ void back_mol(const MolDesc& M, const R* __restrict__
     // W2 = T (V Y): one 16-column tile of Y per wave
     for (int jt = wave; jt < KT; jt += nwave) {
       const int j = 16 * jt + lr;
+      const bool jok = j < k, gok = lr < G;
+      const R* yc = Y + (jok ? j : 0);
+      const R* vr = Vs + (gok ? lr : 0) * n;
       v4 w1 = {0, 0, 0, 0};
-      for (int i0 = k0 + 1; i0 < n; i0 += 4) {
-        const int i = i0 + lk;
-        const R a = (lr < G && i < n) ? Vs[lr * n + i] : (R)0;
-        const R y = (i < n && j < k) ? Y[(int64_t)i * k + j] : (R)0;
-        w1 = Mf::op(a, y, w1);
+      // kSteps k-steps per round: their loads in flight together, then the
+      // instructions in order
+      for (int i0 = k0 + 1; i0 < n; i0 += 4 * kSteps) {
+        R a[kSteps], y[kSteps];
+#pragma unroll
+        for (int u = 0; u < kSteps; ++u) {
+          const int i = i0 + 4 * u + lk, ic = min(i, n - 1);
+          const R av = vr[ic], yv = yc[(int64_t)ic * k];
+          a[u] = (gok && i < n) ? av : (R)0;
+          y[u] = (jok && i < n) ? yv : (R)0;
+        }
+#pragma unroll
+        for (int u = 0; u < kSteps; ++u)
+          if (i0 + 4 * u < n) w1 = Mf::op(a[u], y[u], w1);
       }
       v4 w2 = {0, 0, 0, 0};
 #pragma unroll
@@ -1217,44 +1252,53 @@ __device__ __forceinline__ void back_mol(const MolDesc& M, const R* __restrict__
       }
     }
     __syncthreads();
-    // Y -= V^T W2 over rows k0+1..n-1
+    // Y -= V^T W2 over rows k0+1..n-1, kTiles tiles of a wave per round:
+    // their loads in flight together (addresses clamped into Y), then the
+    // products and stores tile by tile
     {
       const int IT = (n - (k0 + 1) + 15) >> 4;
       const int ntile = IT * KT;
-      auto load = [&](int t, v4& v) {
-        const int I0 = k0 + 1 + 16 * (t / KT), col = 16 * (t % KT) + lr;
+      for (int t0 = wave; t0 < ntile; t0 += kTiles * nwave) {
+        v4 c[kTiles];
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-          const int row = I0 + Mf::row(lane, rr);
-          v[rr] = (row < n && col < k) ? Y[(int64_t)row * k + col] : (R)0;
-        }
-      };
-      v4 cur = {0, 0, 0, 0}, nxt = {0, 0, 0, 0};
-      if (wave < ntile) load(wave, cur);
-      for (int t = wave; t < ntile; t += nwave) {
-        if (t + nwave < ntile) load(t + nwave, nxt);
-        const int I0 = k0 + 1 + 16 * (t / KT), J0 = 16 * (t % KT);
+        for (int u = 0; u < kTiles; ++u) {
+          const int t = t0 + u * nwave;
+          const int I0 = k0 + 1 + 16 * (t / KT), col = 16 * (t % KT) + lr;
+          const bool cok = col < k;
 #pragma unroll
-        for (int st = 0; st < 4; ++st) {
-          const int g = 4 * st + lk;
-          const int i = I0 + lr, j = J0 + lr;
-          const R a = (g < G && i < n) ? -Vs[g * n + i] : (R)0;
-          const R w = (g < G && j < k) ? Rw[g * k + j] : (R)0;
-          cur = Mf::op(a, w, cur);
+          for (int rr = 0; rr < 4; ++rr) {
+            const int row = I0 + Mf::row(lane, rr);
+            const R yv = Y[(int64_t)min(row, n - 1) * k + (cok ? col : 0)];
+            c[u][rr] = (row < n && cok) ? yv : (R)0;
+          }
         }
-        const int col = J0 + lr;
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-          const int row = I0 + Mf::row(lane, rr);
-          if (row < n && col < k) Y[(int64_t)row * k + col] = cur[rr];
+        for (int u = 0; u < kTiles; ++u) {
+          const int t = t0 + u * nwave;
+          if (t >= ntile) break;
+          const int I0 = k0 + 1 + 16 * (t / KT), J0 = 16 * (t % KT);
+#pragma unroll
+          for (int st = 0; st < 4; ++st) {
+            const int g = 4 * st + lk;
+            const int i = I0 + lr, j = J0 + lr;
+            const R a = (g < G && i < n) ? -Vs[g * n + i] : (R)0;
+            const R w = (g < G && j < k) ? Rw[g * k + j] : (R)0;
+            c[u] = Mf::op(a, w, c[u]);
+          }
+          const int col = J0 + lr;
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            const int row = I0 + Mf::row(lane, rr);
+            if (row < n && col < k) Y[(int64_t)row * k + col] = c[u][rr];
+          }
         }
-        cur = nxt;
       }
     }
     __syncthreads();
   }
 }
 
+template <bool WIDE>
 __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict__ mols,
                                                        double* __restrict__ work,
                                                        const double* __restrict__ tauA,
@@ -1274,10 +1318,10 @@ __global__ void __launch_bounds__(kT) k_backtransform(const MolDesc* __restrict_
   double* base = work + 5 * M.mat0;
   double* Y = base + (int64_t)n * n;
   if (prec && prec[m])
-    back_mol<float>(M, (const float*)base, Y, (float*)(base + 2 * (int64_t)n * n),
-                    tauA + M.vec0, lo, k, (float*)lds);
+    back_mol<WIDE, float>(M, (const float*)base, Y, (float*)(base + 2 * (int64_t)n * n),
+                          tauA + M.vec0, lo, k, (float*)lds);
   else
-    back_mol<double>(M, base, Y, Y, tauA + M.vec0, lo, k, lds);
+    back_mol<WIDE, double>(M, base, Y, Y, tauA + M.vec0, lo, k, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1818,8 +1862,11 @@ void launch_eigh_back(const BatchDev& b, hipStream_t s, bool all) {
       const size_t b32 = 4 * (size_t)bt_need(bt_group(n, k, 2 * kBackBudget), n, k);
       bytes = std::max(bytes, std::max(b64, b32));
     }
-    k_backtransform<<<p.count, kT, bytes, s>>>(b.mols, b.W, b.tau, all ? 1 : 0, act, b.status,
-                                               p.order, (all || !b.eig32) ? nullptr : b.prec);
+    // the wide form where the LDS allows two workgroups per CU, the narrow
+    // one where it allows three or more
+    const auto kern = 3 * bytes <= 160 * 1024 ? k_backtransform<false> : k_backtransform<true>;
+    kern<<<p.count, kT, bytes, s>>>(b.mols, b.W, b.tau, all ? 1 : 0, act, b.status, p.order,
+                                    (all || !b.eig32) ? nullptr : b.prec);
   }
   for (int i = 0; i < b.n_large; ++i) launch_back_large(b, s, b.large_idx[i], b.large_mols[i], act, all);
   if (all) {
