@@ -15,10 +15,14 @@ Printed JSON (one line): the driver's contract fields plus
   roofline          the dominant stage, the FP64 Householder reduction of each
                     SCF iteration (head launch + on-chip bucket launches):
                     algorithmic flops per pass / average pass time, the time
-                    from HIP events around its launches on the library's
-                    stream during the timed steps; parts: head and on chip
+                    from HIP events around its launches in the breakdown step;
+                    parts: head and on chip
   fock_roofline     the Fock build's HBM rate (algorithmic bytes / kernel time)
   density_roofline  the density product's flop rate (algorithmic flops / time)
+  stage_ms_per_step the breakdown step's stages: one more step after the timed
+                    ones with every kernel alone on the device (one stream, HIP
+                    events around each stage; the timed steps overlap the large
+                    and the small molecules' kernels on two streams)
   resident_batch    the same SCF with the batch already resident (no setup)
   batched_fp64, mixed_bf16, mixed_eig32   the other policies, same step
   plain_fp64        one molecule at a time through the same kernels, each from
@@ -589,8 +593,6 @@ def main():
         full_step(eng, packed, args.threshold, args.policy)
         print("warmup %d: %.3f s" % (w, time.time() - t), file=sys.stderr, flush=True)
 
-    eng.timing(True)
-    eng.timing_reset()
     times = []
     res = s32 = None
     for k in range(args.steps):
@@ -598,13 +600,28 @@ def main():
         res, s32 = full_step(eng, packed, args.threshold, args.policy)
         times.append(time.perf_counter() - t)
         print("step %d: %.3f s" % (k, times[-1]), file=sys.stderr, flush=True)
+    step_s = float(np.mean(times))
+    # The stage breakdown and the kernels' rates: one more step with every
+    # kernel alone on the device (one stream, no graph capture) and HIP events
+    # around each stage.  The timed steps above overlap the large and the
+    # small molecules' kernels on two streams, so their stage times would
+    # overlap too.
+    eng.overlap(False)
+    eng.timing(True)
+    eng.timing_reset()
+    t = time.perf_counter()
+    res_b, _ = full_step(eng, packed, args.threshold, args.policy)
+    breakdown_s = time.perf_counter() - t
     stage = eng.timing_read()
     eng.timing(False)
-    step_s = float(np.mean(times))
+    eng.overlap(True)
+    print("breakdown step (one stream, stage events): %.3f s" % breakdown_s, file=sys.stderr,
+          flush=True)
+    assert [r["e_tot"] for r in res_b] == [r["e_tot"] for r in res]
     steps = np.array([r["steps"] for r in res])
     conv = np.array([r["converged"] for r in res])
     s32 = s32.astype(np.float64)
-    K = max(1, args.steps)
+    K = 1
     roofline = tridiag_roofline(stage, sizes, steps, s32, K)
 
     T = sizes * (sizes + 1) / 2
@@ -618,7 +635,9 @@ def main():
     fock_bytes_step = float(np.sum((steps + 1 - s32) * bf64 + s32 * bf32))
     fock_avg_ms = fock_ms / max(1, fock_cnt)
     fock_gbs = fock_bytes_step * K / max(1, fock_cnt) / (fock_avg_ms * 1e-3) / 1e9
-    fock_roof = {"kernel": "k_wdiag + k_fock", "bound": "hbm", "achieved": round(fock_gbs, 2),
+    fock_roof = {"kernel": "k_fock_mol<double> / k_fock_mol<float> (one workgroup per molecule; "
+                           "FP64 and FP32 builds in launches of their own)",
+                 "bound": "hbm", "achieved": round(fock_gbs, 2),
                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(fock_gbs / HBM_PEAK_GBS, 5),
                  "traffic": pmc_traffic("fock_traffic_bytes_per_launch"),
                  "avg_launch_ms": round(fock_avg_ms, 4),
@@ -746,6 +765,7 @@ def main():
         "fock_roofline": fock_roof,
         "density_roofline": den_roof,
         "stage_ms_per_step": stage_ms,
+        "breakdown_step_ms": round(breakdown_s * 1e3, 3),
         "converged": int(conv.sum()),
         "within_1e-4_eV_of_fp64": ok,
         "max_abs_dE_vs_fp64_eV": float(dE.max()),

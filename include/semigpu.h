@@ -84,6 +84,13 @@ int sg_ctx_tri_shapes(const sg_context* ctx, int* variant, int* count, int* nw, 
  * the cheaper choice for batches that run once, e.g. a single molecule).  The
  * results are the same either way.  No counterpart in the reference. */
 int sg_ctx_set_graphs(sg_context* ctx, int on);
+/* Whether sg_scf_run / sg_scf_run_policy iterate a batch whose molecules lie
+ * on both sides of 256 basis functions as two groups on two streams (on, the
+ * default: the large molecules' HBM-bound reduction overlaps the small
+ * molecules' on-chip one), or as one group on the context's stream (off: each
+ * kernel runs alone on the device, e.g. to time it).  The results are the
+ * same either way.  No counterpart in the reference. */
+int sg_ctx_set_overlap(sg_context* ctx, int on);
 
 /* Molecules: atoms of molecule m are atom_offsets[m] .. atom_offsets[m+1]-1 of
  * Z[] / xyz_ang[3*i..3*i+2].  Replaces ReadXYZFile + ListAtomicOrbitals::SetOrbitals

@@ -40,7 +40,7 @@ EXPORTS = (
     "sg_get_status", "sg_calibration_copy", "sg_timing_enable", "sg_timing_read", "sg_timing_reset",
     "sg_server_create", "sg_server_destroy", "sg_server_submit", "sg_server_step",
     "sg_server_poll", "sg_server_clock", "sg_server_set_diis", "sg_ctx_set_large_basis", "sg_server_create_classes",
-    "sg_heat_of_formation", "sg_ctx_set_graphs", "sg_eigh_noalloc", "sg_gradient",
+    "sg_heat_of_formation", "sg_ctx_set_graphs", "sg_ctx_set_overlap", "sg_eigh_noalloc", "sg_gradient",
     "sg_scf_record_energies", "sg_scf_energy_history", "sg_ctx_autotune",
     "sg_ctx_set_tri_variant", "sg_ctx_tri_shapes",
 )
@@ -97,6 +97,7 @@ def load_library(path: str = LIB_PATH):
     L.sg_ctx_destroy.argtypes = [_vp]
     L.sg_ctx_set_large_basis.argtypes = [_vp, ctypes.c_int]
     L.sg_ctx_set_graphs.argtypes = [_vp, ctypes.c_int]
+    L.sg_ctx_set_overlap.argtypes = [_vp, ctypes.c_int]
     L.sg_gradient.argtypes = [_vp, _dp]
     L.sg_scf_record_energies.argtypes = [_vp, ctypes.c_int]
     L.sg_ctx_autotune.argtypes = [_vp, ctypes.c_int, _dp, _ip]
@@ -290,6 +291,12 @@ class Engine:
     def graphs(self, on: bool = True):
         """Capture SCF iterations as HIP graphs (default) or launch them directly."""
         _check(self.L.sg_ctx_set_graphs(self.h, 1 if on else 0))
+
+    def overlap(self, on: bool = True):
+        """Iterate batches with molecules on both sides of 256 basis functions
+        as two groups on two streams (default), or as one group (each kernel
+        alone on the device)."""
+        _check(self.L.sg_ctx_set_overlap(self.h, 1 if on else 0))
 
     def timing(self, on: bool = True):
         _check(self.L.sg_timing_enable(self.h, 1 if on else 0))

@@ -352,6 +352,12 @@ int sg_ctx_set_graphs(sg_context* c, int on) {
   return SG_OK;
 }
 
+int sg_ctx_set_overlap(sg_context* c, int on) {
+  if (!c) return fail(SG_ERR_INVALID, "null context");
+  c->overlap = on != 0;
+  return SG_OK;
+}
+
 int sg_calibration_copy(sg_context* c, long long bytes) {
   if (!c || bytes <= 0 || bytes % 16) return fail(SG_ERR_INVALID, "sg_calibration_copy: bytes must be a positive multiple of 16");
   HIP_TRY(hipSetDevice(c->device));

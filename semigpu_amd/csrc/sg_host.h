@@ -71,6 +71,7 @@ struct sg_context {
   bool timing = false;
   int large_min = sg::kMaxBasis + 1;  // batches created from now on
   bool graphs = true;                 // capture SCF iterations as HIP graphs
+  bool overlap = true;                // two-stream iteration of split batches
   int tri_variant[sg::kTriBuckets] = {};  // launch shape per reduction bucket (autotuner)
   double ms[SG_N_STAGES] = {0};
   int counts[SG_N_STAGES] = {0};
@@ -187,11 +188,11 @@ inline void split_views(const BatchDev& d, BatchDev& big, BatchDev& small) {
 }
 
 // Whether the SCF of a batch runs as two groups (split_views) on two streams:
-// molecules on both sides of kRegRows, none on the multi-workgroup
-// eigensolver, no damping.
+// overlap on, molecules on both sides of kRegRows, none on the
+// multi-workgroup eigensolver, no damping.
 inline bool can_split(const sg_context* c, const BatchDev& d) {
-  return c->side && d.order && d.n_large == 0 && d.damping == 0.0 && d.tri_head > 0 &&
-         d.tri_head < d.n_mol;
+  return c->overlap && c->side && d.order && d.n_large == 0 && d.damping == 0.0 &&
+         d.tri_head > 0 && d.tri_head < d.n_mol;
 }
 
 // One SCF iteration of the active molecules of d (scfcalculation.cpp:82-146):

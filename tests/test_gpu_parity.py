@@ -127,6 +127,12 @@ def test_two_groups_equal_alone_bitwise(engine, policy):
                 assert alone == big[i], (graphs, i, sizes[i])
         finally:
             engine.graphs(True)
+    # the same batch as one group on one stream
+    engine.overlap(False)
+    try:
+        assert engine.batch(mols).scf(threshold=1e-6, policy=policy) == big
+    finally:
+        engine.overlap(True)
 
 
 def test_eigensolver_properties(engine):
