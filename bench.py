@@ -60,7 +60,7 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 matrix
 BF16_PEAK_TFLOPS = 2500.0  # MI355X BF16 matrix (dense)
 METRIC = "SCF molecules/s batched mixed vs plain FP64 at |dE|<=1e-4 eV; p50/p99 latency"
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_tridiag_reg_water4096_pmc.md")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "traffic_water4096_pmc.md")
 MFMA_SUMMARY = os.path.join(ROOT, "profiles", "k_density_water4096_mfma.md")
 REG_ROWS = 256            # trailing block the on-chip reduction holds (kRegRows)
 # Compiled reference CPU path (g++ -O2, MKL dsyev, one core) over the oracle's
