@@ -202,6 +202,10 @@ k_tridiag_reg(const MolDesc* __restrict__ mols, const double* __restrict__ F,
   };
   const auto reduce_write = [&](int I, int J, int id, const double (&rp)[4], const double (&cp)[4])
       __attribute__((always_inline)) {
+    // the tile's slot offset made opaque here, so that the compiler forms the
+    // LDS addresses below at each use instead of keeping one per tile in
+    // registers across the column loop (which spills them)
+    asm volatile("" : "+s"(id));
     // rows: over the 8 lanes of a lane row (bits 2, 1, 0 of the lane)
     {
       const bool b2 = lane & 4, b1 = lane & 2;
