@@ -321,9 +321,6 @@ __device__ __forceinline__ void fock_pair(const AtomDesc& A, const AtomDesc& B, 
   constexpr int SA = HA ? 10 : 1, SB = HB ? 10 : 1;
   const S* b = blk + block_offset(A, B);
   const int sb = jslot(B);
-  T wB[SB];
-#pragma unroll
-  for (int l = 0; l < SB; ++l) wB[l] = (T)wl[sb + l];
   T pab[NA][NB];
 #pragma unroll
   for (int x = 0; x < NA; ++x) load_run<NB>(p + (int64_t)(A.ao0 + x) * N + B.ao0, pab[x]);
@@ -338,27 +335,63 @@ __device__ __forceinline__ void fock_pair(const AtomDesc& A, const AtomDesc& B, 
   // H-X blocks)
   constexpr int RUN = SA * SB == 1 ? 1 : 10;
   T run[RUN];
+  // X-X blocks are ten rows of ten, each row group in a basic block of its
+  // own (an opaque branch), so that the compiler keeps one group of rows in
+  // registers at a time instead of issuing every load first.  In FP64 a group
+  // is two rows, whose loads are in flight together, and B's weights are read
+  // from LDS per group (an opaque offset keeps them out of registers across
+  // the groups); in FP32 (fewer registers per wave) one row.
+  if constexpr (HA && HB && sizeof(T) == 8) {
 #pragma unroll
-  for (int k = 0; k < SA; ++k) {
-    // each row of an X-X block in a basic block of its own (an opaque branch),
-    // so that the compiler keeps one row of the block in registers at a time
-    // instead of issuing every load first
-    int go = 1;
-    if constexpr (HB) asm volatile("" : "+s"(go));
-    if (!go) continue;
-    const int mu = mop_a(k), nu = mop_b(k);
-    if constexpr (HB) load_run<10>(b + 10 * k, run);
-    else if (k == 0) load_run<RUN>(b, run);
+    for (int k2 = 0; k2 < 10; k2 += 2) {
+      int go = 1;
+      asm volatile("" : "+s"(go));
+      if (!go) continue;
+      T r2[2][10];
+      load_run<10>(b + 10 * k2, r2[0]);
+      load_run<10>(b + 10 * (k2 + 1), r2[1]);
+      int o = sb;
+      asm volatile("" : "+v"(o));
 #pragma unroll
-    for (int l = 0; l < SB; ++l) {
-      const int b1 = mop_a(l), b2 = mop_b(l);
-      const T val = HB ? run[l] : run[k];
-      jA[k] += val * wB[l];
-      jB[l] += val * wA[k];
-      K[mu][b1] += val * pab[nu][b2];
-      if (mu != nu) K[nu][b1] += val * pab[mu][b2];
-      if (b1 != b2) K[mu][b2] += val * pab[nu][b1];
-      if (mu != nu && b1 != b2) K[nu][b2] += val * pab[mu][b1];
+      for (int h = 0; h < 2; ++h) {
+        const int k = k2 + h;
+        const int mu = mop_a(k), nu = mop_b(k);
+#pragma unroll
+        for (int l = 0; l < 10; ++l) {
+          const int b1 = mop_a(l), b2 = mop_b(l);
+          const T val = r2[h][l];
+          jA[k] += val * (T)wl[o + l];
+          jB[l] += val * wA[k];
+          K[mu][b1] += val * pab[nu][b2];
+          if (mu != nu) K[nu][b1] += val * pab[mu][b2];
+          if (b1 != b2) K[mu][b2] += val * pab[nu][b1];
+          if (mu != nu && b1 != b2) K[nu][b2] += val * pab[mu][b1];
+        }
+      }
+    }
+  } else {
+    T wB[SB];
+#pragma unroll
+    for (int l = 0; l < SB; ++l) wB[l] = (T)wl[sb + l];
+#pragma unroll
+    for (int k = 0; k < SA; ++k) {
+      int go = 1;
+      if constexpr (HB) asm volatile("" : "+s"(go));
+      if (!go) continue;
+      const int mu = mop_a(k), nu = mop_b(k);
+      if constexpr (HB) load_run<10>(b + 10 * k, run);
+      else if (k == 0) load_run<RUN>(b, run);
+#pragma unroll
+      for (int l = 0; l < SB; ++l) {
+        const int b1 = mop_a(l), b2 = mop_b(l);
+        const T val = HB ? run[l] : run[k];
+        jA[k] += val * wB[l];
+        jB[l] += val * wA[k];
+        K[mu][b1] += val * pab[nu][b2];
+        if (mu != nu) K[nu][b1] += val * pab[mu][b2];
+        if (b1 != b2) K[mu][b2] += val * pab[nu][b1];
+        if (mu != nu && b1 != b2) K[nu][b2] += val * pab[mu][b1];
+      }
     }
   }
   if (B.local == A.local) {
