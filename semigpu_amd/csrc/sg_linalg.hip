@@ -1478,7 +1478,12 @@ __device__ __forceinline__ void density_block_bf16(const double* __restrict__ Y,
   density_store<float>(acc, n, lo, i0, j0, lane, p, tr);
 }
 
-__global__ void __launch_bounds__(256) k_density(const MolDesc* __restrict__ mols,
+// REDUCED: this launch takes the molecules whose step runs in reduced
+// precision (FP32 or split BF16), else the FP64 ones; the two are separate
+// launches so that the FP64 one is not held to the registers of the others.
+template <bool REDUCED>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_density(const MolDesc* __restrict__ mols,
                                                   const double* __restrict__ work,
                                                   double* __restrict__ P,
                                                   const int* __restrict__ active,
@@ -1487,6 +1492,7 @@ __global__ void __launch_bounds__(256) k_density(const MolDesc* __restrict__ mol
                                                   const int* __restrict__ order, int float_y) {
   const int m = order ? order[blockIdx.y] : blockIdx.y;
   if (!mol_live(active, status, m)) return;
+  if ((prec && prec[m] != 0) != REDUCED) return;
   const MolDesc M = mols[m];
   const int n = M.n_ao;
   int lo, k;
@@ -1504,14 +1510,14 @@ __global__ void __launch_bounds__(256) k_density(const MolDesc* __restrict__ mol
   for (int blk = blockIdx.x * 4 + wave; blk < nblk; blk += gridDim.x * 4) {
     int bi = 0, bj = blk;
     while (bj > bi) bj -= ++bi;
-    if (pm == 2)
+    if constexpr (!REDUCED)
+      density_block<double>(Y, n, k, lo, 32 * bi, 32 * bj, lane, p, trans[wave]);
+    else if (pm == 2)
       density_block_bf16(Y, n, k, lo, 32 * bi, 32 * bj, lane, p, trans[wave]);
     else if (fp32 && float_y)
       density_block<float>(Yf, n, k, lo, 32 * bi, 32 * bj, lane, p, trans[wave]);
-    else if (fp32)
-      density_block<float>(Y, n, k, lo, 32 * bi, 32 * bj, lane, p, trans[wave]);
     else
-      density_block<double>(Y, n, k, lo, 32 * bi, 32 * bj, lane, p, trans[wave]);
+      density_block<float>(Y, n, k, lo, 32 * bi, 32 * bj, lane, p, trans[wave]);
   }
 }
 
@@ -1887,7 +1893,11 @@ static void density_part(const BatchDev& b, hipStream_t s, const int* order, int
   // as many workgroups per molecule as the largest one has 4-block groups
   const int t32 = (N + 31) >> 5;
   dim3 grid((t32 * (t32 + 1) / 2 + 3) / 4, count);
-  k_density<<<grid, 256, 0, s>>>(b.mols, b.W, b.P, b.active, b.status, b.prec, order, float_y);
+  k_density<false><<<grid, 256, 0, s>>>(b.mols, b.W, b.P, b.active, b.status, b.prec, order,
+                                         float_y);
+  if (b.mixed || b.eig32)
+    k_density<true><<<grid, 256, 0, s>>>(b.mols, b.W, b.P, b.active, b.status, b.prec, order,
+                                          float_y);
 }
 
 void launch_density(const BatchDev& b, hipStream_t s, bool all) {
