@@ -74,10 +74,7 @@ constexpr double kFp32BisectTol = 1e-10;
 // orthogonalisation, which settles each vector to working accuracy, and so
 // does the eigensolver of molecules above one workgroup (N > kMaxBasis),
 // where the shares of ~N eigenvalues add up in the density (one solve left
-// |P^2 - 2P| at 3.6e-9 for N = 1500, two leave ~1e-12).
-__host__ __device__ inline int invit_sweeps(bool fp32, int want_all) {
-  return fp32 || !want_all ? 1 : 2;
-}
+// |P^2 - 2P| at 3.6e-9 for N = 1500, where the test holds it to 1e-9).
 constexpr int kInvitSweepsLarge = 2;
 constexpr int kSolveBatch = 4;  // rows whose loads are in flight ahead of the solve
 
@@ -977,7 +974,13 @@ __device__ void mgs_cluster(double* __restrict__ Y, int n, int k, int c0, int c1
   }
 }
 
-__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(3))) k_invit(const MolDesc* __restrict__ mols,
+// F32: the launch may hold FP32 steps of the mixed_eig32 policy (their solve
+// with FP32 factors is compiled in); ALL: the full decomposition (two solves).
+// The SCF's FP64 launches take the variant with one FP64 solve only, which
+// needs the fewest registers.
+template <bool F32, bool ALL>
+__global__ void __launch_bounds__(kT)
+__attribute__((amdgpu_waves_per_eu(F32 || ALL ? 3 : 4))) k_invit(const MolDesc* __restrict__ mols,
                                                const double* __restrict__ dA,
                                                const double* __restrict__ eA,
                                                const double* __restrict__ lamA,
@@ -997,8 +1000,8 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(3))) k_
   selection(M, want_all, lo, k);
   if (k == 0) return;
   // FP32 steps: one solve (their T and eigenvalues carry FP32-level error)
-  const bool fp32 = prec && prec[m];
-  const int sweeps = invit_sweeps(fp32, want_all);
+  const bool fp32 = F32 && prec && prec[m];
+  const int sweeps = ALL ? 2 : 1;
   double* d = lds;
   double* e = d + n;
   double* sh = e + n;                     // shifts, k
@@ -1019,13 +1022,14 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(3))) k_
   __syncthreads();
   for (int sweep = 0; sweep < sweeps; ++sweep) {
     for (int j = tid; j < k; j += nt) {
-      const double nrm =
-          fp32
-              ? tri_solve<true, float>(d, e, n, sh[j], tiny, Y, k, j, (float*)fac + (j % L), L,
-                                       lo + j)
-          : sweep == 0
-              ? tri_solve<true, double>(d, e, n, sh[j], tiny, Y, k, j, fac + (j % L), L, lo + j)
-              : tri_solve<false, double>(d, e, n, sh[j], tiny, Y, k, j, fac + (j % L), L, lo + j);
+      double nrm;
+      if (F32 && fp32)
+        nrm = tri_solve<true, float>(d, e, n, sh[j], tiny, Y, k, j, (float*)fac + (j % L), L,
+                                     lo + j);
+      else if (!ALL || sweep == 0)
+        nrm = tri_solve<true, double>(d, e, n, sh[j], tiny, Y, k, j, fac + (j % L), L, lo + j);
+      else
+        nrm = tri_solve<false, double>(d, e, n, sh[j], tiny, Y, k, j, fac + (j % L), L, lo + j);
       const bool single = cstart[j] == j && (j + 1 == k || cstart[j + 1] != j);
       if (single && sweep == sweeps - 1) scale_column(Y, n, k, j, 1.0 / nrm);
     }
@@ -1836,9 +1840,11 @@ void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all) {
     // one lane per vector: only as many waves as the widest selection needs
     const int kmax = all ? N : N / 2;
     const int threads = min(kT, max(64, (kmax + 63) / 64 * 64));
-    k_invit<<<p.count, threads, inv_lds, s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm, b.W, all ? 1 : 0,
-                                              act, b.status, p.order,
-                                              (all || !b.eig32) ? nullptr : b.prec);
+    const int* prec = (all || !b.eig32) ? nullptr : b.prec;
+    const auto kern = all ? k_invit<false, true> : prec ? k_invit<true, false>
+                                                        : k_invit<false, false>;
+    kern<<<p.count, threads, inv_lds, s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm, b.W, all ? 1 : 0,
+                                           act, b.status, p.order, prec);
   }
   for (int i = 0; i < b.n_large; ++i) {
     const int m = b.large_idx[i];
