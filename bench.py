@@ -423,7 +423,9 @@ def tridiag_roofline(stage, sizes, steps, s32, K):
     tri_ms, tri_cnt = stage["tridiag"]
     head_ms, head_cnt = stage["tridiag_head"]
     t32_ms, t32_cnt = stage["tridiag_fp32"]
-    passes = max(1, tri_cnt)
+    # passes with work: one per SCF iteration of the longest molecule (the
+    # loop's last launch, after every molecule stopped, finds none active)
+    passes = max(1, int(steps.max()) * K) if tri_cnt else 1
     N3 = sizes.astype(np.float64) ** 3
     R3 = np.minimum(sizes, REG_ROWS).astype(np.float64) ** 3
     steps64 = steps - s32 if t32_cnt else steps
@@ -633,8 +635,11 @@ def main():
     bf64 = 8.0 * I + 3 * 8.0 * T
     bf32 = 4.0 * I + 3 * 8.0 * T
     fock_bytes_step = float(np.sum((steps + 1 - s32) * bf64 + s32 * bf32))
-    fock_avg_ms = fock_ms / max(1, fock_cnt)
-    fock_gbs = fock_bytes_step * K / max(1, fock_cnt) / (fock_avg_ms * 1e-3) / 1e9
+    # builds with work: the guess Fock build and one per SCF iteration of the
+    # longest molecule (each build is the FP64 and the FP32 launch)
+    fock_builds = (int(steps.max()) + 1) * K
+    fock_avg_ms = fock_ms / max(1, fock_builds)
+    fock_gbs = fock_bytes_step * K / max(1, fock_builds) / (fock_avg_ms * 1e-3) / 1e9
     fock_roof = {"kernel": "k_fock_mol<double> / k_fock_mol<float> (one workgroup per molecule; "
                            "FP64 and FP32 builds in launches of their own)",
                  "bound": "hbm", "achieved": round(fock_gbs, 2),
@@ -644,15 +649,16 @@ def main():
                  "work": "8 I + 24 T bytes per molecule in FP64 builds, 4 I + 24 T in FP32 ones"}
     den_ms, den_cnt = stage["density"]
     fpd = sizes * (sizes + 1.0) * occ
-    den_avg_ms = den_ms / max(1, den_cnt)
+    den_launches = int(steps.max()) * K  # launches with work
+    den_avg_ms = den_ms / max(1, den_launches)
     den_roof = {"kernel": "k_density", "bound": "mfma",
                 "avg_launch_ms": round(den_avg_ms, 4),
                 "work": "N(N+1) n_occ flops per molecule (P = 2 C_occ C_occ^T, lower triangle); "
                         "peak of each molecule-step's MFMA type weighted by its flops"}
     # per-launch flops split by instruction type: FP32 steps on the f32 MFMA,
     # FP64 steps on the f64 one (split BF16: 3 bf16 products per flop)
-    f64_flops = float(np.sum((steps - s32) * fpd)) * K / max(1, den_cnt)
-    red_flops = float(np.sum(s32 * fpd)) * K / max(1, den_cnt)
+    f64_flops = float(np.sum((steps - s32) * fpd)) * K / max(1, den_launches)
+    red_flops = float(np.sum(s32 * fpd)) * K / max(1, den_launches)
     red_peak, kf = ((BF16_PEAK_TFLOPS, 3.0) if args.policy == "mixed_bf16"
                     else (FP32_PEAK_TFLOPS, 1.0))
     busy = (f64_flops / (FP64_PEAK_TFLOPS * 1e12) + kf * red_flops / (red_peak * 1e12))
