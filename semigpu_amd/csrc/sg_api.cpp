@@ -551,7 +551,8 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   const size_t o_H = take(sizeof(double) * mat0), o_F = take(sizeof(double) * mat0),
                o_P = take(sizeof(double) * mat0), o_C = take(sizeof(double) * mat0),
                o_W = take(sizeof(double) * 5 * mat0);
-  const size_t o_eig = take(sizeof(double) * vec0), o_tau = take(sizeof(double) * vec0),
+  const size_t o_eig = take(sizeof(double) * vec0), o_dlam = take(sizeof(double) * vec0),
+               o_tau = take(sizeof(double) * vec0),
                o_dT = take(sizeof(double) * vec0), o_eT = take(sizeof(double) * vec0),
                o_tn = take(sizeof(double) * n_mol);
   const size_t o_ecc = take(sizeof(double) * n_mol), o_eel = take(sizeof(double) * n_mol),
@@ -586,6 +587,7 @@ static int batch_create(sg_context* ctx, int n_mol, const int* atom_offsets, con
   d.C = (double*)(base + o_C);
   d.W = (double*)(base + o_W);
   d.eig = (double*)(base + o_eig);
+  d.dlam = (double*)(base + o_dlam);
   d.tau = (double*)(base + o_tau);
   d.dT = (double*)(base + o_dT);
   d.eT = (double*)(base + o_eT);

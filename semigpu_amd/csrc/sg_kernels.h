@@ -73,6 +73,9 @@ struct BatchDev {
   double* C = nullptr;       // eigenvectors
   double* W = nullptr;       // eigensolver workspace (5 * n_mat)
   double* eig = nullptr;     // eigenvalues (n_vec; the selected ones first)
+  // how far each selected eigenvalue moved at its last bisection (n_vec; or
+  // null): the next SCF step brackets it by that change
+  double* dlam = nullptr;
   double* tau = nullptr;     // Householder scalars (n_vec)
   double* dT = nullptr;      // tridiagonal diagonal (n_vec)
   double* eT = nullptr;      // tridiagonal off-diagonal (n_vec)

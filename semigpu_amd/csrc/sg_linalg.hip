@@ -674,7 +674,8 @@ __global__ void __launch_bounds__(256) k_bisect(const MolDesc* __restrict__ mols
                                                  const int* __restrict__ status,
                                                  const int* __restrict__ order,
                                                  const int* __restrict__ steps,
-                                                 const int* __restrict__ prec) {
+                                                 const int* __restrict__ prec,
+                                                 double* __restrict__ dlamA) {
   extern __shared__ double lds[];
   const int m = order ? order[blockIdx.y] : blockIdx.y;
   if (!mol_live(active, status, m)) return;
@@ -731,21 +732,30 @@ __global__ void __launch_bounds__(256) k_bisect(const MolDesc* __restrict__ mols
   const int j0 = blockIdx.x * blockDim.x + threadIdx.x;
   if (j0 >= half) return;
   const int jj[2] = {j0, j0 + half};
-  double lo_[2], hi_[2];
+  double lo_[2], hi_[2], prev_[2] = {0.0, 0.0};
+  const bool warm = steps && steps[m] > 0;
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
     const int j = jj[u];
     const int idx = lo + j;
     // From the second SCF step on, the previous step's eigenvalue (same
     // index) brackets the new one closely: [prev - delta, prev + delta] is used
-    // when two Sturm counts confirm it, with delta = 1e-7 ||T||, else
-    // 1e-4 ||T||, else the Gershgorin interval.
+    // when two Sturm counts confirm it.  From the third step on, delta is
+    // first 8 times the eigenvalue's change at the previous step (at least
+    // 1e-12 ||T||: the SCF's changes shrink from step to step), then
+    // 1e-7 ||T||, then 1e-4 ||T||, else the Gershgorin interval.
     double a = glo, b = ghi;
-    if (j < k && steps && steps[m] > 0) {
+    if (j < k && warm) {
       const double prev = lamA[M.vec0 + j];
+      prev_[u] = prev;
+      const bool have = dlamA && steps[m] >= 2;
+      const double dprev = have ? fmax(8.0 * dlamA[M.vec0 + j], 1e-12 * tnorm) : 0.0;
 #pragma unroll 1
-      for (int lvl = 0; lvl < 2; ++lvl) {
-        const double del = (lvl == 0 ? 1e-7 : 1e-4) * tnorm;
+      for (int lvl = 0; lvl < 3; ++lvl) {
+        if (lvl == 0 && !have) continue;
+        const double del = lvl == 0 ? dprev : (lvl == 1 ? 1e-7 : 1e-4) * tnorm;
+        // a bracket no wider than one that failed fails too
+        if (lvl > 0 && del <= dprev) continue;
         const double x1 = prev - del, x2 = prev + del;
         if (!(x1 > glo && x2 < ghi)) continue;
         int c1, c2;
@@ -786,7 +796,11 @@ __global__ void __launch_bounds__(256) k_bisect(const MolDesc* __restrict__ mols
   }
 #pragma unroll
   for (int u = 0; u < 2; ++u)
-    if (jj[u] < k) lamA[M.vec0 + jj[u]] = 0.5 * (lo_[u] + hi_[u]);
+    if (jj[u] < k) {
+      const double lam = 0.5 * (lo_[u] + hi_[u]);
+      lamA[M.vec0 + jj[u]] = lam;
+      if (dlamA) dlamA[M.vec0 + jj[u]] = warm ? fabs(lam - prev_[u]) : 1e300;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1819,7 +1833,8 @@ static void bisect_part(const BatchDev& b, hipStream_t s, bool all, const int* o
   k_bisect<<<gb, threads, sizeof(double) * (2 * N + 32), s>>>(b.mols, b.dT, b.eT, b.eig, b.tnorm,
                                                           all ? 1 : 0, all ? nullptr : b.active,
                                                           b.status, order,
-                                                          all ? nullptr : b.steps, prec);
+                                                          all ? nullptr : b.steps, prec,
+                                                          all ? nullptr : b.dlam);
 }
 
 void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all) {

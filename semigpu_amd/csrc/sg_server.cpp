@@ -567,8 +567,8 @@ int server_create(sg_context* ctx, double hbm_bytes, int max_slots, int max_atom
   const double fixed = (double)n_classes * (double)lane_bytes(ls, la, pc);
   // shared pools, per N^2 element of matrix space: 9 doubles (H, F, P, C, 5 for
   // the eigensolver), 2.5 pair-block elements (8 + 4 bytes), 1/8 vector element
-  // (4 vectors of doubles)
-  const double per = 72.0 + 2.5 * 12.0 + 4.0;
+  // (5 vectors of doubles)
+  const double per = 72.0 + 2.5 * 12.0 + 5.0;
   if (hbm_bytes <= fixed + per * 1024) {
     delete S;
     return fail(SG_ERR_INVALID, "sg_server_create: HBM budget too small");
@@ -586,7 +586,8 @@ int server_create(sg_context* ctx, double hbm_bytes, int max_slots, int max_atom
   const size_t o_blk = take(8 * bc), o_blk32 = take(4 * bc);
   const size_t o_H = take(8 * mc), o_F = take(8 * mc), o_P = take(8 * mc), o_C = take(8 * mc),
                o_W = take(8 * 5 * mc);
-  const size_t o_eig = take(8 * vc), o_tau = take(8 * vc), o_dT = take(8 * vc), o_eT = take(8 * vc);
+  const size_t o_eig = take(8 * vc), o_dlam = take(8 * vc), o_tau = take(8 * vc),
+               o_dT = take(8 * vc), o_eT = take(8 * vc);
   hipError_t e = hipMalloc(&S->pool, off);
   if (e != hipSuccess) {
     S->pool = nullptr;
@@ -606,6 +607,7 @@ int server_create(sg_context* ctx, double hbm_bytes, int max_slots, int max_atom
   d.C = (double*)(base + o_C);
   d.W = (double*)(base + o_W);
   d.eig = (double*)(base + o_eig);
+  d.dlam = (double*)(base + o_dlam);
   d.tau = (double*)(base + o_tau);
   d.dT = (double*)(base + o_dT);
   d.eT = (double*)(base + o_eT);
