@@ -1149,8 +1149,10 @@ template <bool WIDE>
 struct BackShape {
   static constexpr int kStage = WIDE ? 8 : 4, kSteps = WIDE ? 8 : 4, kTiles = WIDE ? 4 : 1;
 };
+// V (G rows of n) and W2 (G rows of k) are stored with odd row strides, so
+// that the 16 lanes reading one column of 16 rows hit 16 different LDS banks.
 __host__ __device__ inline int bt_need(int g, int n, int k) {
-  return g * n + (g * k > 2048 ? g * k : 2048) + 512;
+  return g * (n | 1) + (g * (k | 1) > 2048 ? g * (k | 1) : 2048) + 512;
 }
 __host__ __device__ inline int bt_group(int n, int k, int budget) {
   int g = 16;
@@ -1175,9 +1177,10 @@ __device__ __forceinline__ void back_mol(const MolDesc& M, const R* __restrict__
   const int lr = lane & 15, lk = lane >> 4;
   const int nref = n > 2 ? n - 2 : 0;
   const int G = bt_group(n, k, kBackBudget * (int)(8 / sizeof(R)));
-  const int rg = max(G * k, 2048);
+  const int ns = n | 1, ks = k | 1;  // row strides of V and W2 (odd: no bank conflicts)
+  const int rg = max(G * ks, 2048);
   R* Vs = lds;          // G x n
-  R* Rw = Vs + G * n;   // W2 (G x k), before that nwave partial 16x16 Gram tiles
+  R* Rw = Vs + G * ns;  // W2 (G x k), before that nwave partial 16x16 Gram tiles
   R* Tm = Rw + rg;      // 16 x 16, T[r][q] at r*16+q
   R* Gm = Tm + 256;     // 16 x 16 Gram
   const int KT = (k + 15) >> 4;
@@ -1200,7 +1203,7 @@ __device__ __forceinline__ void back_mol(const MolDesc& M, const R* __restrict__
       for (int u = 0; u < kStage; ++u) {
         const int t = t0 + u * nt;
         const int q = t / n, i = t - q * n;
-        if (t < G * n) Vs[t] = (q < b && i > k0 + q) ? v[u] : (R)0;
+        if (t < G * n) Vs[q * ns + i] = (q < b && i > k0 + q) ? v[u] : (R)0;
       }
     }
     for (int t = tid; t < 256; t += nt) Tm[t] = (R)0;
@@ -1210,7 +1213,7 @@ __device__ __forceinline__ void back_mol(const MolDesc& M, const R* __restrict__
       v4 acc = {0, 0, 0, 0};
       for (int i0 = k0 + 1 + 4 * wave; i0 < n; i0 += 4 * nwave) {
         const int i = i0 + lk;
-        const R a = (lr < G && i < n) ? Vs[lr * n + i] : (R)0;
+        const R a = (lr < G && i < n) ? Vs[lr * ns + i] : (R)0;
         acc = Mf::op(a, a, acc);
       }
 #pragma unroll
@@ -1243,7 +1246,7 @@ __device__ __forceinline__ void back_mol(const MolDesc& M, const R* __restrict__
       const int j = 16 * jt + lr;
       const bool jok = j < k, gok = lr < G;
       const R* yc = Y + (jok ? j : 0);
-      const R* vr = Vs + (gok ? lr : 0) * n;
+      const R* vr = Vs + (gok ? lr : 0) * ns;
       v4 w1 = {0, 0, 0, 0};
       // kSteps k-steps per round: their loads in flight together, then the
       // instructions in order
@@ -1266,7 +1269,7 @@ __device__ __forceinline__ void back_mol(const MolDesc& M, const R* __restrict__
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
         const int g = Mf::row(lane, rr);
-        if (g < G && j < k) Rw[g * k + j] = w2[rr];
+        if (g < G && j < k) Rw[g * ks + j] = w2[rr];
       }
     }
     __syncthreads();
@@ -1299,8 +1302,8 @@ __device__ __forceinline__ void back_mol(const MolDesc& M, const R* __restrict__
           for (int st = 0; st < 4; ++st) {
             const int g = 4 * st + lk;
             const int i = I0 + lr, j = J0 + lr;
-            const R a = (g < G && i < n) ? -Vs[g * n + i] : (R)0;
-            const R w = (g < G && j < k) ? Rw[g * k + j] : (R)0;
+            const R a = (g < G && i < n) ? -Vs[g * ns + i] : (R)0;
+            const R w = (g < G && j < k) ? Rw[g * ks + j] : (R)0;
             c[u] = Mf::op(a, w, c[u]);
           }
           const int col = J0 + lr;
