@@ -63,6 +63,8 @@ constexpr double kEps = 2.220446049250313e-16;
 // settle a vector whose gap to the next eigenvalue is above 1e-5 ||T|| (the
 // closer ones are orthogonalised as a cluster).
 constexpr double kFp32BisectTol = 1e-10;
+// The same for the reduced-precision steps of a policy with an FP64 T.
+constexpr double kReducedBisectTol = 1e-12;
 // Inverse-iteration solves per vector.  With eigenvalues bisected to a few
 // ulps of ||T||, one solve from a random start already leaves the wanted
 // direction dominant: another eigenvalue at distance g keeps a share of about
@@ -675,7 +677,7 @@ __global__ void __launch_bounds__(256) k_bisect(const MolDesc* __restrict__ mols
                                                  const int* __restrict__ order,
                                                  const int* __restrict__ steps,
                                                  const int* __restrict__ prec,
-                                                 double* __restrict__ dlamA) {
+                                                 double* __restrict__ dlamA, double red_tol) {
   extern __shared__ double lds[];
   const int m = order ? order[blockIdx.y] : blockIdx.y;
   if (!mol_live(active, status, m)) return;
@@ -773,9 +775,9 @@ __global__ void __launch_bounds__(256) k_bisect(const MolDesc* __restrict__ mols
   // bisection keeping count(a) <= idx < count(b), to a few ulps of the
   // eigenvalue or 2 ulps of ||T||, whichever is wider (the default absolute
   // tolerance of LAPACK dstebz: the Sturm recurrence's own rounding is of that
-  // order); in an FP32 step to kFp32BisectTol ||T|| (T itself carries FP32
-  // rounding there)
-  const double atol = pivmin + (prec && prec[m] ? kFp32BisectTol : 2.0 * kEps) * tnorm;
+  // order); in a reduced-precision step (prec) to red_tol ||T|| (its density
+  // carries FP32 rounding)
+  const double atol = pivmin + (prec && prec[m] ? red_tol : 2.0 * kEps) * tnorm;
   bool done[2];
 #pragma unroll
   for (int u = 0; u < 2; ++u)
@@ -1828,7 +1830,7 @@ void launch_eigh_tridiag(const BatchDev& b, hipStream_t s, bool all, int which) 
 }
 
 static void bisect_part(const BatchDev& b, hipStream_t s, bool all, const int* order, int count,
-                        int N, const int* prec) {
+                        int N, const int* prec, double red_tol) {
   // two eigenvalues per lane: (k+1)/2 lanes, k <= N/2 on the SCF path
   const int half = std::max(1, ((all ? N : N / 2) + 1) / 2);  // N = 1: no wanted value
   const int threads = min(256, max(64, (half + 63) / 64 * 64));
@@ -1837,16 +1839,22 @@ static void bisect_part(const BatchDev& b, hipStream_t s, bool all, const int* o
                                                           all ? 1 : 0, all ? nullptr : b.active,
                                                           b.status, order,
                                                           all ? nullptr : b.steps, prec,
-                                                          all ? nullptr : b.dlam);
+                                                          all ? nullptr : b.dlam, red_tol);
 }
 
 void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all) {
   const Part p = small_part(b);
-  // FP32 steps of the one-workgroup path bisect to the accuracy of their
-  // FP32-reduced T; the multi-workgroup path reduces in FP64 throughout
-  const int* prec = (all || !b.eig32) ? nullptr : b.prec;
-  if (p.count > 0) bisect_part(b, s, all, p.order, p.count, p.max_n, prec);
-  if (b.n_large > 0) bisect_part(b, s, all, b.order, b.n_large, b.max_n, nullptr);
+  // Reduced-precision steps of the one-workgroup path bisect less far: their
+  // density carries FP32 rounding, which eigenvalues a few ulps of ||T||
+  // exact would not improve.  Under mixed_eig32 T itself is FP32
+  // (kFp32BisectTol); under the other policies, with T in FP64, to
+  // kReducedBisectTol, where one inverse-iteration solve still leaves the
+  // vectors well within the FP32 step's rounding.  The multi-workgroup path
+  // bisects to ulps throughout.
+  const int* prec = (all || !b.mixed) ? nullptr : b.prec;
+  const double red_tol = b.eig32 ? kFp32BisectTol : kReducedBisectTol;
+  if (p.count > 0) bisect_part(b, s, all, p.order, p.count, p.max_n, prec, red_tol);
+  if (b.n_large > 0) bisect_part(b, s, all, b.order, b.n_large, b.max_n, nullptr, red_tol);
 }
 
 void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all) {
