@@ -211,8 +211,9 @@ def test_above_one_workgroup_limit(engine):
 def test_water_box_1000(engine):
     """The 1000-water box (3000 atoms, N = 6000) on the multi-workgroup path:
     FP64 SCF to 1e-6 eV, the converged density idempotent with the right
-    trace (checked through products with random vectors), its Fock matrix
-    against the oracle's from the same integrals and density, E_el against the
+    trace (checked through products with random vectors), the pair blocks bit
+    for bit against the oracle's, its Fock matrix against the oracle's from
+    the same integrals and density, E_el against the
     exactly rounded sum of its matrices, and the mixed policy within 1e-4 eV
     of FP64."""
     import pyoracle as po
@@ -238,6 +239,8 @@ def test_water_box_1000(engine):
     say("case water box oracle ingredients")
     om = po.Molecule(mol)
     blocks = b.pair_blocks(0)
+    # the 4.5 M atom pairs' integrals bit for bit against the oracle's
+    assert np.array_equal(blocks, om.pair_blocks())
     f_or = om.fock(blocks, H, P)
     assert np.abs(F - f_or).max() <= 1e-10 * np.abs(f_or).max()
     # E_el = 1/2 sum P (H + F) of the device's own matrices, summed exactly
