@@ -155,6 +155,75 @@ def test_eigensolver_properties(engine):
     b.close()
 
 
+def _fused_range_molecules():
+    """Molecules above the on-chip reduction's block (256 rows), whose first
+    N - 256 columns k_tridiag reduces: N = 257 (an OH group beside 42
+    waters), 258, 260, 300 and 384, and N = 386, which puts the batch's head
+    launch on the form for N > 384."""
+    oh = geometry.water_cluster(42, geometry.SplitMix64(3))
+    oh = oh + [(8, 40.0, 40.0, 40.0), (1, 40.0, 40.0, 40.97)]
+    return [oh, geometry.water_cluster(43, geometry.SplitMix64(4)), geometry.alkane(43),
+            geometry.water_cluster(50, geometry.SplitMix64(6)),
+            geometry.water_cluster(64, geometry.SplitMix64(8)), geometry.alkane(64)]
+
+
+def _check_eigenpairs(b, i, n):
+    import pyoracle as po
+    w, C = b.eigen(i)
+    F = po.packed_to_full(b.packed(i, api.SG_FOCK), n)
+    wr = np.linalg.eigh(F)[0]
+    scale = np.abs(wr).max()
+    ew = np.abs(w - wr).max() / scale
+    eo = np.abs(C.T @ C - np.eye(n)).max()
+    er = np.abs(F @ C - C * w).max() / scale
+    say("N", n, "eig", ew, "orth", eo, "resid", er)
+    assert ew <= 1e-12 and eo <= 1e-12 and er <= 1e-11
+    return w, C
+
+
+def test_eigensolver_head_range(engine):
+    """Eigenpairs against numpy across N = 257 .. 386 (1 to 130 columns
+    before the on-chip block), each molecule alone and all in one batch (bit
+    for bit the same, although the batch's head launch takes the form for
+    N > 384)."""
+    mols = _fused_range_molecules()
+    ns = [geometry.n_basis(m) for m in mols]
+    say("case eigh head range", ns)
+    assert ns == [257, 258, 260, 300, 384, 386]
+    b = engine.batch(mols)
+    b.two_center()
+    b.hcore()
+    b.guess_density()
+    b.fock()
+    b.eigh()
+    for i, (m, n) in enumerate(zip(mols, ns)):
+        w, C = _check_eigenpairs(b, i, n)
+        a = engine.batch([m])
+        a.two_center()
+        a.hcore()
+        a.guess_density()
+        a.fock()
+        a.eigh()
+        wa, Ca = a.eigen(0)
+        assert np.array_equal(w, wa) and np.array_equal(C, Ca)
+        a.close()
+    b.close()
+
+
+def test_scf_head_range_against_oracle(engine):
+    """FP64 SCF of (H2O)45 (N = 270) and C43H88 (N = 260) against the oracle's
+    SCF: step counts and energies."""
+    import pyoracle as po
+    mols = [geometry.water_cluster(45, geometry.SplitMix64(21)), geometry.alkane(43)]
+    say("case scf head range", [geometry.n_basis(m) for m in mols])
+    res = engine.batch(mols).scf(threshold=1e-6)
+    for mol, r in zip(mols, res):
+        o = po.Molecule(mol).scf(threshold=1e-6, mode="nddo")
+        say("N", geometry.n_basis(mol), r["steps"], o["steps"], r["e_tot"] - o["e_tot"])
+        assert r["converged"] and r["steps"] == o["steps"]
+        assert abs(r["e_tot"] - o["e_tot"]) <= 1e-6
+
+
 def test_water_clusters_against_oracle(engine):
     import pyoracle as po
     mols = geometry.water_batch(6, seed=5, n_min=2, n_max=10)
