@@ -197,15 +197,21 @@ __host__ __device__ inline void selection(const MolDesc& M, int want_all, int& l
 //   0       A: the reduced matrix, full symmetric; reflector c in the upper part of row c
 //   N^2     Y: selected eigenvectors, N x k row-major (as the one-workgroup solver)
 //   2 N^2   inverse-iteration factors, 3 N x L
-//   4 N^2   V and W panels (kLargeNb x N each), the product vector (N), the
+//   4 N^2   V and W panels (kLargeNb x N each), a spare vector (N), the
 //           compact-WY T factors (16 x 16 per group of 16 reflectors), the
 //           inverse-iteration shifts and cluster tables; during the
 //           tridiagonalisation the T, shift and cluster rooms hold the column
-//           step's partial sums and the current column
+//           step's partial sums and the current column;
+//   4 N^2 + kLargePyOff N: the product's partial rows, one vector per column
+//           chunk (kLargeChunks x N), then their v.py partial dots
+//           ((N / 32 + 1) x kLargeChunks)
+constexpr int kLargeChunkCols = 1024;  // columns per workgroup of the product
+constexpr int kLargeChunks = (kMaxLarge + kLargeChunkCols - 1) / kLargeChunkCols;
+constexpr int kLargePyOff = 100;
 struct LargeWork {
   double *A, *Y, *fac, *V, *W, *py, *T, *sh;
   int *cstart, *clist;  // clist[k + 1] holds the number of clusters
-  double *part, *part2, *xv;
+  double *part, *part2, *xv, *pyp, *partvp;
   __host__ __device__ LargeWork(double* work, const MolDesc& M) {
     const int64_t n = M.n_ao, nn = n * n;
     A = work + 5 * M.mat0;
@@ -221,6 +227,8 @@ struct LargeWork {
     part = T;                              // <= (N/32 + 1) x (2 kLargeNb + 1)
     part2 = sh;                            // <= N/256 + 1
     xv = (double*)cstart;                  // N (cstart and clist: N + 1 doubles)
+    pyp = V + kLargePyOff * n;             // after clist (n >= 128 here)
+    partvp = pyp + kLargeChunks * n;
   }
   // vectors whose factors are in flight at once (3 N x L doubles of factors)
   static __host__ __device__ int lanes(int n, int k) {

@@ -7,10 +7,12 @@
 // the whole GPU for one molecule:
 //  * the work matrix is kept in full symmetric storage, so the product with
 //    the trailing block is a row-wise dot that workgroups split by rows;
-//  * a column is two grid launches: the product (which also forms the
-//    reflector from the column and leaves per-workgroup partial dots), then
-//    the completion of W's column with the next column's corrections (one row
-//    per thread), whose workgroups each add the partial dots in order;
+//  * a column is two grid launches: the product (workgroups over blocks of
+//    32 rows and chunks of kLargeChunkCols columns, so that the grid fills the
+//    GPU down to a few thousand rows; it also forms the reflector from the
+//    column and leaves per-workgroup partial rows and dots), then the
+//    completion of W's column with the next column's corrections (one row per
+//    thread), whose workgroups each add the partial rows and dots in order;
 //  * the trailing update of each panel is a grid of 16 x 16 matrix-core tiles;
 //  * the back-transformation gives each workgroup a 16-column slice of Y and
 //    walks the reflector groups with their T factors precomputed.
@@ -83,16 +85,19 @@ __global__ void __launch_bounds__(kColRows) k_lt_x0(int n, const double* __restr
 
 // Reflector of column c (j-th of its panel) from x and the partial sums of
 // x^2, then the product py = A v over rows i > c (4 rows per wave, v formed
-// on the fly from x) and, per workgroup of 32 rows, the partial dots the next
-// step needs: v.py, W_q.v and V_q.v (q < j).  Writes v into V_j and into the
-// upper part of row c of A; d, e, tau of column c.
+// on the fly from x), workgroup (x, y) over rows c + 1 + 32 x .. and columns
+// c + 1 + kLargeChunkCols y ..: its rows' partial sums into pyp[y] and their
+// dot with v into partvp; the workgroups of chunk 0 also leave the partial
+// dots W_q.v and V_q.v (q < j) of their rows and write v into V_j and into
+// the upper part of row c of A; d, e, tau of column c.
 __global__ void __launch_bounds__(64 * kProdWaves) k_lt_prod(int n, double* __restrict__ A,
                                                               double* __restrict__ V,
                                                               const double* __restrict__ W,
                                                               const double* __restrict__ xv,
                                                               const double* __restrict__ part2,
                                                               double* __restrict__ part,
-                                                              double* __restrict__ py,
+                                                              double* __restrict__ pyp,
+                                                              double* __restrict__ partvp,
                                                               double* __restrict__ d,
                                                               double* __restrict__ e,
                                                               double* __restrict__ tau, int c, int j,
@@ -117,7 +122,8 @@ __global__ void __launch_bounds__(64 * kProdWaves) k_lt_prod(int n, double* __re
     tk = (beta - alpha) / beta;
     sc = 1.0 / (alpha - beta);
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  const int cb = blockIdx.y;  // column chunk
+  if (blockIdx.x == 0 && cb == 0 && threadIdx.x == 0) {
     d[c] = xv[c];
     e[c] = beta;
     tau[c] = tk;
@@ -130,12 +136,13 @@ __global__ void __launch_bounds__(64 * kProdWaves) k_lt_prod(int n, double* __re
   const double* ar[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) ar[r] = A + (int64_t)min(i0 + r, n - 1) * n;
+  const int lbeg = c + 1 + cb * kLargeChunkCols, lend = min(n, lbeg + kLargeChunkCols);
   if (i0 < n) {
     // two 64-column tiles per batch of loads; the second is clamped to the
-    // row's last column and weighted 0 past it
-    for (int l = c + 1 + lane; l < n; l += 128) {
-      const int l2 = l + 64 < n ? l + 64 : n - 1;
-      const double v0 = vof(l), v1 = l + 64 < n ? vof(l2) : 0.0;
+    // chunk's last column and weighted 0 past it
+    for (int l = lbeg + lane; l < lend; l += 128) {
+      const int l2 = l + 64 < lend ? l + 64 : lend - 1;
+      const double v0 = vof(l), v1 = l + 64 < lend ? vof(l2) : 0.0;
       double a0[4], a1[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -165,22 +172,23 @@ __global__ void __launch_bounds__(64 * kProdWaves) k_lt_prod(int n, double* __re
   if ((lane & 15) == 0) {
     const int i = i0 + rho;
     rows_py[wave * 4 + rho] = i < n ? v1 : 0.0;
-    if (i < n) py[i] = v1;
+    if (i < n) pyp[(int64_t)cb * n + i] = v1;
   }
   if (threadIdx.x < R) {
     const int i = r0 + threadIdx.x;
     const double v = i < n ? vof(i) : 0.0;
     rows_v[threadIdx.x] = v;
-    if (i < n) {
+    if (i < n && cb == 0) {
       V[(int64_t)j * n + i] = v;
       A[(int64_t)c * n + i] = v;
     }
   }
   __syncthreads();
-  // partial dots over this workgroup's rows: [0, j) W_q.v, [j, 2j) V_q.v, 2j v.py
-  const int nd = 2 * j + 1;
+  // partial dots over this workgroup's rows: v.py (of this chunk's columns)
+  // and, in chunk 0, [0, j) W_q.v and [j, 2j) V_q.v
+  const int nd = cb == 0 ? 2 * j + 1 : 1;
   if (threadIdx.x < nd) {
-    const int t = threadIdx.x;
+    const int t = cb == 0 ? threadIdx.x : 2 * j;
     const double* src = t < j ? W + (int64_t)t * n : V + (int64_t)(t - j) * n;
     double sacc = 0.0;
     for (int b0 = 0; b0 < R; b0 += 16) {
@@ -193,7 +201,8 @@ __global__ void __launch_bounds__(64 * kProdWaves) k_lt_prod(int n, double* __re
 #pragma unroll
       for (int r = 0; r < 16; ++r) sacc += x[r] * rows_v[b0 + r];  // rows_v is 0 past the matrix
     }
-    part[(int64_t)blockIdx.x * kPartStride + t] = sacc;
+    if (t == 2 * j) partvp[(int64_t)blockIdx.x * gridDim.y + cb] = sacc;
+    else part[(int64_t)blockIdx.x * kPartStride + t] = sacc;
   }
 }
 
@@ -206,8 +215,9 @@ __global__ void __launch_bounds__(64 * kProdWaves) k_lt_prod(int n, double* __re
 __global__ void __launch_bounds__(kColRows) k_lt_cols(int n, const double* __restrict__ A,
                                                        const double* __restrict__ V,
                                                        double* __restrict__ W,
-                                                       const double* __restrict__ py,
+                                                       const double* __restrict__ pyp,
                                                        const double* __restrict__ part,
+                                                       const double* __restrict__ partvp,
                                                        const double* __restrict__ tau,
                                                        double* __restrict__ xv,
                                                        double* __restrict__ part2, int c, int j,
@@ -216,13 +226,28 @@ __global__ void __launch_bounds__(kColRows) k_lt_cols(int n, const double* __res
   if (!mol_live(active, status, m)) return;
   __shared__ double tv[2 * kLargeNb + 1];
   __shared__ double red[kColRows / 64];
+  const int nch = (n - c - 1 + kLargeChunkCols - 1) / kLargeChunkCols;  // the product's chunks
   {
     // the product's partial dots, summed in workgroup order: thread T takes
-    // dot T % 64 over the workgroups g = T / 64, T / 64 + 4, ...; the four
-    // slices are then added in order (every workgroup forms the same sums)
+    // dot T % 64 (< 2 j) over the row blocks g = T / 64, T / 64 + 4, ...; the
+    // four slices are then added in order.  v.py has an entry per (row
+    // block, chunk): every thread sums a strided share, then the workgroup
+    // adds the shares in a fixed order.  (Every workgroup forms the same sums.)
     __shared__ double sl[kColRows / 64][64];
     const int t = threadIdx.x & 63, k = threadIdx.x >> 6;
-    const int G = (n - c - 1 + 4 * kProdWaves - 1) / (4 * kProdWaves), nd = 2 * j + 1;
+    const int G = (n - c - 1 + 4 * kProdWaves - 1) / (4 * kProdWaves), nd = 2 * j;
+    const int GE = G * nch;  // entries of v.py
+    double vacc = 0.0;
+    for (int g0 = threadIdx.x; g0 < GE; g0 += 4 * kColRows) {
+      double x[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int g = g0 + u * kColRows;
+        x[u] = g < GE ? partvp[g] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) vacc += x[u];
+    }
     double sacc = 0.0;
     if (t < nd) {
       for (int g0 = k; g0 < G; g0 += 4 * (kColRows / 64)) {
@@ -237,13 +262,14 @@ __global__ void __launch_bounds__(kColRows) k_lt_cols(int n, const double* __res
       }
     }
     sl[k][t] = sacc;
-    __syncthreads();
+    const double vpy = block_total(vacc, red);  // its barriers also publish sl
     if (threadIdx.x < nd) {
       double v = 0.0;
 #pragma unroll
       for (int q = 0; q < kColRows / 64; ++q) v += sl[q][threadIdx.x];
       tv[threadIdx.x] = v;
     }
+    if (threadIdx.x == 0) tv[2 * j] = vpy;
   }
   __syncthreads();
   const double tk = tau[c];
@@ -253,7 +279,14 @@ __global__ void __launch_bounds__(kColRows) k_lt_cols(int n, const double* __res
   const double* vj = V + (int64_t)j * n;
   // loads of 8 panel columns issued before their products
   auto s_of = [&](int r) {
-    double s = py[r];
+    // the product's row: its chunks' partial sums in order (loads issued together)
+    double pc[kLargeChunks];
+#pragma unroll
+    for (int cb = 0; cb < kLargeChunks; ++cb)
+      pc[cb] = cb < nch ? pyp[(int64_t)cb * n + r] : 0.0;
+    double s = pc[0];
+#pragma unroll
+    for (int cb = 1; cb < kLargeChunks; ++cb) s += pc[cb];
     for (int q0 = 0; q0 < j; q0 += 8) {
       double a[8], b[8];
 #pragma unroll
@@ -488,10 +521,13 @@ void launch_tridiag_large(const BatchDev& b, hipStream_t s, int m, const MolDesc
     for (int j = 0; j < nb; ++j) {
       const int c = k0 + j;
       const int rows = n - c - 1;
-      k_lt_prod<<<(rows + 4 * kProdWaves - 1) / (4 * kProdWaves), 64 * kProdWaves, 0, s>>>(
-          n, L.A, L.V, L.W, L.xv, L.part2, L.part, L.py, d, e, tau, c, j, act, st, m);
+      const dim3 pg((rows + 4 * kProdWaves - 1) / (4 * kProdWaves),
+                    (rows + kLargeChunkCols - 1) / kLargeChunkCols);
+      k_lt_prod<<<pg, 64 * kProdWaves, 0, s>>>(n, L.A, L.V, L.W, L.xv, L.part2, L.part, L.pyp,
+                                               L.partvp, d, e, tau, c, j, act, st, m);
       k_lt_cols<<<(rows + kColRows - 1) / kColRows, kColRows, 0, s>>>(
-          n, L.A, L.V, L.W, L.py, L.part, tau, L.xv, L.part2, c, j, j + 1 < nb, act, st, m);
+          n, L.A, L.V, L.W, L.pyp, L.part, L.partvp, tau, L.xv, L.part2, c, j, j + 1 < nb, act,
+          st, m);
     }
     const int r = k0 + nb;
     const int TT = (n - r + 15) >> 4;
