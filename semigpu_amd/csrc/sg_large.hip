@@ -84,12 +84,14 @@ __global__ void __launch_bounds__(kColRows) k_lt_x0(int n, const double* __restr
 }
 
 // Reflector of column c (j-th of its panel) from x and the partial sums of
-// x^2, then the product py = A v over rows i > c (4 rows per wave, v formed
-// on the fly from x), workgroup (x, y) over rows c + 1 + 32 x .. and columns
-// c + 1 + kLargeChunkCols y ..: its rows' partial sums into pyp[y] and their
-// dot with v into partvp; the workgroups of chunk 0 also leave the partial
-// dots W_q.v and V_q.v (q < j) of their rows and write v into V_j and into
-// the upper part of row c of A; d, e, tau of column c.
+// x^2, then the product py = A v over rows i > c (4 rows per wave),
+// workgroup (x, y) over rows c + 1 + 32 x .. and columns c + 1 +
+// kLargeChunkCols y ..: its rows' partial sums into pyp[y] and their dot with
+// v into partvp; the workgroups of chunk 0 also leave the partial dots W_q.v
+// and V_q.v (q < j) of their rows and write v into V_j and into the upper
+// part of row c of A; d, e, tau of column c.  v = sc x past row c + 1 and
+// 1 at c + 1, so the product and the dots run on x and take the scale sc at
+// the end: their loads do not wait for the norm's partial sums.
 __global__ void __launch_bounds__(64 * kProdWaves) k_lt_prod(int n, double* __restrict__ A,
                                                               double* __restrict__ V,
                                                               const double* __restrict__ W,
@@ -109,30 +111,36 @@ __global__ void __launch_bounds__(64 * kProdWaves) k_lt_prod(int n, double* __re
   __shared__ double red[kProdWaves];
   __shared__ double rows_py[R], rows_v[R];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // reflector scalars (every workgroup computes the same numbers)
+  const int cb = blockIdx.y;              // column chunk
+  const int r0 = c + 1 + blockIdx.x * R;  // first row of this workgroup
+  const int i0 = r0 + wave * 4;
+  // x past row c + 1 (0 at c + 1 and past the matrix)
+  auto xof = [&](int l) { return l > c + 1 && l < n ? xv[l] : 0.0; };
+  // the panel dots over this workgroup's rows (chunk 0): with x, and the
+  // entry at row c + 1 apart
+  const int nd = cb == 0 ? 2 * j : 0;
+  double pdx = 0.0, pd1 = 0.0;
+  if (threadIdx.x < nd) {
+    const int t = threadIdx.x;
+    const double* src = t < j ? W + (int64_t)t * n : V + (int64_t)(t - j) * n;
+    for (int b0 = 0; b0 < R; b0 += 16) {
+      double x[16], y[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = r0 + b0 + r;
+        x[r] = src[min(i, n - 1)];
+        y[r] = xof(i);
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) pdx += x[r] * y[r];
+      if (b0 == 0 && r0 == c + 1) pd1 = x[0];
+    }
+  }
+  // the norm's partial sums (loaded now, added after the product)
   const int nG = (n - c + kColRows - 1) / kColRows;
   double s2 = 0.0;
   for (int g = threadIdx.x; g < nG; g += blockDim.x) s2 += part2[g];
-  s2 = block_total(s2, red);
-  const double alpha = xv[c + 1];
-  const bool zero = s2 == 0.0;
-  double beta = alpha, tk = 0.0, sc = 0.0;
-  if (!zero) {
-    beta = -copysign(sqrt(alpha * alpha + s2), alpha);
-    tk = (beta - alpha) / beta;
-    sc = 1.0 / (alpha - beta);
-  }
-  const int cb = blockIdx.y;  // column chunk
-  if (blockIdx.x == 0 && cb == 0 && threadIdx.x == 0) {
-    d[c] = xv[c];
-    e[c] = beta;
-    tau[c] = tk;
-  }
-  const double one = zero ? 0.0 : 1.0;
-  auto vof = [&](int l) { return l == c + 1 ? one : xv[l] * sc; };
-  const int r0 = c + 1 + blockIdx.x * R;  // first row of this workgroup
-  const int i0 = r0 + wave * 4;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  double acc[4] = {0.0, 0.0, 0.0, 0.0}, acc1[4] = {0.0, 0.0, 0.0, 0.0};
   const double* ar[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) ar[r] = A + (int64_t)min(i0 + r, n - 1) * n;
@@ -142,7 +150,7 @@ __global__ void __launch_bounds__(64 * kProdWaves) k_lt_prod(int n, double* __re
     // chunk's last column and weighted 0 past it
     for (int l = lbeg + lane; l < lend; l += 128) {
       const int l2 = l + 64 < lend ? l + 64 : lend - 1;
-      const double v0 = vof(l), v1 = l + 64 < lend ? vof(l2) : 0.0;
+      const double x0 = xof(l), x1 = l + 64 < lend ? xof(l2) : 0.0;
       double a0[4], a1[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -150,9 +158,32 @@ __global__ void __launch_bounds__(64 * kProdWaves) k_lt_prod(int n, double* __re
         a1[r] = ar[r][l2];
       }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] += a0[r] * v0 + a1[r] * v1;
+      for (int r = 0; r < 4; ++r) acc[r] += a0[r] * x0 + a1[r] * x1;
+      // column c + 1 (the first of chunk 0, lane 0's first load) takes weight 1
+      if (l == c + 1) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc1[r] = a0[r];
+      }
     }
   }
+  // reflector scalars (every workgroup computes the same numbers)
+  s2 = block_total(s2, red);
+  const double alpha = xv[c + 1];
+  const bool zero = s2 == 0.0;
+  double beta = alpha, tk = 0.0, sc = 0.0;
+  if (!zero) {
+    beta = -copysign(sqrt(alpha * alpha + s2), alpha);
+    tk = (beta - alpha) / beta;
+    sc = 1.0 / (alpha - beta);
+  }
+  if (blockIdx.x == 0 && cb == 0 && threadIdx.x == 0) {
+    d[c] = xv[c];
+    e[c] = beta;
+    tau[c] = tk;
+  }
+  const double one = zero ? 0.0 : 1.0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) acc[r] = sc * acc[r] + one * acc1[r];
   // lanes < 32 keep rows 0,1 / >= 32 rows 2,3; then 16-lane halves
   double v2[2], v1;
   {
@@ -176,33 +207,20 @@ __global__ void __launch_bounds__(64 * kProdWaves) k_lt_prod(int n, double* __re
   }
   if (threadIdx.x < R) {
     const int i = r0 + threadIdx.x;
-    const double v = i < n ? vof(i) : 0.0;
+    const double v = i >= n ? 0.0 : (i == c + 1 ? one : xv[i] * sc);
     rows_v[threadIdx.x] = v;
     if (i < n && cb == 0) {
       V[(int64_t)j * n + i] = v;
       A[(int64_t)c * n + i] = v;
     }
   }
+  if (threadIdx.x < nd) part[(int64_t)blockIdx.x * kPartStride + threadIdx.x] = sc * pdx + one * pd1;
   __syncthreads();
-  // partial dots over this workgroup's rows: v.py (of this chunk's columns)
-  // and, in chunk 0, [0, j) W_q.v and [j, 2j) V_q.v
-  const int nd = cb == 0 ? 2 * j + 1 : 1;
-  if (threadIdx.x < nd) {
-    const int t = cb == 0 ? threadIdx.x : 2 * j;
-    const double* src = t < j ? W + (int64_t)t * n : V + (int64_t)(t - j) * n;
+  // v.py over this workgroup's rows (of this chunk's columns)
+  if (threadIdx.x == 0) {
     double sacc = 0.0;
-    for (int b0 = 0; b0 < R; b0 += 16) {
-      double x[16];
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int i = min(r0 + b0 + r, n - 1);
-        x[r] = t == 2 * j ? rows_py[b0 + r] : src[i];
-      }
-#pragma unroll
-      for (int r = 0; r < 16; ++r) sacc += x[r] * rows_v[b0 + r];  // rows_v is 0 past the matrix
-    }
-    if (t == 2 * j) partvp[(int64_t)blockIdx.x * gridDim.y + cb] = sacc;
-    else part[(int64_t)blockIdx.x * kPartStride + t] = sacc;
+    for (int r = 0; r < R; ++r) sacc += rows_py[r] * rows_v[r];  // rows_v is 0 past the matrix
+    partvp[(int64_t)blockIdx.x * gridDim.y + cb] = sacc;
   }
 }
 
@@ -227,6 +245,36 @@ __global__ void __launch_bounds__(kColRows) k_lt_cols(int n, const double* __res
   __shared__ double tv[2 * kLargeNb + 1];
   __shared__ double red[kColRows / 64];
   const int nch = (n - c - 1 + kLargeChunkCols - 1) / kLargeChunkCols;  // the product's chunks
+  const int i = c + 1 + blockIdx.x * kColRows + threadIdx.x, c1 = c + 1;
+  const bool in = i < n;
+  const int ir = in ? i : n - 1;  // loads stay inside the matrix
+  // First the loads that do not need the product's dots: the product's rows
+  // i and c + 1 (their chunks' partial sums) and, when another column
+  // follows, row c + 1 of A at i less the panel's corrections.
+  double pi[kLargeChunks], p1[kLargeChunks];
+#pragma unroll
+  for (int cb = 0; cb < kLargeChunks; ++cb) {
+    pi[cb] = cb < nch ? pyp[(int64_t)cb * n + ir] : 0.0;
+    p1[cb] = cb < nch ? pyp[(int64_t)cb * n + c1] : 0.0;
+  }
+  double xa = 0.0;
+  if (next) {
+    xa = A[(int64_t)c1 * n + ir];
+    for (int q0 = 0; q0 < j; q0 += 8) {
+      double a[8], b[8], ac[8], bc[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int q = min(q0 + u, j - 1);
+        a[u] = V[(int64_t)q * n + ir];
+        b[u] = W[(int64_t)q * n + ir];
+        ac[u] = V[(int64_t)q * n + c1];
+        bc[u] = W[(int64_t)q * n + c1];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (q0 + u < j) xa -= a[u] * bc[u] + b[u] * ac[u];
+    }
+  }
   {
     // the product's partial dots, summed in workgroup order: thread T takes
     // dot T % 64 (< 2 j) over the row blocks g = T / 64, T / 64 + 4, ...; the
@@ -277,57 +325,41 @@ __global__ void __launch_bounds__(kColRows) k_lt_cols(int n, const double* __res
   for (int q = 0; q < j; ++q) cross += tv[q] * tv[j + q];
   const double al = -0.5 * tk * (tk * (tv[2 * j] - 2.0 * cross));
   const double* vj = V + (int64_t)j * n;
-  // loads of 8 panel columns issued before their products
-  auto s_of = [&](int r) {
-    // the product's row: its chunks' partial sums in order (loads issued together)
-    double pc[kLargeChunks];
+  // s = tau (py - V tv1 - W tv2) at rows i and c + 1 in one pass (loads of 8
+  // panel columns issued before their products)
+  double si = pi[0], s1 = p1[0];
 #pragma unroll
-    for (int cb = 0; cb < kLargeChunks; ++cb)
-      pc[cb] = cb < nch ? pyp[(int64_t)cb * n + r] : 0.0;
-    double s = pc[0];
+  for (int cb = 1; cb < kLargeChunks; ++cb) {
+    si += pi[cb];
+    s1 += p1[cb];
+  }
+  for (int q0 = 0; q0 < j; q0 += 8) {
+    double a[8], b[8], ac[8], bc[8];
 #pragma unroll
-    for (int cb = 1; cb < kLargeChunks; ++cb) s += pc[cb];
-    for (int q0 = 0; q0 < j; q0 += 8) {
-      double a[8], b[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int q = min(q0 + u, j - 1);
-        a[u] = V[(int64_t)q * n + r];
-        b[u] = W[(int64_t)q * n + r];
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (q0 + u < j) s -= a[u] * tv[q0 + u] + b[u] * tv[j + q0 + u];
+    for (int u = 0; u < 8; ++u) {
+      const int q = min(q0 + u, j - 1);
+      a[u] = V[(int64_t)q * n + ir];
+      b[u] = W[(int64_t)q * n + ir];
+      ac[u] = V[(int64_t)q * n + c1];
+      bc[u] = W[(int64_t)q * n + c1];
     }
-    return tk * s;
-  };
-  const int i = c + 1 + blockIdx.x * kColRows + threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (q0 + u < j) {
+        si -= a[u] * tv[q0 + u] + b[u] * tv[j + q0 + u];
+        s1 -= ac[u] * tv[q0 + u] + bc[u] * tv[j + q0 + u];
+      }
+  }
   double w = 0.0;
-  if (i < n) {
-    w = s_of(i) + al * vj[i];
+  if (in) {
+    w = tk * si + al * vj[i];
     W[(int64_t)j * n + i] = w;
   }
   if (!next) return;
-  const int c1 = c + 1;
-  const double wc1 = s_of(c1) + al * vj[c1];
+  const double wc1 = tk * s1 + al * vj[c1];
   double sq = 0.0;
-  if (i < n) {
-    double x = A[(int64_t)c1 * n + i];
-    for (int q0 = 0; q0 < j; q0 += 8) {
-      double a[8], b[8], ac[8], bc[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int q = min(q0 + u, j - 1);
-        a[u] = V[(int64_t)q * n + i];
-        b[u] = W[(int64_t)q * n + i];
-        ac[u] = V[(int64_t)q * n + c1];
-        bc[u] = W[(int64_t)q * n + c1];
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (q0 + u < j) x -= a[u] * bc[u] + b[u] * ac[u];
-    }
-    x -= vj[i] * wc1 + w * vj[c1];
+  if (in) {
+    const double x = xa - (vj[i] * wc1 + w * vj[c1]);
     xv[i] = x;
     if (i >= c1 + 2) sq = x * x;
   }
