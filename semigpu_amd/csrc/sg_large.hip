@@ -147,7 +147,8 @@ __global__ void __launch_bounds__(64 * kProdWaves) k_lt_prod(int n, double* __re
   const int lbeg = c + 1 + cb * kLargeChunkCols, lend = min(n, lbeg + kLargeChunkCols);
   if (i0 < n) {
     // two 64-column tiles per batch of loads; the second is clamped to the
-    // chunk's last column and weighted 0 past it
+    // chunk's last column and weighted 0 past it; two batches in flight
+#pragma unroll 2
     for (int l = lbeg + lane; l < lend; l += 128) {
       const int l2 = l + 64 < lend ? l + 64 : lend - 1;
       const double x0 = xof(l), x1 = l + 64 < lend ? xof(l2) : 0.0;
@@ -216,11 +217,11 @@ __global__ void __launch_bounds__(64 * kProdWaves) k_lt_prod(int n, double* __re
   }
   if (threadIdx.x < nd) part[(int64_t)blockIdx.x * kPartStride + threadIdx.x] = sc * pdx + one * pd1;
   __syncthreads();
-  // v.py over this workgroup's rows (of this chunk's columns)
-  if (threadIdx.x == 0) {
-    double sacc = 0.0;
-    for (int r = 0; r < R; ++r) sacc += rows_py[r] * rows_v[r];  // rows_v is 0 past the matrix
-    partvp[(int64_t)blockIdx.x * gridDim.y + cb] = sacc;
+  // v.py over this workgroup's rows (of this chunk's columns), by wave 0
+  if (wave == 0) {
+    const double pr = lane < R ? rows_py[lane] * rows_v[lane] : 0.0;  // rows_v is 0 past the matrix
+    const double sacc = wred(pr);
+    if (lane == 0) partvp[(int64_t)blockIdx.x * gridDim.y + cb] = sacc;
   }
 }
 
@@ -471,27 +472,48 @@ __global__ void __launch_bounds__(256) k_bt_tfactor(int n, int nref, const doubl
 // Y (n x k, row-major) <- Q Y for the 16-column slice of this workgroup,
 // groups from last to first: W1 = V Y (waves split the rows, partial tiles
 // summed in order), W2 = T W1, Y -= V^T W2 (row tiles split over the waves).
-__global__ void __launch_bounds__(256) k_bt_large(int n, int nref, int k,
-                                                   const double* __restrict__ A,
-                                                   const double* __restrict__ Tall,
-                                                   double* __restrict__ Y,
-                                                   const int* __restrict__ active,
-                                                   const int* __restrict__ status, int m) {
+// kBtWaves waves; each keeps kBtSteps k-steps of W1's loads and kBtTiles
+// update tiles' loads in flight (addresses clamped inside the arrays,
+// unused values zeroed after the load; the products run in the order of a
+// load-per-step loop).
+constexpr int kBtWaves = 8, kBtSteps = 8, kBtTiles = 2;
+__global__ void __launch_bounds__(64 * kBtWaves) k_bt_large(int n, int nref, int k,
+                                                             const double* __restrict__ A,
+                                                             const double* __restrict__ Tall,
+                                                             double* __restrict__ Y,
+                                                             const int* __restrict__ active,
+                                                             const int* __restrict__ status,
+                                                             int m) {
   if (!mol_live(active, status, m)) return;
-  __shared__ double part[4][256];
+  __shared__ double part[kBtWaves][256];
   __shared__ double W2[256];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int lr = lane & 15, lk = lane >> 4;
   const int j = 16 * blockIdx.x + lr;
+  const bool jok = j < k;
+  const int jc = jok ? j : k - 1;
   const int ng = (nref + 15) / 16;
   for (int g = ng - 1; g >= 0; --g) {
     const int k0 = 16 * g, b = min(16, nref - k0);
+    const int c = k0 + lr, cc = min(c, n - 1);
     v4d w1 = {0, 0, 0, 0};
-    for (int i0 = k0 + 1 + 4 * wave; i0 < n; i0 += 16) {
-      const int i = i0 + lk, c = k0 + lr;
-      const double a = (lr < b && i < n && i > c) ? A[(int64_t)c * n + i] : 0.0;
-      const double y = (i < n && j < k) ? Y[(int64_t)i * k + j] : 0.0;
-      w1 = mfma(a, y, w1);
+    for (int i0 = k0 + 1 + 4 * wave; i0 < n; i0 += 4 * kBtWaves * kBtSteps) {
+      double a[kBtSteps], y[kBtSteps];
+#pragma unroll
+      for (int u = 0; u < kBtSteps; ++u) {
+        const int i = i0 + 4 * kBtWaves * u + lk, ic = min(i, n - 1);
+        a[u] = A[(int64_t)cc * n + ic];
+        y[u] = Y[(int64_t)ic * k + jc];
+      }
+#pragma unroll
+      for (int u = 0; u < kBtSteps; ++u) {
+        const int i = i0 + 4 * kBtWaves * u + lk;
+        if (i0 + 4 * kBtWaves * u < n) {
+          const double av = (lr < b && i < n && i > c) ? a[u] : 0.0;
+          const double yv = (i < n && jok) ? y[u] : 0.0;
+          w1 = mfma(av, yv, w1);
+        }
+      }
     }
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) part[wave][(4 * rr + lk) * 16 + lr] = w1[rr];
@@ -501,7 +523,10 @@ __global__ void __launch_bounds__(256) k_bt_large(int n, int nref, int k,
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
         const int idx = (4 * rr + lk) * 16 + lr;
-        s[rr] = ((part[0][idx] + part[1][idx]) + part[2][idx]) + part[3][idx];
+        double t = part[0][idx];
+#pragma unroll
+        for (int w = 1; w < kBtWaves; ++w) t += part[w][idx];
+        s[rr] = t;
       }
       const double* T = Tall + (int64_t)g * 256;
       v4d w2 = {0, 0, 0, 0};
@@ -512,24 +537,47 @@ __global__ void __launch_bounds__(256) k_bt_large(int n, int nref, int k,
     }
     __syncthreads();
     const int IT = (n - (k0 + 1) + 15) >> 4;
-    for (int it = wave; it < IT; it += 4) {
-      const int I0 = k0 + 1 + 16 * it;
-      v4d cur;
+    double wq[4];
 #pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        const int row = I0 + 4 * rr + lk;
-        cur[rr] = (row < n && j < k) ? Y[(int64_t)row * k + j] : 0.0;
+    for (int st = 0; st < 4; ++st) wq[st] = W2[(4 * st + lk) * 16 + lr];
+    for (int it0 = wave; it0 < IT; it0 += kBtWaves * kBtTiles) {
+      v4d cur[kBtTiles];
+      double av[kBtTiles][4];
+#pragma unroll
+      for (int u = 0; u < kBtTiles; ++u) {
+        const int I0 = k0 + 1 + 16 * (it0 + kBtWaves * u);
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int row = min(I0 + 4 * rr + lk, n - 1);
+          cur[u][rr] = Y[(int64_t)row * k + jc];
+        }
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+          const int q = 4 * st + lk, i = min(I0 + lr, n - 1);
+          av[u][st] = A[(int64_t)min(k0 + q, n - 1) * n + i];
+        }
       }
 #pragma unroll
-      for (int st = 0; st < 4; ++st) {
-        const int q = 4 * st + lk, i = I0 + lr, c = k0 + q;
-        const double a = (q < b && i < n && i > c) ? -A[(int64_t)c * n + i] : 0.0;
-        cur = mfma(a, W2[q * 16 + lr], cur);
-      }
+      for (int u = 0; u < kBtTiles; ++u) {
+        const int it = it0 + kBtWaves * u;
+        if (it >= IT) break;
+        const int I0 = k0 + 1 + 16 * it;
 #pragma unroll
-      for (int rr = 0; rr < 4; ++rr) {
-        const int row = I0 + 4 * rr + lk;
-        if (row < n && j < k) Y[(int64_t)row * k + j] = cur[rr];
+        for (int rr = 0; rr < 4; ++rr) {
+          const int row = I0 + 4 * rr + lk;
+          if (!(row < n && jok)) cur[u][rr] = 0.0;
+        }
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+          const int q = 4 * st + lk, i = I0 + lr, cq = k0 + q;
+          const double a = (q < b && i < n && i > cq) ? -av[u][st] : 0.0;
+          cur[u] = mfma(a, wq[st], cur[u]);
+        }
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int row = I0 + 4 * rr + lk;
+          if (row < n && jok) Y[(int64_t)row * k + j] = cur[u][rr];
+        }
       }
     }
     __syncthreads();
@@ -579,7 +627,7 @@ void launch_back_large(const BatchDev& b, hipStream_t s, int m, const MolDesc& M
   LargeWork L(b.W, M);
   const int ng = (nref + 15) / 16;
   k_bt_tfactor<<<ng, 256, 0, s>>>(n, nref, L.A, b.tau + M.vec0, L.T, act, b.status, m);
-  k_bt_large<<<(k + 15) / 16, 256, 0, s>>>(n, nref, k, L.A, L.T, L.Y, act, b.status, m);
+  k_bt_large<<<(k + 15) / 16, 64 * kBtWaves, 0, s>>>(n, nref, k, L.A, L.T, L.Y, act, b.status, m);
 }
 
 }  // namespace sg
