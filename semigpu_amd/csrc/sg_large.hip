@@ -299,15 +299,16 @@ __global__ void __launch_bounds__(kColRows) k_lt_cols(int n, const double* __res
     }
     double sacc = 0.0;
     if (t < nd) {
-      for (int g0 = k; g0 < G; g0 += 4 * (kColRows / 64)) {
-        double x[4];
+      constexpr int U = 12;  // loads in flight per thread
+      for (int g0 = k; g0 < G; g0 += U * (kColRows / 64)) {
+        double x[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < U; ++u) {
           const int g = g0 + u * (kColRows / 64);
           x[u] = g < G ? part[(int64_t)g * kPartStride + t] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) sacc += x[u];
+        for (int u = 0; u < U; ++u) sacc += x[u];
       }
     }
     sl[k][t] = sacc;
