@@ -805,6 +805,129 @@ __global__ void __launch_bounds__(256) k_bisect(const MolDesc* __restrict__ mols
     }
 }
 
+// Multisection for the molecules of the multi-workgroup eigensolver: one
+// eigenvalue per group of kMsLanes lanes (a few molecules with N up to
+// kMaxLarge leave k_bisect's two eigenvalues per lane on a few dozen
+// wavefronts).  The previous step's brackets (as in k_bisect) are tried all at
+// once, one endpoint per lane, and the tightest confirmed one kept; then
+// every step counts at kMsLanes interior points x_l = a + (b - a)(l + 1) /
+// (kMsLanes + 1) and keeps the subinterval where the count passes the index
+// (log2(kMsLanes + 1) bits per step), to the same width as k_bisect.  Every
+// lane forms every x_l alike, so the result does not depend on the launch.
+constexpr int kMsLanes = 32;
+__global__ void __launch_bounds__(256) k_bisect_multi(const MolDesc* __restrict__ mols,
+                                                       const double* __restrict__ dA,
+                                                       const double* __restrict__ eA,
+                                                       double* __restrict__ lamA,
+                                                       double* __restrict__ tnormA, int want_all,
+                                                       const int* __restrict__ active,
+                                                       const int* __restrict__ status,
+                                                       const int* __restrict__ order,
+                                                       const int* __restrict__ steps,
+                                                       double* __restrict__ dlamA) {
+  extern __shared__ double lds[];
+  const int m = order ? order[blockIdx.y] : blockIdx.y;
+  if (!mol_live(active, status, m)) return;
+  const MolDesc M = mols[m];
+  const int n = M.n_ao;
+  int lo, k;
+  selection(M, want_all, lo, k);
+  constexpr int GPB = 256 / kMsLanes;  // eigenvalues per workgroup
+  if ((int)blockIdx.x * GPB >= k) return;
+  double* d = lds;
+  double* e2 = d + n;
+  double* red = e2 + n;
+  const double* dg = dA + M.vec0;
+  const double* eg = eA + M.vec0;
+  double glo = 1e300, ghi = -1e300, emax = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double di = dg[i];
+    const double el = i > 0 ? fabs(eg[i - 1]) : 0.0, er = i < n - 1 ? fabs(eg[i]) : 0.0;
+    d[i] = di;
+    e2[i] = i < n - 1 ? eg[i] * eg[i] : 0.0;
+    glo = fmin(glo, di - el - er);
+    ghi = fmax(ghi, di + el + er);
+    emax = fmax(emax, e2[i]);
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    glo = fmin(glo, __shfl_xor(glo, off, 64));
+    ghi = fmax(ghi, __shfl_xor(ghi, off, 64));
+    emax = fmax(emax, __shfl_xor(emax, off, 64));
+  }
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[3 * w] = glo;
+    red[3 * w + 1] = ghi;
+    red[3 * w + 2] = emax;
+  }
+  __syncthreads();
+  glo = red[0];
+  ghi = red[1];
+  emax = red[2];
+  for (int i = 1; i < nw; ++i) {
+    glo = fmin(glo, red[3 * i]);
+    ghi = fmax(ghi, red[3 * i + 1]);
+    emax = fmax(emax, red[3 * i + 2]);
+  }
+  const double tnorm = fmax(fabs(glo), fabs(ghi));
+  const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax);
+  glo -= 2.0 * kEps * tnorm + 2.0 * pivmin;
+  ghi += 2.0 * kEps * tnorm + 2.0 * pivmin;
+  if (blockIdx.x == 0 && threadIdx.x == 0) tnormA[m] = tnorm;
+  const int sl = threadIdx.x % kMsLanes;
+  const int j = blockIdx.x * GPB + (int)threadIdx.x / kMsLanes;
+  if (j >= k) return;
+  const int idx = lo + j;
+  double a = glo, b = ghi, prev = 0.0;
+  const bool warm = steps && steps[m] > 0;
+  if (warm) {
+    // lanes 2 lv and 2 lv + 1 count at prev -+ del of bracket level lv
+    prev = lamA[M.vec0 + j];
+    const bool have = dlamA && steps[m] >= 2;
+    const double dprev = have ? fmax(8.0 * dlamA[M.vec0 + j], 1e-12 * tnorm) : 0.0;
+    const int lv = sl >> 1;
+    const double del = lv == 0 ? dprev : (lv == 1 ? 1e-7 : 1e-4) * tnorm;
+    const bool ok = lv < 3 && (lv > 0 || have) && (lv == 0 || del > dprev) &&
+                    prev - del > glo && prev + del < ghi;
+    const double x = (sl & 1) ? prev + del : prev - del;
+    const int c = ok ? sturm(d, e2, n, x, pivmin) : 0;
+    // the first level (in order) whose two counts confirm it
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      const int c1 = __shfl(c, 2 * l, kMsLanes), c2 = __shfl(c, 2 * l + 1, kMsLanes);
+      const int okl = __shfl((int)ok, 2 * l, kMsLanes);
+      if (okl && c1 <= idx && c2 > idx) {
+        const double dl = l == 0 ? dprev : (l == 1 ? 1e-7 : 1e-4) * tnorm;
+        a = prev - dl;
+        b = prev + dl;
+        break;
+      }
+    }
+  }
+  const double atol = pivmin + 2.0 * kEps * tnorm;
+  for (int it = 0; it < 64; ++it) {
+    if (b - a <= 2.0 * kEps * fmax(fabs(a), fabs(b)) + atol) break;
+    const double h = (b - a) / (kMsLanes + 1);
+    const double x = a + h * (sl + 1);
+    const int c = sturm(d, e2, n, x, pivmin);
+    // first lane whose count passes idx (kMsLanes: none)
+    int first = kMsLanes;
+#pragma unroll 1
+    for (int l = kMsLanes - 1; l >= 0; --l)
+      if (__shfl(c, l, kMsLanes) > idx) first = l;
+    const double na = first == 0 ? a : a + h * first;
+    const double nb = first == kMsLanes ? b : a + h * (first + 1);
+    a = na;
+    b = nb;
+  }
+  if (sl == 0) {
+    const double lam = 0.5 * (a + b);
+    lamA[M.vec0 + j] = lam;
+    if (dlamA) dlamA[M.vec0 + j] = warm ? fabs(lam - prev) : 1e300;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic start vector entries in [-1, 1).
 __device__ __forceinline__ double start_entry(int vec, int row) {
@@ -1854,7 +1977,14 @@ void launch_eigh_bisect(const BatchDev& b, hipStream_t s, bool all) {
   const int* prec = (all || !b.mixed) ? nullptr : b.prec;
   const double red_tol = b.eig32 ? kFp32BisectTol : kReducedBisectTol;
   if (p.count > 0) bisect_part(b, s, all, p.order, p.count, p.max_n, prec, red_tol);
-  if (b.n_large > 0) bisect_part(b, s, all, b.order, b.n_large, b.max_n, nullptr, red_tol);
+  if (b.n_large > 0) {
+    const int N = b.max_n, kmax = all ? N : N / 2;
+    constexpr int GPB = 256 / kMsLanes;
+    dim3 gb(std::max(1, (kmax + GPB - 1) / GPB), b.n_large);
+    k_bisect_multi<<<gb, 256, sizeof(double) * (2 * N + 32), s>>>(
+        b.mols, b.dT, b.eT, b.eig, b.tnorm, all ? 1 : 0, all ? nullptr : b.active, b.status, b.order,
+        all ? nullptr : b.steps, all ? nullptr : b.dlam);
+  }
 }
 
 void launch_eigh_invit(const BatchDev& b, hipStream_t s, bool all) {
