@@ -390,18 +390,35 @@ __global__ void __launch_bounds__(256) k_lt_update(int n, double* __restrict__ A
     acc[rr] = (row < n && col < n) ? A[(int64_t)row * n + col] : 0.0;
   }
   const int ra = min(I0 + lr, n - 1), cb = min(J0 + lr, n - 1);
-  const int nsteps = (2 * nb + 3) >> 2;
-  for (int st = 0; st < nsteps; ++st) {
-    const int k = 4 * st + lk;
-    double av = 0.0, bv = 0.0;
-    if (k < nb) {
-      av = -V[(int64_t)k * n + ra];
-      bv = W[(int64_t)k * n + cb];
-    } else if (k < 2 * nb) {
-      av = -W[(int64_t)(k - nb) * n + ra];
-      bv = V[(int64_t)(k - nb) * n + cb];
+  if (nb == kLargeNb) {
+    // a full panel: the 16 k-steps' operands loaded first (V W^T in steps
+    // 0..7, W V^T in 8..15), then the products in the same order
+    constexpr int S = kLargeNb / 2;
+    double av[S], bv[S];
+#pragma unroll
+    for (int st = 0; st < S; ++st) {
+      const int k = 4 * st + lk, kk = st < S / 2 ? k : k - kLargeNb;
+      const double* pa = st < S / 2 ? V : W;
+      const double* pb = st < S / 2 ? W : V;
+      av[st] = pa[(int64_t)kk * n + ra];
+      bv[st] = pb[(int64_t)kk * n + cb];
     }
-    acc = mfma(av, bv, acc);
+#pragma unroll
+    for (int st = 0; st < S; ++st) acc = mfma(-av[st], bv[st], acc);
+  } else {
+    const int nsteps = (2 * nb + 3) >> 2;
+    for (int st = 0; st < nsteps; ++st) {
+      const int k = 4 * st + lk;
+      double av = 0.0, bv = 0.0;
+      if (k < nb) {
+        av = -V[(int64_t)k * n + ra];
+        bv = W[(int64_t)k * n + cb];
+      } else if (k < 2 * nb) {
+        av = -W[(int64_t)(k - nb) * n + ra];
+        bv = V[(int64_t)(k - nb) * n + cb];
+      }
+      acc = mfma(av, bv, acc);
+    }
   }
 #pragma unroll
   for (int rr = 0; rr < 4; ++rr) {
