@@ -225,7 +225,7 @@ struct LargeWork {
     cstart = (int*)(sh + n);
     clist = cstart + n;
     part = T;                              // <= (N/32 + 1) x (2 kLargeNb + 1)
-    part2 = sh;                            // <= N/256 + 1
+    part2 = sh;                            // <= 4 (N/256 + 1): one per column-kernel wavefront
     xv = (double*)cstart;                  // N (cstart and clist: N + 1 doubles)
     pyp = V + kLargePyOff * n;             // after clist (n >= 128 here)
     partvp = pyp + kLargeChunks * n;

@@ -64,14 +64,13 @@ __device__ double block_total(double v, double* red) {
 }
 
 // Column c with no panel corrections (the first column of a panel): x = row
-// c of A from the diagonal on, and per-workgroup partial sums of x_i^2, i >= c+2.
+// c of A from the diagonal on, and per-wavefront partial sums of x_i^2, i >= c+2.
 __global__ void __launch_bounds__(kColRows) k_lt_x0(int n, const double* __restrict__ A,
                                                      double* __restrict__ xv,
                                                      double* __restrict__ part2, int c,
                                                      const int* __restrict__ active,
                                                      const int* __restrict__ status, int m) {
   if (!mol_live(active, status, m)) return;
-  __shared__ double red[kColRows / 64];
   const int i = c + blockIdx.x * kColRows + threadIdx.x;
   double sq = 0.0;
   if (i < n) {
@@ -79,8 +78,8 @@ __global__ void __launch_bounds__(kColRows) k_lt_x0(int n, const double* __restr
     xv[i] = v;
     if (i >= c + 2) sq = v * v;
   }
-  sq = block_total(sq, red);
-  if (threadIdx.x == 0) part2[blockIdx.x] = sq;
+  sq = wred(sq);
+  if ((threadIdx.x & 63) == 0) part2[blockIdx.x * (kColRows / 64) + (threadIdx.x >> 6)] = sq;
 }
 
 // Reflector of column c (j-th of its panel) from x and the partial sums of
@@ -136,8 +135,9 @@ __global__ void __launch_bounds__(64 * kProdWaves) k_lt_prod(int n, double* __re
       if (b0 == 0 && r0 == c + 1) pd1 = x[0];
     }
   }
-  // the norm's partial sums (loaded now, added after the product)
-  const int nG = (n - c + kColRows - 1) / kColRows;
+  // the norm's partial sums, one per wavefront of the column kernels (loaded
+  // now, added after the product)
+  const int nG = (n - c + kColRows - 1) / kColRows * (kColRows / 64);
   double s2 = 0.0;
   for (int g = threadIdx.x; g < nG; g += blockDim.x) s2 += part2[g];
   double acc[4] = {0.0, 0.0, 0.0, 0.0}, acc1[4] = {0.0, 0.0, 0.0, 0.0};
@@ -365,8 +365,8 @@ __global__ void __launch_bounds__(kColRows) k_lt_cols(int n, const double* __res
     xv[i] = x;
     if (i >= c1 + 2) sq = x * x;
   }
-  sq = block_total(sq, red);
-  if (threadIdx.x == 0) part2[blockIdx.x] = sq;
+  sq = wred(sq);
+  if ((threadIdx.x & 63) == 0) part2[blockIdx.x * (kColRows / 64) + (threadIdx.x >> 6)] = sq;
 }
 
 // A[r:, r:] -= V W^T + W V^T (both triangles), one 16 x 16 tile per wave.
